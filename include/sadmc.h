@@ -1,0 +1,195 @@
+/*
+ * sadmc -- MI355X (gfx950) walker engine for the sad-monte-carlo hot path.
+ *
+ * C ABI.  Plain pointers, sizes and status codes; no C++ or torch types.
+ * The reference's FFI-less Rust types this boundary stands in for are cited
+ * on each entry point; INTEGRATION.md shows the Rust binding.
+ *
+ * Model.  A context owns one HIP stream and any number of walker *sets*.  A
+ * set is a batch of independent walkers of one system kind and size (one
+ * reference `EnergyMC<S>` each, src/mc/energy.rs:167-210).  All walker state
+ * -- positions or spins, the RNG stream, the SAD/WL/SAMC method state and the
+ * per-walker energy bins -- lives in device memory for the life of the
+ * context.  `sadmc_run` advances every walker of every set by K moves, i.e.
+ * K calls of the reference's `EnergyMC::move_once` (energy.rs:904-974), in
+ * one launch.
+ *
+ * Threading: one host thread per context; calls are not re-entrant.
+ * Errors: every entry point returns SADMC_OK (0) or a negative code;
+ * sadmc_last_error() gives the message.  Walker-level failures that the
+ * reference turns into a panic (bin index out of range, energy drift
+ * beyond verify_energy's bound) or that are specific to the fixed device bin
+ * window (capacity) set that walker's `status`, stop that walker, and make
+ * sadmc_sync() return the matching code.
+ */
+#ifndef SADMC_H
+#define SADMC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SADMC_ABI_VERSION 1
+
+enum sadmc_status {
+  SADMC_OK = 0,
+  SADMC_E_ARG = -1,      /* bad argument */
+  SADMC_E_HIP = -2,      /* HIP runtime error */
+  SADMC_E_CAPACITY = -3, /* a walker's energy left its bin window */
+  SADMC_E_VERIFY = -4,   /* verify_energy bound exceeded (lj.rs:249-261) */
+  SADMC_E_INIT = -5,     /* initial configuration could not be built */
+  SADMC_E_STATE = -6,    /* call out of order */
+  SADMC_E_NODEVICE = -7, /* no usable gfx950 device */
+  SADMC_E_INDEX = -8     /* energy index outside the bins (reference panics) */
+};
+
+/* Walker `status` values (sadmc_walker_state.status). */
+enum { SADMC_WS_OK = 0, SADMC_WS_INDEX = 1, SADMC_WS_VERIFY = 2, SADMC_WS_INIT = 3, SADMC_WS_CAPACITY = 4 };
+
+/* src/system/any.rs:61-162 -- the systems the `histogram` binary drives. */
+enum sadmc_system { SADMC_ISING = 0, SADMC_LJ = 1, SADMC_WCA = 2 };
+/* src/mc/energy.rs:44-69 MethodParams. */
+enum sadmc_method { SADMC_SAD = 0, SADMC_SAMC = 1, SADMC_WL = 2, SADMC_INV_T_WL = 3, SADMC_CANONICAL = 4 };
+/* src/mc/energy.rs:73-78 MoveParams. */
+enum sadmc_move_plan { SADMC_TRANSLATION_SCALE = 0, SADMC_ACCEPTANCE_RATE = 1 };
+/* Pair-term precision of the LJ energy difference. */
+enum sadmc_precision { SADMC_FP64 = 0, SADMC_FP32_PAIRS = 1 };
+/* Displacement proposal for LJ/WCA. */
+enum sadmc_move_kind { SADMC_MOVE_CUBE = 0, SADMC_MOVE_LEARNED = 1 };
+
+/* Parameters shared by every walker of a set.  Mirrors the reference's
+ * EnergyMCParams (energy.rs:81-97) plus the system parameters
+ * (ising.rs:12-15, lj.rs:14-19, wca.rs:370-380). */
+typedef struct {
+  int32_t system;        /* sadmc_system */
+  int32_t n;             /* Ising lattice width N (N*N spins, N <= 8), or atom count */
+  int32_t method;        /* sadmc_method */
+  int32_t move_plan;     /* sadmc_move_plan */
+  double method_param;   /* SAD min_T | SAMC t0 | WL min_gamma | canonical T */
+  int32_t has_min_gamma; /* WL only: min_gamma given */
+  int32_t precision;     /* sadmc_precision (LJ only) */
+  double move_param;     /* translation scale (cube half-width) or target acceptance rate */
+  double energy_bin;     /* <= 0: system default (Ising 4, else 1) -- energy.rs:831-833 */
+  int32_t has_min_e;     /* --min-allowed-energy given */
+  int32_t has_max_e;     /* --max-allowed-energy given */
+  double min_e;
+  double max_e;
+  double rc2;            /* WCA cutoff squared (2^(1/3) sigma^2) */
+  int32_t refresh;       /* 0: reference error-estimate refresh; >0: FP64 refresh every `refresh` moves */
+  int32_t ising_init;    /* 0: reference spins (seed 10137, ising.rs:45); 1: per-walker seed */
+  int32_t collect_pressure; /* WCA: virial pressure every N^2 moves (wca.rs:202-218) */
+  int32_t lanes;         /* lanes per walker for LJ/WCA (power of two 2..64); 0 = auto */
+  double e_lo_bound;     /* bin window: lowest energy a walker may reach */
+  double e_hi_bound;     /* bin window: highest energy a walker may reach */
+  int32_t move_kind;     /* sadmc_move_kind */
+  int32_t reserved0;
+  double reserved[4];
+} sadmc_set_params;
+
+/* Device-resident per-walker state; `sadmc_download_raw` returns an array of
+ * these.  Field meanings follow EnergyMC (energy.rs:167-244). */
+typedef struct {
+  uint64_t rng[2];            /* xoroshiro128+ state */
+  double E;                   /* System::energy() */
+  double error;               /* LJ/WCA accumulated error estimate */
+  uint64_t moves, accepted_moves;
+  double acceptance_rate, translation_scale;
+  double bins_min, bins_width;
+  int64_t base;               /* physical slot of logical bin 0 */
+  int64_t len;                /* logical number of bins */
+  double max_S;
+  int64_t max_S_index;
+  uint64_t epoch;             /* round-trip visit epoch */
+  uint64_t bulk_visited;      /* value of have_visited for bins not marked this epoch */
+  int32_t method;             /* current method (1/t-WL switches to SAMC) */
+  int32_t status;             /* SADMC_WS_* */
+  double too_lo, too_hi, latest_parameter, min_T;
+  uint64_t tL, tF, num_states, highest_hist;
+  uint64_t tmax;              /* max t_found over [too_lo, too_hi] (tracked incrementally) */
+  double t0;                  /* SAMC */
+  double wl_gamma, wl_num_states, wl_min_energy;
+  uint64_t wl_lowest_hist, wl_highest_hist, wl_total_hist;
+  int64_t wl_len;
+  int32_t has_pressure;
+  int32_t reserved0;
+  double geom;                /* LJ sphere radius or WCA box edge */
+  int64_t cap;                /* bin capacity */
+  uint64_t reserved[3];
+} sadmc_walker_state;
+
+/* One energy bin (64 bytes).  The per-bin vectors of energy.rs:146-163 and
+ * 202-205, interleaved so that one walker's touch of one bin is one half
+ * cache line. */
+typedef struct {
+  double lnw, energy_total, energy_squared_total;
+  uint64_t histogram, t_found, round_trips, visit_epoch, wl_hist;
+} sadmc_bin;
+
+typedef struct sadmc_ctx sadmc_ctx;
+
+/* ---- context --------------------------------------------------------- */
+int sadmc_abi_version(void);
+/* Create a context on HIP device `device`.  Fails with SADMC_E_NODEVICE when
+ * no gfx950 device is present. */
+int sadmc_create(sadmc_ctx **out, int device);
+void sadmc_destroy(sadmc_ctx *ctx);
+const char *sadmc_last_error(const sadmc_ctx *ctx);
+
+/* ---- building walkers: MonteCarlo::from_params (energy.rs:830-898) ----
+ * seeds[w]: the --seed of walker w.  geom[w]: LJ sphere radius or WCA box
+ * edge (ignored for Ising; may be NULL then).  coords: optional initial
+ * positions, n_walkers x [x[n], y[n], z[n]]; NULL generates them with the
+ * documented init rule.  spins: optional n_walkers x n*n int8 (+1/-1). */
+int sadmc_add_set(sadmc_ctx *ctx, const sadmc_set_params *p, int32_t n_walkers, const uint64_t *seeds,
+                  const double *geom, const double *coords, const int8_t *spins, int32_t *set_id);
+/* Allocate device state, upload, and run from_params on the device (initial
+ * energy, optional greedy descent below max_e, one-bin histogram). */
+int sadmc_init(sadmc_ctx *ctx);
+
+/* ---- the hot path: K x move_once for every walker (energy.rs:904-974) - */
+/* Enqueue one launch advancing every walker of every set by `moves`. */
+int sadmc_run(sadmc_ctx *ctx, uint64_t moves);
+/* Wait for the stream; returns the first walker failure as an error code. */
+int sadmc_sync(sadmc_ctx *ctx);
+/* Duration of the last sadmc_run launch in ms (HIP events on the context
+ * stream around the kernel). */
+int sadmc_last_run_ms(const sadmc_ctx *ctx, double *ms);
+
+/* ---- single-walker pair for parity tests: MovableSystem::plan_move and
+ * ConfirmSystem::confirm (system/mod.rs:101-120).  plan_move consumes the
+ * walker's RNG exactly as inside move_once; *some = 0 means None. */
+int sadmc_plan_move(sadmc_ctx *ctx, int32_t set, int32_t walker, double scale, double *e_new, int32_t *some);
+int sadmc_confirm(sadmc_ctx *ctx, int32_t set, int32_t walker);
+
+/* ---- state: the serde view of EnergyMC (mc/mod.rs:111-120) ----------- */
+int sadmc_walker_count(const sadmc_ctx *ctx, int32_t set, int32_t *n_walkers, int64_t *bin_capacity);
+int sadmc_get_walker_state(sadmc_ctx *ctx, int32_t set, int32_t walker, sadmc_walker_state *out);
+/* Logical bins [0, len): any output pointer may be NULL; `cap` bounds them. */
+int sadmc_walker_bins(sadmc_ctx *ctx, int32_t set, int32_t walker, int64_t cap, double *lnw,
+                      uint64_t *histogram, uint64_t *t_found, double *energy_total,
+                      double *energy_squared_total, uint8_t *have_visited, uint64_t *round_trips,
+                      uint64_t *wl_hist, uint64_t *pressure_count, double *pressure_total);
+int sadmc_walker_coords(sadmc_ctx *ctx, int32_t set, int32_t walker, double *coords);
+int sadmc_walker_spins(sadmc_ctx *ctx, int32_t set, int32_t walker, int8_t *spins);
+/* Whole-set dumps: n_walkers states, n_walkers*cap bins. */
+int sadmc_download_raw(sadmc_ctx *ctx, int32_t set, sadmc_walker_state *states, sadmc_bin *bins);
+/* Resume (mc/mod.rs:70-84): overwrite a whole set's device state. */
+int sadmc_upload_raw(sadmc_ctx *ctx, int32_t set, const sadmc_walker_state *states, const sadmc_bin *bins,
+                     const double *coords, const int8_t *spins);
+
+/* ---- host-side helpers (no device needed) ----------------------------- */
+/* The initial configurations sadmc_add_set generates when coords/spins are
+ * NULL.  Returns SADMC_E_INIT if LJ placement fails. */
+int sadmc_init_coords(int32_t system, int32_t n, double geom, uint64_t seed, double *coords);
+int sadmc_init_spins(int32_t n, int32_t rule, uint64_t seed, int8_t *spins);
+/* Bin window (capacity, e_lo, e_hi) chosen for a parameter set when
+ * e_lo_bound == e_hi_bound. */
+int sadmc_bin_window(const sadmc_set_params *p, double geom, double *e_lo, double *e_hi, int64_t *cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,273 @@
+"""ctypes wrapper of the C oracle (liboracle.so) -- TEST INFRASTRUCTURE.
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg import
+this module, and only as the checker.  See sadmc_oracle.h for what each
+function restates from the reference.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+ISING, LJ, WCA, SCRIPTED = 0, 1, 2, 3
+SAD, SAMC, WL, INV_T_WL, CANONICAL = 0, 1, 2, 3, 4
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("system", C.c_int32), ("n", C.c_int32), ("method", C.c_int32), ("move_plan", C.c_int32),
+        ("method_param", C.c_double), ("has_min_gamma", C.c_int32), ("precision", C.c_int32),
+        ("move_param", C.c_double), ("energy_bin", C.c_double),
+        ("has_min_e", C.c_int32), ("has_max_e", C.c_int32), ("min_e", C.c_double), ("max_e", C.c_double),
+        ("radius", C.c_double), ("box", C.c_double), ("rc2", C.c_double),
+        ("refresh", C.c_int32), ("order_g", C.c_int32), ("ising_init", C.c_int32),
+        ("collect_pressure", C.c_int32), ("seed", C.c_uint64),
+    ]
+
+
+class Scalars(C.Structure):
+    _fields_ = [
+        ("E", C.c_double), ("error", C.c_double), ("moves", C.c_uint64), ("accepted", C.c_uint64),
+        ("acceptance_rate", C.c_double), ("translation_scale", C.c_double), ("rng", C.c_uint64 * 2),
+        ("bins_min", C.c_double), ("bins_width", C.c_double), ("len", C.c_int64),
+        ("max_S", C.c_double), ("max_S_index", C.c_int64), ("method", C.c_int32), ("status", C.c_int32),
+        ("too_lo", C.c_double), ("too_hi", C.c_double), ("lambda_", C.c_double), ("min_T", C.c_double),
+        ("tL", C.c_uint64), ("tF", C.c_uint64), ("num_states", C.c_uint64), ("highest_hist", C.c_uint64),
+        ("t0", C.c_double),
+        ("wl_gamma", C.c_double), ("wl_num_states", C.c_double), ("wl_min_energy", C.c_double),
+        ("wl_lowest", C.c_uint64), ("wl_highest", C.c_uint64), ("wl_total", C.c_uint64),
+        ("wl_len", C.c_int64), ("has_pressure", C.c_int32),
+    ]
+
+    def as_dict(self):
+        d = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            d[name] = list(v) if name == "rng" else v
+        return d
+
+
+def build():
+    """Compile liboracle.so from the C sources (make)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = os.path.join(_HERE, "liboracle.so")
+    if not os.path.exists(path):
+        build()
+    L = C.CDLL(path)
+    u64p = C.POINTER(C.c_uint64)
+    dp = C.POINTER(C.c_double)
+    L.or_splitmix64_next.argtypes = [u64p]; L.or_splitmix64_next.restype = C.c_uint64
+    L.or_rng_seed.argtypes = [C.c_uint64, u64p]
+    L.or_rng_next.argtypes = [u64p]; L.or_rng_next.restype = C.c_uint64
+    L.or_gen_range.argtypes = [u64p, C.c_uint64, C.c_uint64]; L.or_gen_range.restype = C.c_uint64
+    L.or_uniform_index.argtypes = [u64p, C.c_uint64]; L.or_uniform_index.restype = C.c_uint64
+    L.or_gen_f64.argtypes = [u64p]; L.or_gen_f64.restype = C.c_double
+    L.or_uniform_f64.argtypes = [u64p, C.c_double, C.c_double]; L.or_uniform_f64.restype = C.c_double
+    L.or_lj_pair.argtypes = [C.c_double]; L.or_lj_pair.restype = C.c_double
+    L.or_lj_pair_f32.argtypes = [C.c_float]; L.or_lj_pair_f32.restype = C.c_float
+    L.or_wca_pair.argtypes = [C.c_double, C.c_double]; L.or_wca_pair.restype = C.c_double
+    L.or_mc_new.argtypes = [C.POINTER(Params), C.c_void_p, C.c_void_p]; L.or_mc_new.restype = C.c_void_p
+    L.or_mc_new_scripted.argtypes = [C.POINTER(Params), C.c_double, dp, C.c_int64]
+    L.or_mc_new_scripted.restype = C.c_void_p
+    L.or_mc_free.argtypes = [C.c_void_p]
+    L.or_mc_run.argtypes = [C.c_void_p, C.c_uint64]; L.or_mc_run.restype = C.c_int
+    L.or_mc_scalars.argtypes = [C.c_void_p, C.POINTER(Scalars)]
+    L.or_mc_bins.argtypes = [C.c_void_p] + [C.c_void_p] * 10
+    L.or_mc_coords.argtypes = [C.c_void_p, C.c_void_p]
+    L.or_mc_spins.argtypes = [C.c_void_p, C.c_void_p]
+    L.or_mc_compute_energy.argtypes = [C.c_void_p]; L.or_mc_compute_energy.restype = C.c_double
+    L.or_mc_reference_energy.argtypes = [C.c_void_p]; L.or_mc_reference_energy.restype = C.c_double
+    L.or_mc_trace.argtypes = [C.c_void_p, C.c_uint64]
+    L.or_mc_trace_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]; L.or_mc_trace_copy.restype = C.c_uint64
+    L.or_ising_init_spins.argtypes = [C.c_int32, C.c_int32, C.c_uint64, C.c_void_p]
+    L.or_ising_energy.argtypes = [C.c_int32, C.c_void_p]; L.or_ising_energy.restype = C.c_double
+    L.or_lj_init.argtypes = [C.c_int32, C.c_double, C.c_uint64, C.c_void_p]; L.or_lj_init.restype = C.c_int
+    L.or_wca_init.argtypes = [C.c_int32, C.c_double, C.c_uint64, C.c_void_p]; L.or_wca_init.restype = C.c_int
+    L.or_lj_energy.argtypes = [C.c_int32, C.c_void_p, C.c_int32]; L.or_lj_energy.restype = C.c_double
+    L.or_lj_energy_ld.argtypes = [C.c_int32, C.c_void_p]; L.or_lj_energy_ld.restype = C.c_double
+    L.or_wca_energy_celllist.argtypes = [C.c_int32, C.c_void_p, C.c_double, C.c_double]
+    L.or_wca_energy_celllist.restype = C.c_double
+    L.or_wca_energy.argtypes = [C.c_int32, C.c_void_p, C.c_double, C.c_double, C.c_int32]
+    L.or_wca_energy.restype = C.c_double
+    L.or_wca_energy_ld.argtypes = [C.c_int32, C.c_void_p, C.c_double, C.c_double]
+    L.or_wca_energy_ld.restype = C.c_double
+    L.or_sad_gamma.argtypes = [C.c_double] * 4; L.or_sad_gamma.restype = C.c_double
+    L.or_bench.argtypes = [C.POINTER(Params), C.c_int32, C.c_uint64, C.c_uint64]; L.or_bench.restype = C.c_double
+    _LIB = L
+    return L
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def make_params(**kw):
+    p = Params()
+    defaults = dict(energy_bin=0.0, move_param=0.05, rc2=2.0 ** (1.0 / 3.0))
+    defaults.update(kw)
+    for k, v in defaults.items():
+        setattr(p, k, v)
+    return p
+
+
+class RNG:
+    """Xoroshiro128Plus seeded by SplitMix64 (src/rng.rs:27, energy.rs:835)."""
+
+    def __init__(self, seed=0, state=None):
+        self.s = (C.c_uint64 * 2)()
+        if state is not None:
+            self.s[0], self.s[1] = state
+        else:
+            lib().or_rng_seed(seed, self.s)
+
+    def next_u64(self):
+        return lib().or_rng_next(self.s)
+
+    def gen_range(self, lo, hi):
+        return lib().or_gen_range(self.s, lo, hi)
+
+    def uniform_index(self, n):
+        return lib().or_uniform_index(self.s, n)
+
+    def gen_f64(self):
+        return lib().or_gen_f64(self.s)
+
+    def uniform_f64(self, lo, hi):
+        return lib().or_uniform_f64(self.s, lo, hi)
+
+    @property
+    def state(self):
+        return (self.s[0], self.s[1])
+
+
+class Walker:
+    """One EnergyMC walker (energy.rs:167-210) over a system."""
+
+    def __init__(self, params, coords=None, spins=None, script=None, e0=0.0):
+        L = lib()
+        self.p = params
+        if script is not None:
+            s = np.ascontiguousarray(script, dtype=np.float64)
+            self.h = L.or_mc_new_scripted(C.byref(params), e0, s.ctypes.data_as(C.POINTER(C.c_double)), len(s))
+        else:
+            c = None if coords is None else np.ascontiguousarray(coords, dtype=np.float64)
+            sp = None if spins is None else np.ascontiguousarray(spins, dtype=np.int8)
+            self.h = L.or_mc_new(C.byref(params), _ptr(c), _ptr(sp))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_mc_free(self.h)
+            self.h = None
+
+    def run(self, moves):
+        return lib().or_mc_run(self.h, moves)
+
+    def trace(self, cap):
+        lib().or_mc_trace(self.h, cap)
+
+    def trace_copy(self, cap):
+        e = np.zeros(cap, np.float64)
+        a = np.zeros(cap, np.uint8)
+        n = lib().or_mc_trace_copy(self.h, _ptr(e), _ptr(a))
+        return e[:n], a[:n]
+
+    def scalars(self):
+        s = Scalars()
+        lib().or_mc_scalars(self.h, C.byref(s))
+        return s.as_dict()
+
+    def bins(self):
+        n = self.scalars()["len"]
+        out = dict(
+            lnw=np.zeros(n), histogram=np.zeros(n, np.uint64), t_found=np.zeros(n, np.uint64),
+            energy_total=np.zeros(n), energy_squared_total=np.zeros(n), visited=np.zeros(n, np.uint8),
+            round_trips=np.zeros(n, np.uint64), wl_hist=np.zeros(n, np.uint64),
+            pressure_count=np.zeros(n, np.uint64), pressure_total=np.zeros(n),
+        )
+        lib().or_mc_bins(self.h, *[_ptr(out[k]) for k in (
+            "lnw", "histogram", "t_found", "energy_total", "energy_squared_total", "visited",
+            "round_trips", "wl_hist", "pressure_count", "pressure_total")])
+        return out
+
+    def coords(self):
+        c = np.zeros(3 * self.p.n)
+        lib().or_mc_coords(self.h, _ptr(c))
+        return c
+
+    def spins(self):
+        s = np.zeros(self.p.n * self.p.n, np.int8)
+        lib().or_mc_spins(self.h, _ptr(s))
+        return s
+
+    def compute_energy(self):
+        return lib().or_mc_compute_energy(self.h)
+
+    def reference_energy(self):
+        return lib().or_mc_reference_energy(self.h)
+
+
+def lj_init(n, radius, seed):
+    c = np.zeros(3 * n)
+    rc = lib().or_lj_init(n, radius, seed, _ptr(c))
+    assert rc == 0
+    return c
+
+
+def wca_init(n, box, seed):
+    c = np.zeros(3 * n)
+    lib().or_wca_init(n, box, seed, _ptr(c))
+    return c
+
+
+def ising_spins(n, rule, seed):
+    s = np.zeros(n * n, np.int8)
+    lib().or_ising_init_spins(n, rule, seed, _ptr(s))
+    return s
+
+
+def ising_energy(n, spins):
+    s = np.ascontiguousarray(spins, dtype=np.int8)
+    return lib().or_ising_energy(n, _ptr(s))
+
+
+def lj_energy(coords, order_g=0):
+    c = np.ascontiguousarray(coords, dtype=np.float64)
+    return lib().or_lj_energy(len(c) // 3, _ptr(c), order_g)
+
+
+def lj_energy_ld(coords):
+    c = np.ascontiguousarray(coords, dtype=np.float64)
+    return lib().or_lj_energy_ld(len(c) // 3, _ptr(c))
+
+
+def wca_energy(coords, box, rc2, order_g=0):
+    c = np.ascontiguousarray(coords, dtype=np.float64)
+    return lib().or_wca_energy(len(c) // 3, _ptr(c), box, rc2, order_g)
+
+
+def wca_energy_celllist(coords, box, rc2):
+    c = np.ascontiguousarray(coords, dtype=np.float64)
+    return lib().or_wca_energy_celllist(len(c) // 3, _ptr(c), box, rc2)
+
+
+def wca_energy_ld(coords, box, rc2):
+    c = np.ascontiguousarray(coords, dtype=np.float64)
+    return lib().or_wca_energy_ld(len(c) // 3, _ptr(c), box, rc2)
+
+
+def sad_gamma(lam, t, tF, ns):
+    return lib().or_sad_gamma(lam, t, tF, ns)
+
+
+def bench(params, threads, moves, seed0=0):
+    return lib().or_bench(C.byref(params), threads, moves, seed0)

@@ -1,0 +1,1247 @@
+/*
+ * CPU oracle for the sad-monte-carlo walker hot path -- TEST INFRASTRUCTURE.
+ *
+ * A line-by-line restatement, in plain C, of the reference's EnergyMC loop
+ * and of the systems it drives.  Each function cites the reference lines it
+ * follows.  Built with -ffp-contract=off so that every floating-point
+ * expression rounds exactly as written, like Rust's (which never contracts).
+ * Loaded only by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+ * leg; the product never links it.  Pins are listed in sadmc_oracle.h.
+ */
+#include "sadmc_oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+/* Seed tweak for the initial-configuration stream (LJ/WCA positions and
+ * per-walker Ising spins), so that it is independent of the MC stream that
+ * seed_from_u64(seed) drives.  No reference counterpart (its inits use the
+ * fixed seeds 0 and 10137, lj.rs:128, wca.rs:402, ising.rs:45). */
+#define OR_INIT_TAG 0xA5F0C3E1D2B49687ULL
+
+enum { ST_OK = 0, ST_INDEX = 1, ST_VERIFY = 2, ST_INIT = 3 };
+
+/* ===================================================================== */
+/* Random numbers                                                         */
+/* ===================================================================== */
+
+static inline uint64_t rotl64(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+
+/* SplitMix64 (Vigna), as rand_xoshiro 0.4's SplitMix64::next_u64. */
+uint64_t or_splitmix64_next(uint64_t *x) {
+  *x += 0x9e3779b97f4a7c15ULL;
+  uint64_t z = *x;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+/* Xoroshiro128Plus::seed_from_u64 (rand_xoshiro 0.4, called at
+ * energy.rs:835): SplitMix64 state = seed; the 16-byte seed is filled
+ * little-endian from two SplitMix64 outputs, so s0, s1 = outputs 1, 2.
+ * (Not src/rng.rs:73-93, which the crate does not use: rng.rs:27.) */
+void or_rng_seed(uint64_t seed, uint64_t s[2]) {
+  uint64_t x = seed;
+  s[0] = or_splitmix64_next(&x);
+  s[1] = or_splitmix64_next(&x);
+}
+
+/* xoroshiro128+ step, src/rng.rs:48-57 (a=24, b=16, c=37). */
+uint64_t or_rng_next(uint64_t s[2]) {
+  uint64_t s0 = s[0], s1 = s[1];
+  uint64_t r = s0 + s1;
+  s1 ^= s0;
+  s[0] = rotl64(s0, 24) ^ s1 ^ (s1 << 16);
+  s[1] = rotl64(s1, 37);
+  return r;
+}
+
+/* rng.gen_range(low, high) for usize (ising.rs:105-106): rand 0.7
+ * UniformInt::sample_single with the conservative zone
+ * (range << leading_zeros(range)) - 1 and widening multiply. */
+uint64_t or_gen_range(uint64_t s[2], uint64_t low, uint64_t high) {
+  uint64_t range = high - low;
+  uint64_t zone = (range << __builtin_clzll(range)) - 1;
+  for (;;) {
+    uint64_t v = or_rng_next(s);
+    unsigned __int128 m = (unsigned __int128)v * range;
+    if ((uint64_t)m <= zone) return low + (uint64_t)(m >> 64);
+  }
+}
+
+/* rng.sample(Uniform::new(0, n)) for usize (lj.rs:368, wca.rs:345):
+ * rand 0.7 UniformInt with exact zone MAX - (2^64 mod n). */
+uint64_t or_uniform_index(uint64_t s[2], uint64_t n) {
+  uint64_t reject = (UINT64_MAX - n + 1) % n;
+  uint64_t zone = UINT64_MAX - reject;
+  for (;;) {
+    uint64_t v = or_rng_next(s);
+    unsigned __int128 m = (unsigned __int128)v * n;
+    if ((uint64_t)m <= zone) return (uint64_t)(m >> 64);
+  }
+}
+
+/* rng.gen::<f64>() (energy.rs:465,489,498,508): rand 0.7 Standard,
+ * (next_u64 >> 11) * 2^-53. */
+double or_gen_f64(uint64_t s[2]) {
+  uint64_t v = or_rng_next(s);
+  return (1.0 / 9007199254740992.0) * (double)(v >> 11);
+}
+
+static inline double f64_from_bits(uint64_t b) { double d; memcpy(&d, &b, 8); return d; }
+static inline uint64_t f64_bits(double d) { uint64_t b; memcpy(&b, &d, 8); return b; }
+
+/* rng.sample(Uniform::new(low, high)) for f64 (lj.rs:138 uses it): rand 0.7
+ * UniformFloat -- scale = high-low shrunk until scale*max_rand+low < high,
+ * sample = ((bits>>12 | 1.0) - 1) * scale + low. */
+double or_uniform_f64(uint64_t s[2], double low, double high) {
+  const double max_rand = f64_from_bits((UINT64_MAX >> 12) | 0x3FF0000000000000ULL) - 1.0;
+  double scale = high - low;
+  while (scale * max_rand + low >= high) scale = f64_from_bits(f64_bits(scale) - 1);
+  uint64_t v = or_rng_next(s) >> 12;
+  double v12 = f64_from_bits(v | 0x3FF0000000000000ULL);
+  return (v12 - 1.0) * scale + low;
+}
+
+/* ===================================================================== */
+/* Potentials                                                             */
+/* ===================================================================== */
+
+/* lj.rs:78-81: 4 eps ((s^2/r^2)^6 - (s^2/r^2)^3), sigma = eps = 1.  powi is
+ * expanded by LLVM as binary powering: x^6 = x^2 * (x^2)^2, x^3 = x * x^2. */
+double or_lj_pair(double r2) {
+  double s = 1.0 / r2;
+  double s2 = s * s;
+  double s4 = s2 * s2;
+  double s6 = s2 * s4;
+  double s3 = s * s2;
+  return 4.0 * (s6 - s3);
+}
+
+/* The same pair term in binary32 (precision variant; no reference
+ * counterpart). */
+float or_lj_pair_f32(float r2) {
+  float s = 1.0f / r2;
+  float s2 = s * s;
+  float s4 = s2 * s2;
+  float s6 = s2 * s4;
+  float s3 = s * s2;
+  return 4.0f * (s6 - s3);
+}
+
+/* wca.rs:66-76: shifted LJ, cut at r_c = 2^(1/6) sigma. */
+double or_wca_pair(double r2, double rc2) {
+  if (r2 < rc2) {
+    double s = 1.0 / r2;
+    double s2 = s * s;
+    double s4 = s2 * s2;
+    double s6 = s2 * s4;
+    double s3 = s * s2;
+    return 4.0 * (s6 - s3) + 1.0;
+  }
+  return 0.0;
+}
+
+/* wca.rs:79-92: -dV/dr r / 2 = 12 (2 s^6 - s^3). */
+static double wca_pair_pressure(double r2, double rc2) {
+  if (r2 < rc2) {
+    double s = 1.0 / r2;
+    double s2 = s * s;
+    double s4 = s2 * s2;
+    double s6 = s2 * s4;
+    double s3 = s * s2;
+    return 12.0 * (2.0 * s6 - s3);
+  }
+  return 0.0;
+}
+
+static inline double norm2(double x, double y, double z) { return x * x + y * y + z * z; }
+
+/* Minimum-image separation component for positions inside [0, L). */
+static inline double min_image(double d, double L, double halfL) {
+  if (d > halfL) return d - L;
+  if (d < -halfL) return d + L;
+  return d;
+}
+
+/* Combine G per-lane partial sums the way an xor butterfly of GPU lanes
+ * does (every lane ends with the same total). */
+static double butterfly(double *acc, int G) {
+  double tmp[64];
+  for (int off = 1; off < G; off <<= 1) {
+    for (int l = 0; l < G; l++) tmp[l] = acc[l] + acc[l ^ off];
+    for (int l = 0; l < G; l++) acc[l] = tmp[l];
+  }
+  return acc[0];
+}
+
+/* ===================================================================== */
+/* Walker state                                                           */
+/* ===================================================================== */
+
+struct or_mc {
+  or_params p;
+  int sys, n, G;
+  /* system */
+  double E, error;
+  double *x, *y, *z;
+  int8_t *S;
+  double R2, L, halfL, rc2;
+  int nsub;
+  /* pending change (possible_change) */
+  int ch_kind; /* 0 none, 1 move */
+  int64_t ch_which;
+  double ch_x, ch_y, ch_z, ch_e, ch_dabse;
+  /* scripted energies */
+  double *script;
+  int64_t script_len, script_pos;
+  /* MC */
+  uint64_t rng[2];
+  uint64_t moves, accepted;
+  double acc_rate, tscale;
+  double width, min;
+  int64_t len, cap;
+  double *lnw, *etot, *esq, *ptotal;
+  uint64_t *hist, *tfound, *rt, *pcount;
+  uint8_t *vis;
+  int has_pressure;
+  double max_S;
+  int64_t max_S_index;
+  int method;
+  double min_T, too_lo, too_hi, lambda;
+  uint64_t tL, tF, Ns, highest;
+  double t0, T;
+  double wl_gamma, wl_Ns, wl_min_energy, min_gamma;
+  uint64_t wl_lowest, wl_highest, wl_total;
+  uint64_t *wlh;
+  int64_t wl_len, wl_cap;
+  int wl_inv_t, has_min_gamma;
+  /* trace */
+  double *tr_e;
+  uint8_t *tr_a;
+  uint64_t tr_cap, tr_n;
+  int status;
+};
+
+/* ---- Ising (ising.rs) ------------------------------------------------ */
+
+/* ising.rs:45-49: spins are the low bit of successive draws, in site order.
+ * rule 0 uses the reference's fixed seed 10137; rule 1 a per-walker stream
+ * (BASELINE "i.i.d. from the seed"). */
+void or_ising_init_spins(int32_t n, int32_t rule, uint64_t seed, int8_t *S) {
+  uint64_t s[2];
+  or_rng_seed(rule == 0 ? 10137ULL : (seed ^ OR_INIT_TAG), s);
+  for (int64_t k = 0; k < (int64_t)n * n; k++) S[k] = (int8_t)(((int8_t)or_rng_next(s) & 1) * 2 - 1);
+}
+
+/* ising.rs:59-75. */
+double or_ising_energy(int32_t N, const int8_t *S) {
+  double e = 0.0;
+  for (int i1 = 0; i1 < N; i1++)
+    for (int j1 = 0; j1 < N; j1++) {
+      int j2 = (j1 + 1) % N;
+      int nt = S[i1 + j2 * N];
+      int i2 = (i1 + 1) % N;
+      nt += S[i2 + j1 * N];
+      e += (double)nt * (double)S[i1 + j1 * N];
+    }
+  return e;
+}
+
+/* ---- LJ (lj.rs) ------------------------------------------------------- */
+
+/* Initial positions (BASELINE rule, no reference counterpart; lj.rs:126-205
+ * is the reference's own rule): atoms added one at a time, each drawn
+ * uniformly in the sphere by rejection from the cube [-R,R)^3 and redrawn
+ * until every earlier atom is at least 0.8 sigma away. */
+int or_lj_init(int32_t n, double radius, uint64_t seed, double *c) {
+  uint64_t s[2];
+  or_rng_seed(seed ^ OR_INIT_TAG, s);
+  double *x = c, *y = c + n, *z = c + 2 * n;
+  const double R2 = radius * radius, dmin2 = 0.8 * 0.8;
+  for (int a = 0; a < n; a++) {
+    uint64_t tries = 0;
+    for (;;) {
+      if (++tries > 100000000ULL) return -1;
+      double px = or_uniform_f64(s, -radius, radius);
+      double py = or_uniform_f64(s, -radius, radius);
+      double pz = or_uniform_f64(s, -radius, radius);
+      if (norm2(px, py, pz) >= R2) continue;
+      int ok = 1;
+      for (int b = 0; b < a; b++)
+        if (norm2(px - x[b], py - y[b], pz - z[b]) < dmin2) { ok = 0; break; }
+      if (!ok) continue;
+      x[a] = px; y[a] = py; z[a] = pz;
+      break;
+    }
+  }
+  return 0;
+}
+
+/* lj.rs:236-244 (order 0) or the lane-group order. */
+double or_lj_energy(int32_t n, const double *c, int32_t G) {
+  const double *x = c, *y = c + n, *z = c + 2 * n;
+  if (G <= 0) {
+    double e = 0.0;
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < i; j++) e += or_lj_pair(norm2(x[i] - x[j], y[i] - y[j], z[i] - z[j]));
+    return e;
+  }
+  double acc[64];
+  for (int l = 0; l < G; l++) {
+    acc[l] = 0.0;
+    for (int i = l; i < n; i += G)
+      for (int j = 0; j < i; j++) acc[l] += or_lj_pair(norm2(x[i] - x[j], y[i] - y[j], z[i] - z[j]));
+  }
+  return butterfly(acc, G);
+}
+
+double or_lj_energy_ld(int32_t n, const double *c) {
+  const double *x = c, *y = c + n, *z = c + 2 * n;
+  long double e = 0.0L;
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < i; j++) {
+      long double dx = (long double)x[i] - x[j], dy = (long double)y[i] - y[j],
+                  dz = (long double)z[i] - z[j];
+      long double r2 = dx * dx + dy * dy + dz * dz;
+      long double s = 1.0L / r2, s3 = s * s * s;
+      e += 4.0L * (s3 * s3 - s3);
+    }
+  return (double)e;
+}
+
+/* ---- WCA (wca.rs, optcell.rs) ------------------------------------------ */
+
+/* Initial positions (no reference counterpart; wca.rs:392-498 keeps the
+ * lowest of N^2 random trials).  Of the simple-cubic, bcc and fcc lattices,
+ * each with the fewest cubic cells per edge m that give at least n sites,
+ * take the one with the largest nearest-neighbour distance (factor/m, ties
+ * to fcc, then bcc); choose n of its sites by a partial Fisher-Yates
+ * shuffle driven by the init stream. */
+static const double WCA_BASIS[3][4][3] = {
+    {{0.0, 0.0, 0.0}, {0.0, 0.5, 0.5}, {0.5, 0.0, 0.5}, {0.5, 0.5, 0.0}}, /* fcc */
+    {{0.0, 0.0, 0.0}, {0.5, 0.5, 0.5}},                                   /* bcc */
+    {{0.0, 0.0, 0.0}}};                                                   /* sc */
+static const int WCA_NBASIS[3] = {4, 2, 1};
+static const double WCA_NN[3] = {0.7071067811865476, 0.8660254037844386, 1.0};
+
+int or_wca_init(int32_t n, double box, uint64_t seed, double *c) {
+  uint64_t s[2];
+  or_rng_seed(seed ^ OR_INIT_TAG, s);
+  int best = -1, bm = 0;
+  double bestnn = -1.0;
+  for (int t = 0; t < 3; t++) {
+    int m = 1;
+    while ((int64_t)WCA_NBASIS[t] * m * m * m < n) m++;
+    double nn = WCA_NN[t] / (double)m;
+    if (nn > bestnn) { bestnn = nn; best = t; bm = m; }
+  }
+  int m = bm, nb = WCA_NBASIS[best];
+  int64_t ns = (int64_t)nb * m * m * m;
+  double *sx = malloc(sizeof(double) * ns), *sy = malloc(sizeof(double) * ns),
+         *sz = malloc(sizeof(double) * ns);
+  double a = box / (double)m;
+  int64_t k = 0;
+  for (int ix = 0; ix < m; ix++)
+    for (int iy = 0; iy < m; iy++)
+      for (int iz = 0; iz < m; iz++)
+        for (int b = 0; b < nb; b++) {
+          sx[k] = ((double)ix + WCA_BASIS[best][b][0]) * a;
+          sy[k] = ((double)iy + WCA_BASIS[best][b][1]) * a;
+          sz[k] = ((double)iz + WCA_BASIS[best][b][2]) * a;
+          k++;
+        }
+  for (int64_t i = 0; i < n; i++) {
+    int64_t r = i + (int64_t)or_uniform_index(s, (uint64_t)(ns - i));
+    double t;
+    t = sx[i]; sx[i] = sx[r]; sx[r] = t;
+    t = sy[i]; sy[i] = sy[r]; sy[r] = t;
+    t = sz[i]; sz[i] = sz[r]; sz[r] = t;
+  }
+  for (int i = 0; i < n; i++) { c[i] = sx[i]; c[n + i] = sy[i]; c[2 * n + i] = sz[i]; }
+  free(sx); free(sy); free(sz);
+  return 0;
+}
+
+/* optcell.rs:61-73 (cell count), :112-124 (get_subcell). */
+static int wca_nsub(double L, double rc2) { return (int)floor(L / sqrt(rc2)); }
+static inline int64_t subcell1(double r, double L, int nsub) { return (int64_t)floor(r / L * (double)nsub); }
+
+/* optcell.rs:373-405 stencil. */
+static const int NB[27][3] = {
+    {0, 0, 0},   {1, 0, 0},   {-1, 0, 0},  {0, 1, 0},   {0, -1, 0},  {0, 0, 1},  {0, 0, -1},
+    {0, 1, 1},   {0, 1, -1},  {0, -1, 1},  {0, -1, -1}, {1, 0, 1},   {1, 0, -1}, {-1, 0, 1},
+    {-1, 0, -1}, {1, 1, 0},   {1, -1, 0},  {-1, 1, 0},  {-1, -1, 0}, {1, 1, 1},  {-1, 1, 1},
+    {1, -1, 1},  {1, 1, -1},  {1, -1, -1}, {-1, 1, -1}, {-1, -1, 1}, {-1, -1, -1}};
+
+static inline int64_t modulus(int64_t i, int64_t sz) { return (i + sz) % sz; }
+
+/* Sum of f over the reference's maybe_interacting_atoms_excluding(r, which)
+ * (optcell.rs:93-110): every (atom, image) entry that add_to_subcells
+ * (optcell.rs:131-160) files under the subcell of r, with the image shifted
+ * by -offset*L.  Entries are visited in atom order, then stencil order. */
+typedef double (*pair_fn)(double r2, double rc2);
+static double celllist_sum(double init, const double *x, const double *y, const double *z, int n,
+                           int64_t which, double rx, double ry, double rz, double L, double rc2, int nsub,
+                           pair_fn f, double sign_r_minus_img, double *abs_acc) {
+  int64_t scx = subcell1(rx, L, nsub), scy = subcell1(ry, L, nsub), scz = subcell1(rz, L, nsub);
+  int64_t qx = modulus(scx, nsub), qy = modulus(scy, nsub), qz = modulus(scz, nsub);
+  double acc = init;
+  for (int j = 0; j < n; j++) {
+    if (j == which) continue;
+    int64_t jx = subcell1(x[j], L, nsub), jy = subcell1(y[j], L, nsub), jz = subcell1(z[j], L, nsub);
+    for (int t = 0; t < 27; t++) {
+      int64_t px = jx + NB[t][0], py = jy + NB[t][1], pz = jz + NB[t][2];
+      if (modulus(px, nsub) != qx || modulus(py, nsub) != qy || modulus(pz, nsub) != qz) continue;
+      int ox = px < 0 ? -1 : (px == nsub ? 1 : 0);
+      int oy = py < 0 ? -1 : (py == nsub ? 1 : 0);
+      int oz = pz < 0 ? -1 : (pz == nsub ? 1 : 0);
+      double ix = x[j] - (double)ox * L, iy = y[j] - (double)oy * L, iz = z[j] - (double)oz * L;
+      double r2 = sign_r_minus_img > 0 ? norm2(rx - ix, ry - iy, rz - iz) : norm2(ix - rx, iy - ry, iz - rz);
+      double v = f(r2, rc2);
+      acc += v;
+      if (abs_acc) *abs_acc += v;
+    }
+  }
+  return acc;
+}
+
+/* wca.rs:222-230 through the cell lists. */
+double or_wca_energy_celllist(int32_t n, const double *c, double L, double rc2) {
+  const double *x = c, *y = c + n, *z = c + 2 * n;
+  int nsub = wca_nsub(L, rc2);
+  double e = 0.0;
+  for (int i = 0; i < n; i++) {
+    /* e += V(|r1 - r2|^2) for each neighbour entry r2 of r1 */
+    int64_t scx = subcell1(x[i], L, nsub), scy = subcell1(y[i], L, nsub), scz = subcell1(z[i], L, nsub);
+    int64_t qx = modulus(scx, nsub), qy = modulus(scy, nsub), qz = modulus(scz, nsub);
+    for (int j = 0; j < n; j++) {
+      if (j == i) continue;
+      int64_t jx = subcell1(x[j], L, nsub), jy = subcell1(y[j], L, nsub), jz = subcell1(z[j], L, nsub);
+      for (int t = 0; t < 27; t++) {
+        int64_t px = jx + NB[t][0], py = jy + NB[t][1], pz = jz + NB[t][2];
+        if (modulus(px, nsub) != qx || modulus(py, nsub) != qy || modulus(pz, nsub) != qz) continue;
+        int ox = px < 0 ? -1 : (px == nsub ? 1 : 0);
+        int oy = py < 0 ? -1 : (py == nsub ? 1 : 0);
+        int oz = pz < 0 ? -1 : (pz == nsub ? 1 : 0);
+        double ix = x[j] - (double)ox * L, iy = y[j] - (double)oy * L, iz = z[j] - (double)oz * L;
+        e += or_wca_pair(norm2(x[i] - ix, y[i] - iy, z[i] - iz), rc2);
+      }
+    }
+  }
+  return e * 0.5;
+}
+
+/* Minimum-image energy, reference-like sequential or lane-group order. */
+double or_wca_energy(int32_t n, const double *c, double L, double rc2, int32_t G) {
+  const double *x = c, *y = c + n, *z = c + 2 * n;
+  double h = 0.5 * L;
+  if (G <= 0) G = 1;
+  double acc[64];
+  for (int l = 0; l < G; l++) {
+    acc[l] = 0.0;
+    for (int i = l; i < n; i += G)
+      for (int j = 0; j < i; j++)
+        acc[l] += or_wca_pair(norm2(min_image(x[i] - x[j], L, h), min_image(y[i] - y[j], L, h),
+                                    min_image(z[i] - z[j], L, h)),
+                              rc2);
+  }
+  return butterfly(acc, G);
+}
+
+double or_wca_energy_ld(int32_t n, const double *c, double L, double rc2) {
+  const double *x = c, *y = c + n, *z = c + 2 * n;
+  long double e = 0.0L, h = 0.5L * L;
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < i; j++) {
+      long double d[3] = {(long double)x[i] - x[j], (long double)y[i] - y[j], (long double)z[i] - z[j]};
+      for (int k = 0; k < 3; k++) {
+        if (d[k] > h) d[k] -= L;
+        else if (d[k] < -h) d[k] += L;
+      }
+      long double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      if (r2 < (long double)rc2) {
+        long double s = 1.0L / r2, s3 = s * s * s;
+        e += 4.0L * (s3 * s3 - s3) + 1.0L;
+      }
+    }
+  return (double)e;
+}
+
+/* optcell.rs:277-309. */
+static inline double put_in_cell1(double r, double L) {
+  if (r < 0.0) {
+    do { r += L; } while (r < 0.0);
+  } else {
+    while (r >= L) r -= L;
+  }
+  return r;
+}
+
+/* ===================================================================== */
+/* System dispatch (any.rs) for the walker                               */
+/* ===================================================================== */
+
+static double sys_compute_energy(const or_mc *m, int G) {
+  if (m->sys == OR_ISING) return or_ising_energy(m->n, m->S);
+  if (m->sys == OR_SCRIPTED) return m->E;
+  int n = m->n;
+  double *c = malloc(sizeof(double) * 3 * n);
+  memcpy(c, m->x, sizeof(double) * n);
+  memcpy(c + n, m->y, sizeof(double) * n);
+  memcpy(c + 2 * n, m->z, sizeof(double) * n);
+  double e;
+  if (m->sys == OR_LJ) e = or_lj_energy(n, c, G);
+  else if (G <= 0) e = or_wca_energy_celllist(n, c, m->L, m->rc2);
+  else e = or_wca_energy(n, c, m->L, m->rc2, G);
+  free(c);
+  return e;
+}
+
+/* Pair-term difference for atom `w` moving from f to t against atom j. */
+static inline double lj_term(const or_mc *m, int j, double tx, double ty, double tz, double fx, double fy,
+                             double fz) {
+  double xj = m->x[j], yj = m->y[j], zj = m->z[j];
+  if (m->p.precision == 1) {
+    float an = (float)(xj - tx), bn = (float)(yj - ty), cn = (float)(zj - tz);
+    float ao = (float)(xj - fx), bo = (float)(yj - fy), co = (float)(zj - fz);
+    float r2n = an * an + bn * bn + cn * cn;
+    float r2o = ao * ao + bo * bo + co * co;
+    return (double)or_lj_pair_f32(r2n) - (double)or_lj_pair_f32(r2o);
+  }
+  return or_lj_pair(norm2(xj - tx, yj - ty, zj - tz)) - or_lj_pair(norm2(xj - fx, yj - fy, zj - fz));
+}
+
+/* lj.rs:365-374 with the cube move, lj.rs:86-105 for the energy. */
+static int lj_plan_move(or_mc *m, double d, double *e_out) {
+  int64_t w = (int64_t)or_uniform_index(m->rng, (uint64_t)m->n);
+  double ux = or_uniform_f64(m->rng, -d, d);
+  double uy = or_uniform_f64(m->rng, -d, d);
+  double uz = or_uniform_f64(m->rng, -d, d);
+  double fx = m->x[w], fy = m->y[w], fz = m->z[w];
+  double tx = fx + ux, ty = fy + uy, tz = fz + uz;
+  double r2to = norm2(tx, ty, tz), r2from = norm2(fx, fy, fz);
+  if (r2to > m->R2 && r2to > r2from) return 0;
+  double e;
+  if (m->G <= 0) {
+    e = m->E;
+    for (int j = 0; j < m->n; j++)
+      if (j != w) e += lj_term(m, j, tx, ty, tz, fx, fy, fz);
+  } else {
+    double acc[64];
+    for (int l = 0; l < m->G; l++) {
+      acc[l] = 0.0;
+      for (int j = l; j < m->n; j += m->G)
+        if (j != w) acc[l] += lj_term(m, j, tx, ty, tz, fx, fy, fz);
+    }
+    e = m->E + butterfly(acc, m->G);
+  }
+  m->ch_kind = 1; m->ch_which = w; m->ch_x = tx; m->ch_y = ty; m->ch_z = tz; m->ch_e = e;
+  *e_out = e;
+  return 1;
+}
+
+/* wca.rs:341-353 with the cube move, wca.rs:119-140 for the energy. */
+static int wca_plan_move(or_mc *m, double d, double *e_out) {
+  int64_t w = (int64_t)or_uniform_index(m->rng, (uint64_t)m->n);
+  double ux = or_uniform_f64(m->rng, -d, d);
+  double uy = or_uniform_f64(m->rng, -d, d);
+  double uz = or_uniform_f64(m->rng, -d, d);
+  double fx = m->x[w], fy = m->y[w], fz = m->z[w];
+  double tx = put_in_cell1(fx + ux, m->L), ty = put_in_cell1(fy + uy, m->L), tz = put_in_cell1(fz + uz, m->L);
+  double e, dabse;
+  if (m->G <= 0) {
+    dabse = 0.0;
+    e = celllist_sum(m->E, m->x, m->y, m->z, m->n, w, tx, ty, tz, m->L, m->rc2, m->nsub, or_wca_pair, -1.0,
+                     &dabse);
+    /* e -= de for each entry, one at a time (wca.rs:128-132) */
+    {
+      int64_t scx = subcell1(fx, m->L, m->nsub), scy = subcell1(fy, m->L, m->nsub), scz = subcell1(fz, m->L, m->nsub);
+      int64_t qx = modulus(scx, m->nsub), qy = modulus(scy, m->nsub), qz = modulus(scz, m->nsub);
+      for (int j = 0; j < m->n; j++) {
+        if (j == w) continue;
+        int64_t jx = subcell1(m->x[j], m->L, m->nsub), jy = subcell1(m->y[j], m->L, m->nsub),
+                jz = subcell1(m->z[j], m->L, m->nsub);
+        for (int t = 0; t < 27; t++) {
+          int64_t px = jx + NB[t][0], py = jy + NB[t][1], pz = jz + NB[t][2];
+          if (modulus(px, m->nsub) != qx || modulus(py, m->nsub) != qy || modulus(pz, m->nsub) != qz) continue;
+          int ox = px < 0 ? -1 : (px == m->nsub ? 1 : 0);
+          int oy = py < 0 ? -1 : (py == m->nsub ? 1 : 0);
+          int oz = pz < 0 ? -1 : (pz == m->nsub ? 1 : 0);
+          double ix = m->x[j] - (double)ox * m->L, iy = m->y[j] - (double)oy * m->L, iz = m->z[j] - (double)oz * m->L;
+          double de = or_wca_pair(norm2(ix - fx, iy - fy, iz - fz), m->rc2);
+          e -= de;
+          dabse += de;
+        }
+      }
+    }
+  } else {
+    double acc[64], accabs[64];
+    double h = m->halfL, L = m->L;
+    for (int l = 0; l < m->G; l++) {
+      acc[l] = 0.0; accabs[l] = 0.0;
+      for (int j = l; j < m->n; j += m->G) {
+        if (j == w) continue;
+        double vn = or_wca_pair(norm2(min_image(m->x[j] - tx, L, h), min_image(m->y[j] - ty, L, h),
+                                      min_image(m->z[j] - tz, L, h)), m->rc2);
+        double vo = or_wca_pair(norm2(min_image(m->x[j] - fx, L, h), min_image(m->y[j] - fy, L, h),
+                                      min_image(m->z[j] - fz, L, h)), m->rc2);
+        acc[l] += vn - vo;
+        accabs[l] += vn + vo;
+      }
+    }
+    e = m->E + butterfly(acc, m->G);
+    dabse = butterfly(accabs, m->G);
+  }
+  m->ch_kind = 1; m->ch_which = w; m->ch_x = tx; m->ch_y = ty; m->ch_z = tz; m->ch_e = e; m->ch_dabse = dabse;
+  *e_out = e;
+  return 1;
+}
+
+/* ising.rs:104-119. */
+static int ising_plan_move(or_mc *m, double *e_out) {
+  int N = m->n;
+  int64_t i = (int64_t)or_gen_range(m->rng, 0, (uint64_t)N);
+  int64_t j = (int64_t)or_gen_range(m->rng, 0, (uint64_t)N);
+  int64_t j2 = (j + 1) % N;
+  int nt = m->S[i + j2 * N];
+  j2 = (j + N - 1) % N;
+  nt += m->S[i + j2 * N];
+  int64_t i2 = (i + 1) % N;
+  nt += m->S[i2 + j * N];
+  i2 = (i + N - 1) % N;
+  nt += m->S[i2 + j * N];
+  double e = m->E - (double)nt * (double)m->S[i + j * N] * 2.0;
+  m->ch_kind = 1; m->ch_which = i + j * N; m->ch_e = e;
+  *e_out = e;
+  return 1;
+}
+
+static int sys_plan_move(or_mc *m, double d, double *e) {
+  switch (m->sys) {
+    case OR_ISING: return ising_plan_move(m, e);
+    case OR_LJ: return lj_plan_move(m, d, e);
+    case OR_WCA: return wca_plan_move(m, d, e);
+    default: {
+      double v = m->script[m->script_pos % m->script_len];
+      m->script_pos++;
+      if (isnan(v)) return 0;
+      m->ch_kind = 1; m->ch_e = v;
+      *e = v;
+      return 1;
+    }
+  }
+}
+
+/* lj.rs:106-123 (error-estimate refresh). */
+static void lj_set_energy(or_mc *m, double new_e) {
+  double n = (double)m->n;
+  double new_error = fabs(new_e) > fabs(m->E) ? fabs(new_e) * 1e-15 * n : fabs(m->E) * 1e-15 * n;
+  m->error = new_error + m->error;
+  if (m->error > fabs(new_e) * 1e-14 * n * n) {
+    m->error *= 0.0;
+    m->E = sys_compute_energy(m, m->G);
+  } else {
+    m->E = new_e;
+  }
+}
+
+/* wca.rs:164-180. */
+static void wca_set_energy(or_mc *m, double new_e, double dabse) {
+  double n = (double)m->n;
+  double single = dabse > fabs(new_e) ? 1e-14 * dabse * n : 1e-14 * fabs(new_e) * n;
+  m->error += single * n;
+  if (m->error > fabs(new_e) * 1e-13 * n * n) {
+    m->E = sys_compute_energy(m, m->G);
+    m->error = 1e-15 * m->E * n;
+  } else {
+    m->E = new_e;
+  }
+}
+
+/* ConfirmSystem::confirm: ising.rs:94-101, lj.rs:338-358, wca.rs:276-301. */
+static void sys_confirm(or_mc *m) {
+  if (!m->ch_kind) return;
+  m->ch_kind = 0;
+  switch (m->sys) {
+    case OR_ISING:
+      m->S[m->ch_which] = (int8_t)(m->S[m->ch_which] * -1);
+      m->E = m->ch_e;
+      break;
+    case OR_LJ:
+      m->x[m->ch_which] = m->ch_x; m->y[m->ch_which] = m->ch_y; m->z[m->ch_which] = m->ch_z;
+      if (m->p.refresh > 0) m->E = m->ch_e;
+      else lj_set_energy(m, m->ch_e);
+      break;
+    case OR_WCA:
+      m->x[m->ch_which] = m->ch_x; m->y[m->ch_which] = m->ch_y; m->z[m->ch_which] = m->ch_z;
+      if (m->p.refresh > 0) m->E = m->ch_e;
+      else wca_set_energy(m, m->ch_e, m->ch_dabse);
+      break;
+    default:
+      m->E = m->ch_e;
+  }
+}
+
+/* lj.rs:249-261, wca.rs:237-250. */
+static void sys_verify_energy(or_mc *m) {
+  if (m->sys != OR_LJ && m->sys != OR_WCA) return;
+  double egood = sys_compute_energy(m, m->G);
+  double n = (double)m->n;
+  double expected = m->sys == OR_LJ ? fabs(m->E) * 1e-14 * n * n : fabs(m->E) * 1e-13 * n * n;
+  if (fabs(egood - m->E) > expected) m->status = ST_VERIFY;
+}
+
+/* wca.rs:202-218: pressure every N^2 moves. */
+static int sys_pressure(const or_mc *m, double *val) {
+  if (m->sys != OR_WCA || !m->p.collect_pressure) return 0;
+  uint64_t nn = (uint64_t)m->n * (uint64_t)m->n;
+  if (m->moves % nn != 0) return 0;
+  double p;
+  int n = m->n;
+  if (m->G <= 0) {
+    p = 0.0;
+    for (int i = 0; i < n; i++)
+      p = celllist_sum(p, m->x, m->y, m->z, n, i, m->x[i], m->y[i], m->z[i], m->L, m->rc2, m->nsub,
+                       wca_pair_pressure, 1.0, NULL);
+  } else {
+    double acc[64], L = m->L, h = m->halfL;
+    for (int l = 0; l < m->G; l++) {
+      acc[l] = 0.0;
+      for (int i = l; i < n; i += m->G)
+        for (int j = 0; j < n; j++)
+          if (j != i)
+            acc[l] += wca_pair_pressure(norm2(min_image(m->x[i] - m->x[j], L, h), min_image(m->y[i] - m->y[j], L, h),
+                                              min_image(m->z[i] - m->z[j], L, h)), m->rc2);
+    }
+    p = butterfly(acc, m->G);
+  }
+  *val = p / (3.0 * (m->L * m->L * m->L));
+  return 1;
+}
+
+/* ===================================================================== */
+/* EnergyMC (energy.rs)                                                   */
+/* ===================================================================== */
+
+static void bins_reserve(or_mc *m, int64_t need) {
+  if (need <= m->cap) return;
+  int64_t c = m->cap ? m->cap : 16;
+  while (c < need) c *= 2;
+#define GROW(ptr, T) ptr = (T *)realloc(ptr, sizeof(T) * c)
+  GROW(m->lnw, double); GROW(m->etot, double); GROW(m->esq, double); GROW(m->ptotal, double);
+  GROW(m->hist, uint64_t); GROW(m->tfound, uint64_t); GROW(m->rt, uint64_t); GROW(m->pcount, uint64_t);
+  GROW(m->vis, uint8_t);
+#undef GROW
+  m->cap = c;
+}
+
+/* Bins::state_to_index (energy.rs:371-373): `as usize` saturates. */
+static inline int64_t e_index(const or_mc *m, double e) {
+  double v = (e - m->min) / m->width;
+  if (!(v > 0.0)) return 0;
+  if (v >= 9.2e18) return INT64_MAX;
+  return (int64_t)v;
+}
+static inline double i_energy(const or_mc *m, int64_t i) { return m->min + ((double)i + 0.5) * m->width; }
+
+/* energy.rs:400-434. */
+static void prepare_for_state(or_mc *m, double e) {
+  while (e < m->min) {
+    bins_reserve(m, m->len + 1);
+#define INS0(a) memmove(m->a + 1, m->a, sizeof(m->a[0]) * m->len)
+    INS0(lnw); INS0(etot); INS0(esq); INS0(ptotal); INS0(hist); INS0(tfound); INS0(rt); INS0(pcount); INS0(vis);
+#undef INS0
+    m->lnw[0] = 0.0; m->etot[0] = 0.0; m->esq[0] = 0.0; m->ptotal[0] = 0.0;
+    m->hist[0] = 0; m->tfound[0] = 0; m->pcount[0] = 0;
+    m->vis[0] = 1;
+    m->rt[0] = 1;
+    m->len++;
+    m->min -= m->width;
+  }
+  while (e >= m->min + m->width * (double)m->len) {
+    bins_reserve(m, m->len + 1);
+    int64_t k = m->len;
+    m->lnw[k] = 0.0; m->hist[k] = 0; m->tfound[k] = 0; m->pcount[k] = 0; m->ptotal[k] = 0.0;
+    m->etot[k] = 0.0; m->esq[k] = 0.0; m->vis[k] = 1; m->rt[k] = 1;
+    m->len++;
+  }
+}
+
+/* energy.rs:22-38. */
+double or_sad_gamma(double lambda, double t, double tF, double ns) {
+  if (lambda * tF * ns == 0.0) return 0.0;
+  return (lambda + t / tF) / (lambda + t / ns * (t / tF));
+}
+
+/* energy.rs:799-824. */
+static double mc_gamma(const or_mc *m) {
+  switch (m->method) {
+    case OR_CANONICAL: return 0.0;
+    case OR_SAD: return or_sad_gamma(m->lambda, (double)m->moves, (double)m->tF, (double)m->Ns);
+    case OR_SAMC: {
+      double t = (double)m->moves;
+      return t > m->t0 ? m->t0 / t : 1.0;
+    }
+    default: return m->wl_gamma;
+  }
+}
+
+static int idx_ok(or_mc *m, int64_t i) {
+  if (i < 0 || i >= m->len) { m->status = ST_INDEX; return 0; }
+  return 1;
+}
+
+/* energy.rs:440-512. */
+static int reject_move(or_mc *m, double e1, double e2) {
+  int64_t i1 = e_index(m, e1), i2 = e_index(m, e2);
+  if (!idx_ok(m, i1) || !idx_ok(m, i2)) return 1;
+  switch (m->method) {
+    case OR_SAD: {
+      int64_t ilo = e_index(m, m->too_lo), ihi = e_index(m, m->too_hi);
+      if (!idx_ok(m, ilo) || !idx_ok(m, ihi)) return 1;
+      double lnw1 = e1 < m->too_lo ? m->lnw[ilo] + (e1 - m->too_lo) / m->min_T
+                    : e1 > m->too_hi ? m->lnw[ihi] : m->lnw[i1];
+      double lnw2 = e2 < m->too_lo ? m->lnw[ilo] + (e2 - m->too_lo) / m->min_T
+                    : e2 > m->too_hi ? m->lnw[ihi] : m->lnw[i2];
+      int rejected = lnw2 > lnw1 && or_gen_f64(m->rng) > exp(lnw1 - lnw2);
+      if (!rejected && m->hist[i2] == 0 && e2 < m->too_hi && e2 > m->too_lo) {
+        m->lambda = (m->too_hi - m->too_lo) / m->min_T;
+        m->Ns += 1;
+        m->tL = m->moves;
+      }
+      return rejected;
+    }
+    case OR_SAMC: {
+      double lnw1 = m->lnw[i1], lnw2 = m->lnw[i2];
+      return lnw2 > lnw1 && or_gen_f64(m->rng) > exp(lnw1 - lnw2);
+    }
+    case OR_WL:
+    case OR_INV_T_WL: {
+      double lnw1 = m->lnw[i1], lnw2 = m->lnw[i2];
+      int rejected = lnw2 > lnw1 && or_gen_f64(m->rng) > exp(lnw1 - lnw2);
+      if (!rejected && m->hist[i2] == 0 && m->wl_lowest > 0) m->wl_Ns += 1.0;
+      return rejected;
+    }
+    default: /* canonical */
+      if (e1 >= e2) return 0;
+      return or_gen_f64(m->rng) > exp((e1 - e2) / m->T);
+  }
+}
+
+static void wl_reserve(or_mc *m, int64_t need) {
+  if (need <= m->wl_cap) return;
+  int64_t c = m->wl_cap ? m->wl_cap : 16;
+  while (c < need) c *= 2;
+  m->wlh = (uint64_t *)realloc(m->wlh, sizeof(uint64_t) * c);
+  m->wl_cap = c;
+}
+
+/* energy.rs:514-761. */
+static void update_weights(or_mc *m, double E) {
+  int64_t i = e_index(m, E);
+  if (!idx_ok(m, i)) return;
+  double gamma = mc_gamma(m);
+  double old_lnw = m->lnw[i];
+  m->lnw[i] += gamma;
+  int switch_to_samc = 0;
+  double samc_t0 = 0.0;
+  if (m->method == OR_SAD) {
+    if (m->too_lo > m->too_hi || E < m->too_lo || E > m->too_hi) m->lnw[i] = old_lnw;
+    if (m->hist[i] > m->highest) {
+      m->highest = m->hist[i];
+      if (E > m->too_hi) {
+        int64_t ihi = e_index(m, m->too_hi);
+        if (!idx_ok(m, ihi)) return;
+        for (int64_t j = 0; j < m->len; j++) {
+          double ej = i_energy(m, j);
+          if (ej > m->too_hi && ej <= E) {
+            if (m->hist[j] != 0) { m->lnw[j] = m->lnw[ihi]; m->Ns += 1; }
+            else m->lnw[j] = 0.0;
+          }
+        }
+        m->lambda = (E - m->too_lo) / m->min_T;
+        m->tL = m->moves;
+        m->too_hi = i_energy(m, e_index(m, E));
+      } else if (E < m->too_lo) {
+        int64_t ilo = e_index(m, m->too_lo);
+        if (!idx_ok(m, ilo)) return;
+        for (int64_t j = 0; j < m->len; j++) {
+          double ej = i_energy(m, j);
+          if (ej < m->too_lo && ej >= E) {
+            if (m->hist[j] != 0) {
+              m->lnw[j] = m->lnw[ilo] + (ej - m->too_lo) / m->min_T;
+              if (m->lnw[j] < 0.0) m->lnw[j] = 0.0;
+              m->Ns += 1;
+            } else {
+              m->lnw[j] = 0.0;
+            }
+          }
+        }
+        m->lambda = (m->too_hi - E) / m->min_T;
+        m->tL = m->moves;
+        m->too_lo = i_energy(m, e_index(m, E));
+      }
+    }
+    if (m->tL == m->moves) {
+      int64_t ilo = e_index(m, m->too_lo), ihi = e_index(m, m->too_hi);
+      if (!idx_ok(m, ilo) || !idx_ok(m, ihi)) return;
+      uint64_t mx = m->tfound[ilo];
+      for (int64_t j = ilo; j <= ihi; j++)
+        if (m->tfound[j] > mx) mx = m->tfound[j];
+      uint64_t old_tF = m->tF;
+      m->tF = mx;
+      if (old_tF != m->tF && m->p.move_plan == 1) {
+        double s = m->acc_rate / m->p.move_param;
+        s = s < 0.8 ? 0.8 : (s > 1.2 ? 1.2 : s);
+        m->tscale *= s;
+      }
+    }
+  } else if (m->method == OR_WL || m->method == OR_INV_T_WL) {
+    if (m->has_min_gamma && m->wl_gamma < m->min_gamma) {
+      if (i >= m->wl_len) { m->status = ST_INDEX; return; }
+      m->wlh[i] += 1;
+      return;
+    }
+    if (m->wl_len != m->len) {
+      if (m->wl_len == 0 || (m->wl_gamma != 1.0 && m->wl_lowest > 0)) {
+        m->wl_gamma = 1.0;
+        m->wl_lowest = 0; m->wl_highest = 0; m->wl_total = 0;
+        wl_reserve(m, m->len);
+        memset(m->wlh, 0, sizeof(uint64_t) * m->len);
+        m->wl_len = m->len;
+        m->wl_min_energy = m->min;
+      } else {
+        while (m->wl_min_energy > m->min) {
+          m->wl_min_energy -= m->width;
+          wl_reserve(m, m->wl_len + 1);
+          memmove(m->wlh + 1, m->wlh, sizeof(uint64_t) * m->wl_len);
+          m->wlh[0] = 0;
+          m->wl_len++;
+        }
+        while (m->wl_len < m->len) {
+          wl_reserve(m, m->wl_len + 1);
+          m->wlh[m->wl_len++] = 0;
+        }
+        uint64_t mn = UINT64_MAX;
+        for (int64_t j = 0; j < m->wl_len; j++)
+          if (m->wlh[j] < mn) mn = m->wlh[j];
+        m->wl_lowest = mn;
+      }
+    }
+    if (i >= m->wl_len) { m->status = ST_INDEX; return; }
+    m->wlh[i] += 1;
+    if (m->wlh[i] > m->wl_highest) m->wl_highest = m->wlh[i];
+    m->wl_total += 1;
+    double max_energy = m->wl_min_energy + (double)m->wl_len * m->width;
+    if (m->wlh[i] == m->wl_lowest + 1 && m->wl_len > 1 &&
+        (!m->p.has_min_e || m->p.min_e >= m->wl_min_energy) &&
+        (!m->p.has_max_e || m->p.max_e <= max_energy)) {
+      int found = 0;
+      uint64_t mn = UINT64_MAX;
+      for (int64_t j = 0; j < m->wl_len; j++)
+        if (m->hist[j] != 0) { found = 1; if (m->wlh[j] < mn) mn = m->wlh[j]; }
+      if (found && mn == m->wl_lowest + 1) {
+        m->wl_lowest = m->wlh[i];
+        if ((m->wl_inv_t && m->wl_lowest > 0) ||
+            (double)m->wl_lowest >= 0.8 * (double)m->wl_total / m->wl_Ns) {
+          m->wl_gamma *= 0.5;
+          memset(m->wlh, 0, sizeof(uint64_t) * m->wl_len);
+          m->wl_total = 0; m->wl_lowest = 0; m->wl_highest = 0;
+          if (m->has_min_gamma && m->wl_gamma < m->min_gamma) m->wl_gamma = 0.0;
+        }
+        if (m->wl_inv_t && m->wl_gamma < m->wl_Ns / (double)m->moves) {
+          switch_to_samc = 1;
+          samc_t0 = m->wl_Ns;
+        }
+      }
+    }
+  }
+  if (switch_to_samc) {
+    m->method = OR_SAMC;
+    m->t0 = samc_t0;
+  }
+}
+
+/* energy.rs:904-974. */
+static void move_once(or_mc *m) {
+  m->moves += 1;
+  if (m->sys == OR_LJ || m->sys == OR_WCA) {
+    if (m->p.refresh > 0) {
+      if (m->moves % (uint64_t)m->p.refresh == 0) m->E = sys_compute_energy(m, m->G);
+    } else {
+      uint64_t L = (uint64_t)m->len;
+      if (m->moves % (L * L * 1000ULL) == 0) sys_verify_energy(m);
+    }
+  }
+  double e1 = m->E;
+  double rs = sqrt(1.0 / (double)m->moves);
+  m->acc_rate *= 1.0 - rs;
+  double e2;
+  int acc = 0;
+  if (sys_plan_move(m, m->tscale, &e2)) {
+    int oob = 0;
+    if (m->p.has_max_e) oob = e2 > m->p.max_e && e2 > e1;
+    if (m->p.has_min_e) oob = oob || (e2 < m->p.min_e && e2 < e1);
+    if (!oob) {
+      prepare_for_state(m, e2);
+      if (!reject_move(m, e1, e2)) {
+        m->accepted += 1;
+        m->acc_rate += rs;
+        sys_confirm(m);
+        acc = 1;
+      }
+    }
+  }
+  if (m->status) return;
+  double E = m->E;
+  int64_t i = e_index(m, E);
+  if (!idx_ok(m, i)) return;
+  if (m->hist[i] == 0) m->tfound[i] = m->moves;
+  m->hist[i] += 1;
+  m->etot[i] += E;
+  m->esq[i] += E * E;
+  double pv;
+  if (sys_pressure(m, &pv)) {
+    m->has_pressure = 1;
+    m->pcount[i] += 1;
+    m->ptotal[i] += pv;
+  }
+  update_weights(m, E);
+  if (m->status) return;
+  if (m->lnw[i] > m->max_S) {
+    m->max_S = m->lnw[i];
+    m->max_S_index = i;
+    memset(m->vis, 1, (size_t)m->len);
+  } else if (i == m->max_S_index) {
+    if (e_index(m, e1) != i) memset(m->vis, 0, (size_t)m->len);
+  } else if (!m->vis[i]) {
+    m->vis[i] = 1;
+    m->rt[i] += 1;
+  }
+  if (m->tr_n < m->tr_cap) {
+    m->tr_e[m->tr_n] = m->E;
+    m->tr_a[m->tr_n] = (uint8_t)acc;
+    m->tr_n++;
+  }
+}
+
+static or_mc *mc_alloc(const or_params *p) {
+  or_mc *m = (or_mc *)calloc(1, sizeof(or_mc));
+  m->p = *p;
+  m->sys = p->system;
+  m->n = p->n;
+  m->G = p->order_g;
+  return m;
+}
+
+/* energy.rs:830-898 (from_params) after the system exists. */
+static void mc_finish_init(or_mc *m) {
+  const or_params *p = &m->p;
+  double w = p->energy_bin > 0 ? p->energy_bin : (m->sys == OR_ISING ? 4.0 : 1.0);
+  or_rng_seed(p->seed, m->rng);
+  if (p->has_max_e) {
+    for (uint64_t k = 0; k < 100000000ULL; k++) {
+      double newe;
+      if (sys_plan_move(m, 0.05, &newe)) {
+        if (newe < m->E) sys_confirm(m);
+        if (m->E < p->max_e) break;
+      }
+    }
+  }
+  double E = m->E;
+  double emin = (round(E / w) - 0.5) * w;
+  m->width = w;
+  m->min = emin;
+  m->len = 1;
+  bins_reserve(m, 16);
+  m->hist[0] = 1; m->tfound[0] = 0; m->lnw[0] = 0.0; m->etot[0] = E; m->esq[0] = E * E;
+  m->pcount[0] = 0; m->ptotal[0] = 0.0;
+  m->vis[0] = 0; m->rt[0] = 1;
+  m->max_S = 0.0; m->max_S_index = 0;
+  m->moves = 0; m->accepted = 0; m->acc_rate = 0.5;
+  m->tscale = p->move_plan == 0 ? p->move_param : 0.05;
+  m->method = p->method;
+  switch (p->method) {
+    case OR_SAD:
+      m->min_T = p->method_param; m->too_lo = E; m->too_hi = E; m->tL = 0; m->tF = 0;
+      m->Ns = 1; m->highest = 1; m->lambda = 0.0;
+      break;
+    case OR_SAMC: m->t0 = p->method_param; break;
+    case OR_WL:
+    case OR_INV_T_WL:
+      m->wl_gamma = 1.0;
+      m->wl_lowest = (p->has_min_e && p->has_max_e) ? 0 : 1;
+      m->wl_highest = 1; m->wl_total = 0;
+      m->wl_Ns = (p->has_min_e && p->has_max_e) ? (p->max_e - p->min_e) / w : 1.0;
+      m->wl_len = 0;
+      m->wl_min_energy = E;
+      m->wl_inv_t = p->method == OR_INV_T_WL;
+      m->has_min_gamma = p->method == OR_WL && p->has_min_gamma;
+      m->min_gamma = p->method_param;
+      break;
+    default: m->T = p->method_param;
+  }
+}
+
+or_mc *or_mc_new(const or_params *p, const double *coords, const int8_t *spins) {
+  or_mc *m = mc_alloc(p);
+  int n = p->n;
+  if (m->sys == OR_ISING) {
+    m->S = (int8_t *)malloc((size_t)n * n);
+    if (spins) memcpy(m->S, spins, (size_t)n * n);
+    else or_ising_init_spins(n, p->ising_init, p->seed, m->S);
+    m->E = or_ising_energy(n, m->S);
+  } else {
+    double *c = (double *)malloc(sizeof(double) * 3 * n);
+    int rc = 0;
+    if (coords) memcpy(c, coords, sizeof(double) * 3 * n);
+    else if (m->sys == OR_LJ) rc = or_lj_init(n, p->radius, p->seed, c);
+    else rc = or_wca_init(n, p->box, p->seed, c);
+    m->x = c; m->y = c + n; m->z = c + 2 * n;
+    m->R2 = p->radius * p->radius;
+    m->L = p->box; m->halfL = 0.5 * p->box; m->rc2 = p->rc2;
+    if (m->sys == OR_WCA) m->nsub = wca_nsub(m->L, m->rc2);
+    if (rc) m->status = ST_INIT;
+    m->E = sys_compute_energy(m, m->G);
+  }
+  mc_finish_init(m);
+  return m;
+}
+
+or_mc *or_mc_new_scripted(const or_params *p, double e0, const double *script, int64_t len) {
+  or_mc *m = mc_alloc(p);
+  m->sys = OR_SCRIPTED;
+  m->script = (double *)malloc(sizeof(double) * len);
+  memcpy(m->script, script, sizeof(double) * len);
+  m->script_len = len;
+  m->E = e0;
+  mc_finish_init(m);
+  return m;
+}
+
+void or_mc_free(or_mc *m) {
+  if (!m) return;
+  free(m->x); free(m->S); free(m->script);
+  free(m->lnw); free(m->etot); free(m->esq); free(m->ptotal);
+  free(m->hist); free(m->tfound); free(m->rt); free(m->pcount); free(m->vis); free(m->wlh);
+  free(m->tr_e); free(m->tr_a);
+  free(m);
+}
+
+void or_mc_trace(or_mc *m, uint64_t cap) {
+  free(m->tr_e); free(m->tr_a);
+  m->tr_e = (double *)malloc(sizeof(double) * (cap ? cap : 1));
+  m->tr_a = (uint8_t *)malloc(cap ? cap : 1);
+  m->tr_cap = cap; m->tr_n = 0;
+}
+
+int or_mc_run(or_mc *m, uint64_t moves) {
+  for (uint64_t k = 0; k < moves && !m->status; k++) move_once(m);
+  return m->status;
+}
+
+void or_mc_scalars(const or_mc *m, or_scalars *o) {
+  memset(o, 0, sizeof(*o));
+  o->E = m->E; o->error = m->error; o->moves = m->moves; o->accepted = m->accepted;
+  o->acceptance_rate = m->acc_rate; o->translation_scale = m->tscale;
+  o->rng[0] = m->rng[0]; o->rng[1] = m->rng[1];
+  o->bins_min = m->min; o->bins_width = m->width; o->len = m->len;
+  o->max_S = m->max_S; o->max_S_index = m->max_S_index;
+  o->method = m->method; o->status = m->status;
+  o->too_lo = m->too_lo; o->too_hi = m->too_hi; o->lambda = m->lambda; o->min_T = m->min_T;
+  o->tL = m->tL; o->tF = m->tF; o->num_states = m->Ns; o->highest_hist = m->highest;
+  o->t0 = m->t0;
+  o->wl_gamma = m->wl_gamma; o->wl_num_states = m->wl_Ns; o->wl_min_energy = m->wl_min_energy;
+  o->wl_lowest = m->wl_lowest; o->wl_highest = m->wl_highest; o->wl_total = m->wl_total;
+  o->wl_len = m->wl_len; o->has_pressure = m->has_pressure;
+}
+
+void or_mc_bins(const or_mc *m, double *lnw, uint64_t *hist, uint64_t *t_found, double *e_tot, double *e_sq,
+                uint8_t *visited, uint64_t *round_trips, uint64_t *wl_hist, uint64_t *p_count, double *p_total) {
+  size_t n = (size_t)m->len;
+  if (lnw) memcpy(lnw, m->lnw, 8 * n);
+  if (hist) memcpy(hist, m->hist, 8 * n);
+  if (t_found) memcpy(t_found, m->tfound, 8 * n);
+  if (e_tot) memcpy(e_tot, m->etot, 8 * n);
+  if (e_sq) memcpy(e_sq, m->esq, 8 * n);
+  if (visited) memcpy(visited, m->vis, n);
+  if (round_trips) memcpy(round_trips, m->rt, 8 * n);
+  if (wl_hist) {
+    memset(wl_hist, 0, 8 * n);
+    size_t k = (size_t)m->wl_len < n ? (size_t)m->wl_len : n;
+    if (k) memcpy(wl_hist, m->wlh, 8 * k);
+  }
+  if (p_count) memcpy(p_count, m->pcount, 8 * n);
+  if (p_total) memcpy(p_total, m->ptotal, 8 * n);
+}
+
+void or_mc_coords(const or_mc *m, double *c) {
+  if (m->sys != OR_LJ && m->sys != OR_WCA) return;
+  memcpy(c, m->x, sizeof(double) * m->n);
+  memcpy(c + m->n, m->y, sizeof(double) * m->n);
+  memcpy(c + 2 * m->n, m->z, sizeof(double) * m->n);
+}
+
+void or_mc_spins(const or_mc *m, int8_t *s) {
+  if (m->sys == OR_ISING) memcpy(s, m->S, (size_t)m->n * m->n);
+}
+
+double or_mc_compute_energy(const or_mc *m) { return sys_compute_energy(m, m->G); }
+double or_mc_reference_energy(const or_mc *m) { return sys_compute_energy(m, 0); }
+
+/* Trace accessors live on the struct; expose copies. */
+uint64_t or_mc_trace_copy(const or_mc *m, double *energies, uint8_t *accepted) {
+  if (energies) memcpy(energies, m->tr_e, sizeof(double) * m->tr_n);
+  if (accepted) memcpy(accepted, m->tr_a, m->tr_n);
+  return m->tr_n;
+}
+
+/* ===================================================================== */
+/* CPU baseline: one walker per thread                                   */
+/* ===================================================================== */
+
+typedef struct {
+  or_params p;
+  uint64_t moves;
+  pthread_barrier_t *bar;
+  or_mc *mc;
+} bench_arg;
+
+static void *bench_thread(void *v) {
+  bench_arg *a = (bench_arg *)v;
+  a->mc = or_mc_new(&a->p, NULL, NULL);
+  pthread_barrier_wait(a->bar);
+  or_mc_run(a->mc, a->moves);
+  pthread_barrier_wait(a->bar);
+  return NULL;
+}
+
+double or_bench(const or_params *p, int32_t threads, uint64_t moves, uint64_t seed0) {
+  pthread_t *t = (pthread_t *)malloc(sizeof(pthread_t) * threads);
+  bench_arg *a = (bench_arg *)malloc(sizeof(bench_arg) * threads);
+  pthread_barrier_t bar;
+  pthread_barrier_init(&bar, NULL, (unsigned)threads + 1);
+  for (int i = 0; i < threads; i++) {
+    a[i].p = *p;
+    a[i].p.seed = seed0 + (uint64_t)i;
+    a[i].moves = moves;
+    a[i].bar = &bar;
+    pthread_create(&t[i], NULL, bench_thread, &a[i]);
+  }
+  pthread_barrier_wait(&bar);
+  struct timespec t0, t1;
+  clock_gettime(CLOCK_MONOTONIC, &t0);
+  pthread_barrier_wait(&bar);
+  clock_gettime(CLOCK_MONOTONIC, &t1);
+  for (int i = 0; i < threads; i++) {
+    pthread_join(t[i], NULL);
+    or_mc_free(a[i].mc);
+  }
+  pthread_barrier_destroy(&bar);
+  free(t); free(a);
+  return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+}

@@ -1,0 +1,153 @@
+/*
+ * CPU oracle for the sad-monte-carlo walker hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the
+ * reference's per-walker loop (src/mc/energy.rs:904-974) and the three
+ * systems it drives (src/system/ising.rs, lj.rs, wca.rs), together with the
+ * random-number stream (rand_xoshiro 0.4 Xoroshiro128Plus seeded by
+ * SplitMix64, restated step at src/rng.rs:48-57) and the rand 0.7 adapters
+ * the reference calls.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * CPU-baseline leg load it, and only as the checker.  The product library
+ * never links it.
+ *
+ * Summation order.  `order_g == 0` follows the reference's sequential sums
+ * (lj.rs:93-102, lj.rs:236-244, wca.rs:119-140, wca.rs:222-230).
+ * `order_g == G > 0` sums in the order a group of G GPU lanes does: lane l
+ * accumulates atoms l, l+G, l+2G, ... and the G partial sums are combined by
+ * an xor butterfly.  Both orders are the same mathematics; they differ only
+ * in rounding, and tests check that they agree to ~1e-12.
+ *
+ * Pinning (see tests/test_oracle_*.py): SplitMix64 against the published
+ * seed-0 vector; the xoroshiro128 linear engine against its full period and
+ * the published jump polynomial; Ising against brute force and the exact
+ * 8x8 density of states; LJ against closed-form potentials and the known
+ * cluster minima; SAD gamma against its closed form; the SAD/WL/SAMC
+ * bookkeeping against hand-built scripted-energy trajectories.  The rand 0.7
+ * integer/float adapters (gen_range, Uniform) are restated from the rand
+ * 0.7.3 sources as recalled -- parity unpinned against the Rust crate,
+ * which cannot be built here.
+ */
+#ifndef SADMC_ORACLE_H
+#define SADMC_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- random numbers -------------------------------------------------- */
+uint64_t or_splitmix64_next(uint64_t *x);
+void or_rng_seed(uint64_t seed, uint64_t s[2]);
+uint64_t or_rng_next(uint64_t s[2]);
+uint64_t or_gen_range(uint64_t s[2], uint64_t low, uint64_t high);
+uint64_t or_uniform_index(uint64_t s[2], uint64_t n);
+double or_gen_f64(uint64_t s[2]);
+double or_uniform_f64(uint64_t s[2], double low, double high);
+
+/* ---- potentials ------------------------------------------------------- */
+double or_lj_pair(double r2);
+float or_lj_pair_f32(float r2);
+double or_wca_pair(double r2, double rc2);
+
+/* ---- walker ----------------------------------------------------------- */
+enum { OR_ISING = 0, OR_LJ = 1, OR_WCA = 2, OR_SCRIPTED = 3 };
+enum { OR_SAD = 0, OR_SAMC = 1, OR_WL = 2, OR_INV_T_WL = 3, OR_CANONICAL = 4 };
+
+typedef struct {
+  int32_t system;
+  int32_t n;
+  int32_t method;
+  int32_t move_plan;        /* 0 TranslationScale, 1 AcceptanceRate */
+  double method_param;      /* SAD min_T, SAMC t0, canonical T, WL min_gamma */
+  int32_t has_min_gamma;
+  int32_t precision;        /* 0 fp64 pairs, 1 fp32 pairs + fp64 sums */
+  double move_param;        /* translation scale, or target acceptance rate */
+  double energy_bin;        /* <= 0: system default */
+  int32_t has_min_e;
+  int32_t has_max_e;
+  double min_e;
+  double max_e;
+  double radius;            /* LJ sphere radius */
+  double box;               /* WCA cube edge */
+  double rc2;               /* WCA cutoff squared */
+  int32_t refresh;          /* 0: reference error model; >0 fixed interval */
+  int32_t order_g;          /* 0 reference order; G>0 lane-group order */
+  int32_t ising_init;       /* 0 reference seed 10137; 1 per-walker seed */
+  int32_t collect_pressure; /* WCA pressure every N^2 moves */
+  uint64_t seed;
+} or_params;
+
+typedef struct {
+  double E;
+  double error;
+  uint64_t moves;
+  uint64_t accepted;
+  double acceptance_rate;
+  double translation_scale;
+  uint64_t rng[2];
+  double bins_min;
+  double bins_width;
+  int64_t len;
+  double max_S;
+  int64_t max_S_index;
+  int32_t method;           /* may switch 1/t-WL -> SAMC */
+  int32_t status;
+  /* SAD */
+  double too_lo, too_hi, lambda, min_T;
+  uint64_t tL, tF, num_states, highest_hist;
+  /* SAMC */
+  double t0;
+  /* WL */
+  double wl_gamma, wl_num_states, wl_min_energy;
+  uint64_t wl_lowest, wl_highest, wl_total;
+  int64_t wl_len;
+  int32_t has_pressure;
+} or_scalars;
+
+typedef struct or_mc or_mc;
+
+/* Build a walker.  coords: SoA x[n],y[n],z[n] (NULL: generate with the
+ * documented init rule); spins: n*n int8 (NULL: generate). */
+or_mc *or_mc_new(const or_params *p, const double *coords, const int8_t *spins);
+void or_mc_free(or_mc *mc);
+int or_mc_run(or_mc *mc, uint64_t moves);
+void or_mc_scalars(const or_mc *mc, or_scalars *out);
+/* Copy bins (logical order, len entries).  Any pointer may be NULL. */
+void or_mc_bins(const or_mc *mc, double *lnw, uint64_t *hist, uint64_t *t_found,
+                double *e_tot, double *e_sq, uint8_t *visited, uint64_t *round_trips,
+                uint64_t *wl_hist, uint64_t *p_count, double *p_total);
+void or_mc_coords(const or_mc *mc, double *coords);
+void or_mc_spins(const or_mc *mc, int8_t *spins);
+double or_mc_compute_energy(const or_mc *mc);     /* order-aware, fp64 */
+double or_mc_reference_energy(const or_mc *mc);   /* reference order */
+
+/* Per-move trace: energy after each move and whether it was accepted.
+ * or_mc_trace arms a buffer for the next cap moves; or_mc_trace_copy
+ * copies what was recorded and returns the count. */
+void or_mc_trace(or_mc *mc, uint64_t cap);
+uint64_t or_mc_trace_copy(const or_mc *mc, double *energies, uint8_t *accepted);
+
+/* Scripted system: plan_move returns script[k % len] (NaN = None). */
+or_mc *or_mc_new_scripted(const or_params *p, double e0, const double *script, int64_t len);
+
+/* System-level helpers for invariant tests. */
+void or_ising_init_spins(int32_t n, int32_t rule, uint64_t seed, int8_t *spins);
+double or_ising_energy(int32_t n, const int8_t *spins);
+int or_lj_init(int32_t n, double radius, uint64_t seed, double *coords);
+int or_wca_init(int32_t n, double box, uint64_t seed, double *coords);
+double or_lj_energy(int32_t n, const double *coords, int32_t order_g);
+double or_wca_energy_celllist(int32_t n, const double *coords, double box, double rc2);
+double or_wca_energy(int32_t n, const double *coords, double box, double rc2, int32_t order_g);
+double or_wca_energy_ld(int32_t n, const double *coords, double box, double rc2);
+double or_lj_energy_ld(int32_t n, const double *coords);
+double or_sad_gamma(double lambda, double t, double tF, double num_states);
+
+/* CPU baseline: run `threads` independent walkers (one per thread) for
+ * `moves` moves each; returns wall seconds (init excluded). */
+double or_bench(const or_params *p, int32_t threads, uint64_t moves, uint64_t seed0);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,738 @@
+// gfx950 kernels and the C ABI (include/sadmc.h) of the walker engine.
+//
+// Launch shape: one 64-lane wavefront per workgroup; a workgroup holds
+// 64/G walkers of one set, G lanes each (G = 1 for Ising).  Each workgroup
+// carries the LDS mirror of its walkers' positions: (64/G) * 3 * npad doubles.
+// The run kernel is persistent over the K moves of a batch: walker state is
+// read from HBM once, kept in registers for the whole batch, and written back
+// once.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sadmc_engine.h"
+
+using namespace sadmc;
+
+// ============================================================== kernels
+
+template <int SYS, int KMAX, int PREC>
+__global__ __launch_bounds__(64) void k_run(SetArgs a, uint64_t moves) {
+  extern __shared__ double lds[];
+  const int G = a.G;
+  const int group = threadIdx.x / G;
+  const int lane = threadIdx.x & (G - 1);
+  const int walker = blockIdx.x * (64 / G) + group;
+  if (walker >= a.W) return;
+  Engine<SYS, KMAX, PREC> e;
+  e.load(a, walker, lane, lds + group * 3 * a.npad);
+  e.run(moves);
+  e.store(walker);
+}
+
+template <int SYS, int KMAX, int PREC>
+__global__ __launch_bounds__(64) void k_init(SetArgs a, double e_lo, double width) {
+  extern __shared__ double lds[];
+  const int G = a.G;
+  const int group = threadIdx.x / G;
+  const int lane = threadIdx.x & (G - 1);
+  const int walker = blockIdx.x * (64 / G) + group;
+  if (walker >= a.W) return;
+  Engine<SYS, KMAX, PREC> e;
+  e.load(a, walker, lane, lds + group * 3 * a.npad);
+  e.from_params(e_lo, width);
+  e.store(walker);
+}
+
+template <int SYS, int KMAX, int PREC>
+__global__ __launch_bounds__(64) void k_plan(SetArgs a, int walker, double scale, Pending *out) {
+  extern __shared__ double lds[];
+  const int G = a.G;
+  if ((int)threadIdx.x >= G) return;
+  Engine<SYS, KMAX, PREC> e;
+  e.load(a, walker, threadIdx.x, lds);
+  double e2 = 0.0;
+  bool some = e.plan_move(scale, e2);
+  if (threadIdx.x == 0) {
+    Pending p;
+    p.some = some ? 1 : 0;
+    p.which = e.pw;
+    p.tx = e.ptx;
+    p.ty = e.pty;
+    p.tz = e.ptz;
+    p.e = e2;
+    p.dabse = e.pdabse;
+    *out = p;
+  }
+  e.store(walker);
+}
+
+template <int SYS, int KMAX, int PREC>
+__global__ __launch_bounds__(64) void k_confirm(SetArgs a, int walker, const Pending *pend) {
+  extern __shared__ double lds[];
+  const int G = a.G;
+  if ((int)threadIdx.x >= G) return;
+  Engine<SYS, KMAX, PREC> e;
+  e.load(a, walker, threadIdx.x, lds);
+  Pending p = *pend;
+  if (p.some) {
+    e.pw = p.which;
+    e.ptx = p.tx;
+    e.pty = p.ty;
+    e.ptz = p.tz;
+    e.pe = p.e;
+    e.pdabse = p.dabse;
+    e.confirm();
+  }
+  e.store(walker);
+}
+
+// ============================================================ dispatch
+
+enum KernelKind { KK_RUN, KK_INIT, KK_PLAN, KK_CONFIRM };
+
+struct LaunchArgs {
+  uint64_t moves;
+  double e_lo, width;
+  int walker;
+  double scale;
+  Pending *pending;
+};
+
+template <int SYS, int KMAX, int PREC>
+static hipError_t launch_t(KernelKind kk, const SetArgs &a, const LaunchArgs &la, hipStream_t s) {
+  const int G = a.G;
+  const int per_block = 64 / G;
+  const size_t lds = SYS == SYS_ISING ? 0 : (size_t)per_block * 3 * a.npad * sizeof(double);
+  const dim3 grid((a.W + per_block - 1) / per_block), block(64);
+  switch (kk) {
+    case KK_RUN:
+      hipLaunchKernelGGL((k_run<SYS, KMAX, PREC>), grid, block, lds, s, a, la.moves);
+      break;
+    case KK_INIT:
+      hipLaunchKernelGGL((k_init<SYS, KMAX, PREC>), grid, block, lds, s, a, la.e_lo, la.width);
+      break;
+    case KK_PLAN:
+      hipLaunchKernelGGL((k_plan<SYS, KMAX, PREC>), dim3(1), block, (size_t)3 * a.npad * sizeof(double), s, a,
+                         la.walker, la.scale, la.pending);
+      break;
+    case KK_CONFIRM:
+      hipLaunchKernelGGL((k_confirm<SYS, KMAX, PREC>), dim3(1), block, (size_t)3 * a.npad * sizeof(double), s, a,
+                         la.walker, la.pending);
+      break;
+  }
+  return hipGetLastError();
+}
+
+template <int SYS, int PREC>
+static hipError_t launch_k(int kmax, KernelKind kk, const SetArgs &a, const LaunchArgs &la, hipStream_t s) {
+  switch (kmax) {
+    case 1: return launch_t<SYS, 1, PREC>(kk, a, la, s);
+    case 2: return launch_t<SYS, 2, PREC>(kk, a, la, s);
+    case 4: return launch_t<SYS, 4, PREC>(kk, a, la, s);
+    case 8: return launch_t<SYS, 8, PREC>(kk, a, la, s);
+    default: return launch_t<SYS, 16, PREC>(kk, a, la, s);
+  }
+}
+
+static hipError_t launch(int sys, int prec, int kmax, KernelKind kk, const SetArgs &a, const LaunchArgs &la,
+                         hipStream_t s) {
+  if (sys == SYS_ISING) return launch_t<SYS_ISING, 1, 0>(kk, a, la, s);
+  if (sys == SYS_LJ) {
+    if (prec == SADMC_FP32_PAIRS) return launch_k<SYS_LJ, 1>(kmax, kk, a, la, s);
+    return launch_k<SYS_LJ, 0>(kmax, kk, a, la, s);
+  }
+  return launch_k<SYS_WCA, 0>(kmax, kk, a, la, s);
+}
+
+// ============================================================ host side
+
+namespace {
+
+struct SetHost {
+  sadmc_set_params p;
+  int32_t W = 0;
+  int G = 1, K = 1, KMAX = 1, npad = 0;
+  int64_t cap = 0;
+  double e_lo = 0, e_hi = 0, width = 1;
+  std::vector<uint64_t> seeds;
+  std::vector<double> geom;
+  std::vector<double> coords;  // W * 3 * n, unpadded
+  std::vector<int8_t> spins;   // W * n * n
+  sadmc_walker_state *d_ws = nullptr;
+  double *d_pos = nullptr;
+  uint64_t *d_spins = nullptr;
+  sadmc_bin *d_bins = nullptr;
+  PExtra *d_extra = nullptr;
+  Pending *d_pending = nullptr;
+  int32_t *d_flag = nullptr;
+  SetArgs args;
+};
+
+}  // namespace
+
+struct sadmc_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<SetHost> sets;
+  std::string err;
+  bool inited = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool timed = false;
+};
+
+static int fail(sadmc_ctx *c, int code, const std::string &msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHECK(ctx, call)                                                                          \
+  do {                                                                                               \
+    hipError_t _e = (call);                                                                          \
+    if (_e != hipSuccess) return fail(ctx, SADMC_E_HIP, std::string(#call ": ") + hipGetErrorString(_e)); \
+  } while (0)
+
+// ---------------------------------------------------------------- init rules
+
+// LJ initial positions (uniform in the sphere, all pairs >= 0.8 sigma):
+// atoms are added one at a time, each drawn by rejection from [-R, R)^3 and
+// redrawn until it is inside the sphere and 0.8 sigma from every earlier atom.
+static int lj_init_coords(int n, double radius, uint64_t seed, double *c) {
+  uint64_t s0, s1;
+  rng_seed(seed ^ kInitTag, s0, s1);
+  double *x = c, *y = c + n, *z = c + 2 * n;
+  const double R2 = radius * radius, dmin2 = 0.8 * 0.8;
+  const double sc = uniform_scale(-radius, radius);
+  for (int a = 0; a < n; a++) {
+    uint64_t tries = 0;
+    for (;;) {
+      if (++tries > 100000000ULL) return SADMC_E_INIT;
+      double px = uniform_sample(s0, s1, -radius, sc);
+      double py = uniform_sample(s0, s1, -radius, sc);
+      double pz = uniform_sample(s0, s1, -radius, sc);
+      if (norm2(px, py, pz) >= R2) continue;
+      bool ok = true;
+      for (int b = 0; b < a && ok; b++) ok = norm2(px - x[b], py - y[b], pz - z[b]) >= dmin2;
+      if (!ok) continue;
+      x[a] = px;
+      y[a] = py;
+      z[a] = pz;
+      break;
+    }
+  }
+  return SADMC_OK;
+}
+
+// WCA initial positions: of the simple-cubic, bcc and fcc lattices with the
+// fewest cells per edge m giving >= n sites, the one with the largest
+// nearest-neighbour distance (factor / m; ties to fcc, then bcc); n of its
+// sites chosen by a partial Fisher-Yates shuffle.
+static int wca_init_coords(int n, double box, uint64_t seed, double *c) {
+  static const double basis[3][4][3] = {
+      {{0.0, 0.0, 0.0}, {0.0, 0.5, 0.5}, {0.5, 0.0, 0.5}, {0.5, 0.5, 0.0}},
+      {{0.0, 0.0, 0.0}, {0.5, 0.5, 0.5}},
+      {{0.0, 0.0, 0.0}}};
+  static const int nbasis[3] = {4, 2, 1};
+  static const double nnf[3] = {0.7071067811865476, 0.8660254037844386, 1.0};
+  uint64_t s0, s1;
+  rng_seed(seed ^ kInitTag, s0, s1);
+  int best = 0, bm = 1;
+  double bestnn = -1.0;
+  for (int t = 0; t < 3; t++) {
+    int m = 1;
+    while ((int64_t)nbasis[t] * m * m * m < n) m++;
+    double nn = nnf[t] / (double)m;
+    if (nn > bestnn) {
+      bestnn = nn;
+      best = t;
+      bm = m;
+    }
+  }
+  const int m = bm, nb = nbasis[best];
+  const int64_t ns = (int64_t)nb * m * m * m;
+  std::vector<double> sx(ns), sy(ns), sz(ns);
+  const double a = box / (double)m;
+  int64_t k = 0;
+  for (int ix = 0; ix < m; ix++)
+    for (int iy = 0; iy < m; iy++)
+      for (int iz = 0; iz < m; iz++)
+        for (int b = 0; b < nb; b++) {
+          sx[k] = ((double)ix + basis[best][b][0]) * a;
+          sy[k] = ((double)iy + basis[best][b][1]) * a;
+          sz[k] = ((double)iz + basis[best][b][2]) * a;
+          k++;
+        }
+  for (int64_t i = 0; i < n; i++) {
+    int64_t r = i + (int64_t)uniform_index(s0, s1, (uint64_t)(ns - i));
+    std::swap(sx[i], sx[r]);
+    std::swap(sy[i], sy[r]);
+    std::swap(sz[i], sz[r]);
+  }
+  for (int i = 0; i < n; i++) {
+    c[i] = sx[i];
+    c[n + i] = sy[i];
+    c[2 * n + i] = sz[i];
+  }
+  return SADMC_OK;
+}
+
+// Known global minima of the free LJ clusters (Wales & Doye 1997); the
+// confined cluster cannot go below them.
+static double lj_lowest(int n) {
+  switch (n) {
+    case 13: return -44.326801;
+    case 19: return -72.659782;
+    case 31: return -133.586422;
+    case 38: return -173.928427;
+    default: return -0.5 * n * (n - 1.0);  // lj.rs:245-248
+  }
+}
+
+extern "C" {
+
+int sadmc_abi_version(void) { return SADMC_ABI_VERSION; }
+
+int sadmc_init_coords(int32_t system, int32_t n, double geom, uint64_t seed, double *coords) {
+  if (!coords || n < 2) return SADMC_E_ARG;
+  if (system == SADMC_LJ) return lj_init_coords(n, geom, seed, coords);
+  if (system == SADMC_WCA) return wca_init_coords(n, geom, seed, coords);
+  return SADMC_E_ARG;
+}
+
+int sadmc_init_spins(int32_t n, int32_t rule, uint64_t seed, int8_t *spins) {
+  if (!spins || n < 2) return SADMC_E_ARG;
+  uint64_t s0, s1;
+  rng_seed(rule == 0 ? 10137ULL : (seed ^ kInitTag), s0, s1);
+  for (int64_t k = 0; k < (int64_t)n * n; k++) spins[k] = (int8_t)(((int8_t)rng_next(s0, s1) & 1) * 2 - 1);
+  return SADMC_OK;
+}
+
+int sadmc_bin_window(const sadmc_set_params *p, double geom, double *e_lo, double *e_hi, int64_t *cap) {
+  if (!p) return SADMC_E_ARG;
+  const double w = p->energy_bin > 0 ? p->energy_bin : (p->system == SADMC_ISING ? 4.0 : 1.0);
+  double lo, hi;
+  if (p->e_lo_bound < p->e_hi_bound) {
+    lo = p->e_lo_bound;
+    hi = p->e_hi_bound;
+  } else if (p->system == SADMC_ISING) {
+    lo = -2.0 * p->n * p->n;
+    hi = 2.0 * p->n * p->n;
+  } else {
+    lo = p->system == SADMC_LJ ? lj_lowest(p->n) - 1.0 : 0.0;
+    if (p->has_min_e && p->min_e - w > lo) lo = p->min_e - w;
+    hi = p->has_max_e ? p->max_e : lo + 16384.0 * w;
+  }
+  (void)geom;
+  if (e_lo) *e_lo = lo;
+  if (e_hi) *e_hi = hi;
+  if (cap) *cap = (int64_t)std::ceil((hi - lo) / w) + 8;
+  return SADMC_OK;
+}
+
+int sadmc_create(sadmc_ctx **out, int device) {
+  if (!out) return SADMC_E_ARG;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= device || device < 0) return SADMC_E_NODEVICE;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return SADMC_E_NODEVICE;
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return SADMC_E_NODEVICE;
+  sadmc_ctx *c = new sadmc_ctx();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    delete c;
+    return SADMC_E_HIP;
+  }
+  *out = c;
+  return SADMC_OK;
+}
+
+static void free_set(SetHost &s) {
+  (void)hipFree(s.d_ws);
+  (void)hipFree(s.d_pos);
+  (void)hipFree(s.d_spins);
+  (void)hipFree(s.d_bins);
+  (void)hipFree(s.d_extra);
+  (void)hipFree(s.d_pending);
+  (void)hipFree(s.d_flag);
+  s.d_ws = nullptr;
+  s.d_pos = nullptr;
+  s.d_spins = nullptr;
+  s.d_bins = nullptr;
+  s.d_extra = nullptr;
+  s.d_pending = nullptr;
+  s.d_flag = nullptr;
+}
+
+void sadmc_destroy(sadmc_ctx *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto &s : c->sets) free_set(s);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char *sadmc_last_error(const sadmc_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+static int auto_lanes(int n) {
+  int want = (n + 3) / 4;
+  int g = 2;
+  while (g < want && g < 64) g <<= 1;
+  return g;
+}
+
+int sadmc_add_set(sadmc_ctx *c, const sadmc_set_params *p, int32_t W, const uint64_t *seeds, const double *geom,
+                  const double *coords, const int8_t *spins, int32_t *set_id) {
+  if (!c || !p || W <= 0 || !seeds) return fail(c, SADMC_E_ARG, "null argument or no walkers");
+  if (c->inited) return fail(c, SADMC_E_STATE, "sets must be added before sadmc_init");
+  SetHost s;
+  s.p = *p;
+  s.W = W;
+  const int n = p->n;
+  if (p->system == SADMC_ISING) {
+    if (n < 2 || n > 8) return fail(c, SADMC_E_ARG, "Ising lattice width must be 2..8");
+    s.G = 1;
+    s.K = 1;
+    s.KMAX = 1;
+    s.npad = 0;
+  } else if (p->system == SADMC_LJ || p->system == SADMC_WCA) {
+    if (n < 2 || n > 1024) return fail(c, SADMC_E_ARG, "atom count must be 2..1024");
+    if (!geom) return fail(c, SADMC_E_ARG, "LJ/WCA need per-walker geometry");
+    int G = p->lanes > 0 ? p->lanes : auto_lanes(n);
+    if (G < 2 || G > 64 || (G & (G - 1))) return fail(c, SADMC_E_ARG, "lanes must be a power of two in 2..64");
+    s.G = G;
+    s.K = (n + G - 1) / G;
+    if (s.K > 16) return fail(c, SADMC_E_ARG, "too few lanes for this many atoms (max 16 atoms per lane)");
+    s.KMAX = s.K <= 1 ? 1 : s.K <= 2 ? 2 : s.K <= 4 ? 4 : s.K <= 8 ? 8 : 16;
+    s.npad = G * s.K;
+    if (p->precision == SADMC_FP32_PAIRS && p->system != SADMC_LJ)
+      return fail(c, SADMC_E_ARG, "FP32 pair terms are implemented for LJ only");
+  } else {
+    return fail(c, SADMC_E_ARG, "unknown system");
+  }
+  if (p->method < SADMC_SAD || p->method > SADMC_CANONICAL) return fail(c, SADMC_E_ARG, "unknown method");
+  if (p->method == SADMC_CANONICAL && !(p->method_param > 0)) return fail(c, SADMC_E_ARG, "canonical T must be > 0");
+  if (p->move_param <= 0 && p->move_plan == SADMC_TRANSLATION_SCALE && p->system != SADMC_ISING)
+    return fail(c, SADMC_E_ARG, "translation scale must be > 0");
+  if (p->move_kind != SADMC_MOVE_CUBE) return fail(c, SADMC_E_ARG, "learned moves use the learned-proposal API");
+  s.width = p->energy_bin > 0 ? p->energy_bin : (p->system == SADMC_ISING ? 4.0 : 1.0);
+  sadmc_bin_window(p, geom ? geom[0] : 0.0, &s.e_lo, &s.e_hi, &s.cap);
+  s.seeds.assign(seeds, seeds + W);
+  if (p->system == SADMC_ISING) {
+    s.spins.resize((size_t)W * n * n);
+    for (int w = 0; w < W; w++) {
+      if (spins)
+        std::memcpy(&s.spins[(size_t)w * n * n], spins + (size_t)w * n * n, (size_t)n * n);
+      else
+        sadmc_init_spins(n, p->ising_init, seeds[w], &s.spins[(size_t)w * n * n]);
+    }
+  } else {
+    s.geom.assign(geom, geom + W);
+    s.coords.resize((size_t)W * 3 * n);
+    for (int w = 0; w < W; w++) {
+      double *dst = &s.coords[(size_t)w * 3 * n];
+      if (coords) {
+        std::memcpy(dst, coords + (size_t)w * 3 * n, sizeof(double) * 3 * n);
+      } else {
+        int rc = sadmc_init_coords(p->system, n, geom[w], seeds[w], dst);
+        if (rc) return fail(c, SADMC_E_INIT, "initial configuration failed");
+      }
+    }
+  }
+  c->sets.push_back(std::move(s));
+  if (set_id) *set_id = (int32_t)c->sets.size() - 1;
+  return SADMC_OK;
+}
+
+static void fill_args(SetHost &s) {
+  SetArgs &a = s.args;
+  a.ws = s.d_ws;
+  a.pos = s.d_pos;
+  a.spins = s.d_spins;
+  a.bins = s.d_bins;
+  a.extra = s.d_extra;
+  a.cap = s.cap;
+  a.W = s.W;
+  a.n = s.p.n;
+  a.npad = s.npad;
+  a.G = s.G;
+  a.K = s.K;
+  a.method = s.p.method;
+  a.move_plan = s.p.move_plan;
+  a.has_min_e = s.p.has_min_e;
+  a.has_max_e = s.p.has_max_e;
+  a.has_min_gamma = s.p.has_min_gamma;
+  a.refresh = s.p.refresh;
+  a.collect_pressure = s.p.collect_pressure;
+  a.precision = s.p.precision;
+  a.move_kind = s.p.move_kind;
+  a.method_param = s.p.method_param;
+  a.move_param = s.p.move_param;
+  a.min_e = s.p.min_e;
+  a.max_e = s.p.max_e;
+  a.rc2 = s.p.rc2;
+  a.flag = s.d_flag;
+}
+
+static int check_flags(sadmc_ctx *c) {
+  for (size_t k = 0; k < c->sets.size(); k++) {
+    SetHost &s = c->sets[k];
+    int32_t flag = 0;
+    HIPCHECK(c, hipMemcpy(&flag, s.d_flag, sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (!flag) continue;
+    std::vector<sadmc_walker_state> ws(s.W);
+    HIPCHECK(c, hipMemcpy(ws.data(), s.d_ws, sizeof(sadmc_walker_state) * s.W, hipMemcpyDeviceToHost));
+    for (int w = 0; w < s.W; w++) {
+      int st = ws[w].status;
+      if (!st) continue;
+      char buf[160];
+      std::snprintf(buf, sizeof buf, "set %zu walker %d stopped with status %d at move %llu", k, w, st,
+                    (unsigned long long)ws[w].moves);
+      int code = st == SADMC_WS_CAPACITY ? SADMC_E_CAPACITY
+                 : st == SADMC_WS_VERIFY ? SADMC_E_VERIFY
+                 : st == SADMC_WS_INIT   ? SADMC_E_INIT
+                                         : SADMC_E_INDEX;
+      return fail(c, code, buf);
+    }
+  }
+  return SADMC_OK;
+}
+
+int sadmc_init(sadmc_ctx *c) {
+  if (!c) return SADMC_E_ARG;
+  if (c->inited) return fail(c, SADMC_E_STATE, "already initialised");
+  if (c->sets.empty()) return fail(c, SADMC_E_STATE, "no walker sets");
+  HIPCHECK(c, hipSetDevice(c->device));
+  for (auto &s : c->sets) {
+    const int W = s.W, n = s.p.n;
+    HIPCHECK(c, hipMalloc(&s.d_ws, sizeof(sadmc_walker_state) * W));
+    HIPCHECK(c, hipMalloc(&s.d_bins, sizeof(sadmc_bin) * (size_t)W * s.cap));
+    HIPCHECK(c, hipMemsetAsync(s.d_bins, 0, sizeof(sadmc_bin) * (size_t)W * s.cap, c->stream));
+    HIPCHECK(c, hipMalloc(&s.d_pending, sizeof(Pending)));
+    HIPCHECK(c, hipMalloc(&s.d_flag, sizeof(int32_t)));
+    HIPCHECK(c, hipMemsetAsync(s.d_flag, 0, sizeof(int32_t), c->stream));
+    std::vector<sadmc_walker_state> ws(W);
+    std::memset(ws.data(), 0, sizeof(sadmc_walker_state) * W);
+    for (int w = 0; w < W; w++) {
+      rng_seed(s.seeds[w], ws[w].rng[0], ws[w].rng[1]);
+      ws[w].geom = s.p.system == SADMC_ISING ? 0.0 : s.geom[w];
+      ws[w].cap = s.cap;
+    }
+    HIPCHECK(c, hipMemcpyAsync(s.d_ws, ws.data(), sizeof(sadmc_walker_state) * W, hipMemcpyHostToDevice, c->stream));
+    if (s.p.system == SADMC_ISING) {
+      std::vector<uint64_t> bits(W, 0);
+      for (int w = 0; w < W; w++)
+        for (int b = 0; b < n * n; b++)
+          if (s.spins[(size_t)w * n * n + b] > 0) bits[w] |= 1ULL << b;
+      HIPCHECK(c, hipMalloc(&s.d_spins, sizeof(uint64_t) * W));
+      HIPCHECK(c, hipMemcpyAsync(s.d_spins, bits.data(), sizeof(uint64_t) * W, hipMemcpyHostToDevice, c->stream));
+    } else {
+      std::vector<double> pos((size_t)W * 3 * s.npad, 0.0);
+      for (int w = 0; w < W; w++)
+        for (int d = 0; d < 3; d++)
+          for (int j = 0; j < n; j++) pos[((size_t)w * 3 + d) * s.npad + j] = s.coords[((size_t)w * 3 + d) * n + j];
+      HIPCHECK(c, hipMalloc(&s.d_pos, sizeof(double) * pos.size()));
+      HIPCHECK(c, hipMemcpyAsync(s.d_pos, pos.data(), sizeof(double) * pos.size(), hipMemcpyHostToDevice, c->stream));
+      if (s.p.system == SADMC_WCA && s.p.collect_pressure) {
+        HIPCHECK(c, hipMalloc(&s.d_extra, sizeof(PExtra) * (size_t)W * s.cap));
+        HIPCHECK(c, hipMemsetAsync(s.d_extra, 0, sizeof(PExtra) * (size_t)W * s.cap, c->stream));
+      }
+    }
+    fill_args(s);
+    LaunchArgs la{};
+    la.e_lo = s.e_lo;
+    la.width = s.width;
+    HIPCHECK(c, launch(s.p.system, s.p.precision, s.KMAX, KK_INIT, s.args, la, c->stream));
+  }
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  c->inited = true;
+  return check_flags(c);
+}
+
+int sadmc_run(sadmc_ctx *c, uint64_t moves) {
+  if (!c) return SADMC_E_ARG;
+  if (!c->inited) return fail(c, SADMC_E_STATE, "sadmc_init first");
+  HIPCHECK(c, hipSetDevice(c->device));
+  HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
+  for (auto &s : c->sets) {
+    LaunchArgs la{};
+    la.moves = moves;
+    HIPCHECK(c, launch(s.p.system, s.p.precision, s.KMAX, KK_RUN, s.args, la, c->stream));
+  }
+  HIPCHECK(c, hipEventRecord(c->ev1, c->stream));
+  c->timed = true;
+  return SADMC_OK;
+}
+
+int sadmc_sync(sadmc_ctx *c) {
+  if (!c) return SADMC_E_ARG;
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  return check_flags(c);
+}
+
+int sadmc_last_run_ms(const sadmc_ctx *c, double *ms) {
+  if (!c || !ms || !c->timed) return SADMC_E_STATE;
+  float f = 0.0f;
+  if (hipEventElapsedTime(&f, c->ev0, c->ev1) != hipSuccess) return SADMC_E_HIP;
+  *ms = f;
+  return SADMC_OK;
+}
+
+static SetHost *get_set(sadmc_ctx *c, int32_t set, int32_t walker) {
+  if (!c || set < 0 || set >= (int32_t)c->sets.size()) return nullptr;
+  SetHost *s = &c->sets[set];
+  if (walker < 0 || walker >= s->W) return nullptr;
+  return s;
+}
+
+int sadmc_plan_move(sadmc_ctx *c, int32_t set, int32_t walker, double scale, double *e_new, int32_t *some) {
+  SetHost *s = get_set(c, set, walker);
+  if (!s || !c->inited) return fail(c, SADMC_E_ARG, "bad set/walker or not initialised");
+  LaunchArgs la{};
+  la.walker = walker;
+  la.scale = scale;
+  la.pending = s->d_pending;
+  HIPCHECK(c, launch(s->p.system, s->p.precision, s->KMAX, KK_PLAN, s->args, la, c->stream));
+  Pending p;
+  HIPCHECK(c, hipMemcpyAsync(&p, s->d_pending, sizeof p, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  if (e_new) *e_new = p.e;
+  if (some) *some = p.some;
+  return SADMC_OK;
+}
+
+int sadmc_confirm(sadmc_ctx *c, int32_t set, int32_t walker) {
+  SetHost *s = get_set(c, set, walker);
+  if (!s || !c->inited) return fail(c, SADMC_E_ARG, "bad set/walker or not initialised");
+  LaunchArgs la{};
+  la.walker = walker;
+  la.pending = s->d_pending;
+  HIPCHECK(c, launch(s->p.system, s->p.precision, s->KMAX, KK_CONFIRM, s->args, la, c->stream));
+  Pending none{};
+  HIPCHECK(c, hipMemcpyAsync(s->d_pending, &none, sizeof none, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  return SADMC_OK;
+}
+
+int sadmc_walker_count(const sadmc_ctx *c, int32_t set, int32_t *n, int64_t *cap) {
+  if (!c || set < 0 || set >= (int32_t)c->sets.size()) return SADMC_E_ARG;
+  if (n) *n = c->sets[set].W;
+  if (cap) *cap = c->sets[set].cap;
+  return SADMC_OK;
+}
+
+int sadmc_get_walker_state(sadmc_ctx *c, int32_t set, int32_t walker, sadmc_walker_state *out) {
+  SetHost *s = get_set(c, set, walker);
+  if (!s || !out || !c->inited) return fail(c, SADMC_E_ARG, "bad set/walker or not initialised");
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  HIPCHECK(c, hipMemcpy(out, s->d_ws + walker, sizeof *out, hipMemcpyDeviceToHost));
+  return SADMC_OK;
+}
+
+int sadmc_walker_bins(sadmc_ctx *c, int32_t set, int32_t walker, int64_t cap, double *lnw, uint64_t *hist,
+                      uint64_t *tf, double *et, double *es, uint8_t *vis, uint64_t *rt, uint64_t *wlh,
+                      uint64_t *pc, double *pt) {
+  SetHost *s = get_set(c, set, walker);
+  if (!s || !c->inited) return fail(c, SADMC_E_ARG, "bad set/walker or not initialised");
+  sadmc_walker_state st;
+  int rc = sadmc_get_walker_state(c, set, walker, &st);
+  if (rc) return rc;
+  if (st.len > cap) return fail(c, SADMC_E_ARG, "output arrays too short for len");
+  std::vector<sadmc_bin> b(st.len);
+  HIPCHECK(c, hipMemcpy(b.data(), s->d_bins + (size_t)walker * s->cap + st.base, sizeof(sadmc_bin) * st.len,
+                        hipMemcpyDeviceToHost));
+  std::vector<PExtra> x;
+  if (s->d_extra) {
+    x.resize(st.len);
+    HIPCHECK(c, hipMemcpy(x.data(), s->d_extra + (size_t)walker * s->cap + st.base, sizeof(PExtra) * st.len,
+                          hipMemcpyDeviceToHost));
+  }
+  for (int64_t i = 0; i < st.len; i++) {
+    if (lnw) lnw[i] = b[i].lnw;
+    if (hist) hist[i] = b[i].histogram;
+    if (tf) tf[i] = b[i].t_found;
+    if (et) et[i] = b[i].energy_total;
+    if (es) es[i] = b[i].energy_squared_total;
+    if (vis) vis[i] = (st.bulk_visited || b[i].visit_epoch == st.epoch) ? 1 : 0;
+    if (rt) rt[i] = b[i].round_trips;
+    if (wlh) wlh[i] = i < st.wl_len ? b[i].wl_hist : 0;
+    if (pc) pc[i] = x.empty() ? 0 : x[i].count;
+    if (pt) pt[i] = x.empty() ? 0.0 : x[i].total;
+  }
+  return SADMC_OK;
+}
+
+int sadmc_walker_coords(sadmc_ctx *c, int32_t set, int32_t walker, double *coords) {
+  SetHost *s = get_set(c, set, walker);
+  if (!s || !coords || !c->inited || s->p.system == SADMC_ISING) return fail(c, SADMC_E_ARG, "bad arguments");
+  std::vector<double> p((size_t)3 * s->npad);
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  HIPCHECK(c, hipMemcpy(p.data(), s->d_pos + (size_t)walker * 3 * s->npad, sizeof(double) * p.size(),
+                        hipMemcpyDeviceToHost));
+  const int n = s->p.n;
+  for (int d = 0; d < 3; d++)
+    for (int j = 0; j < n; j++) coords[d * n + j] = p[(size_t)d * s->npad + j];
+  return SADMC_OK;
+}
+
+int sadmc_walker_spins(sadmc_ctx *c, int32_t set, int32_t walker, int8_t *spins) {
+  SetHost *s = get_set(c, set, walker);
+  if (!s || !spins || !c->inited || s->p.system != SADMC_ISING) return fail(c, SADMC_E_ARG, "bad arguments");
+  uint64_t bits = 0;
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  HIPCHECK(c, hipMemcpy(&bits, s->d_spins + walker, sizeof bits, hipMemcpyDeviceToHost));
+  const int n = s->p.n;
+  for (int b = 0; b < n * n; b++) spins[b] = ((bits >> b) & 1ULL) ? 1 : -1;
+  return SADMC_OK;
+}
+
+int sadmc_download_raw(sadmc_ctx *c, int32_t set, sadmc_walker_state *states, sadmc_bin *bins) {
+  if (!c || set < 0 || set >= (int32_t)c->sets.size() || !c->inited) return fail(c, SADMC_E_ARG, "bad set");
+  SetHost &s = c->sets[set];
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  if (states)
+    HIPCHECK(c, hipMemcpy(states, s.d_ws, sizeof(sadmc_walker_state) * s.W, hipMemcpyDeviceToHost));
+  if (bins)
+    HIPCHECK(c, hipMemcpy(bins, s.d_bins, sizeof(sadmc_bin) * (size_t)s.W * s.cap, hipMemcpyDeviceToHost));
+  return SADMC_OK;
+}
+
+int sadmc_upload_raw(sadmc_ctx *c, int32_t set, const sadmc_walker_state *states, const sadmc_bin *bins,
+                     const double *coords, const int8_t *spins) {
+  if (!c || set < 0 || set >= (int32_t)c->sets.size() || !c->inited) return fail(c, SADMC_E_ARG, "bad set");
+  SetHost &s = c->sets[set];
+  const int n = s.p.n;
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  if (states)
+    HIPCHECK(c, hipMemcpy(s.d_ws, states, sizeof(sadmc_walker_state) * s.W, hipMemcpyHostToDevice));
+  if (bins)
+    HIPCHECK(c, hipMemcpy(s.d_bins, bins, sizeof(sadmc_bin) * (size_t)s.W * s.cap, hipMemcpyHostToDevice));
+  if (coords && s.p.system != SADMC_ISING) {
+    std::vector<double> pos((size_t)s.W * 3 * s.npad, 0.0);
+    for (int w = 0; w < s.W; w++)
+      for (int d = 0; d < 3; d++)
+        for (int j = 0; j < n; j++) pos[((size_t)w * 3 + d) * s.npad + j] = coords[((size_t)w * 3 + d) * n + j];
+    HIPCHECK(c, hipMemcpy(s.d_pos, pos.data(), sizeof(double) * pos.size(), hipMemcpyHostToDevice));
+  }
+  if (spins && s.p.system == SADMC_ISING) {
+    std::vector<uint64_t> bits(s.W, 0);
+    for (int w = 0; w < s.W; w++)
+      for (int b = 0; b < n * n; b++)
+        if (spins[(size_t)w * n * n + b] > 0) bits[w] |= 1ULL << b;
+    HIPCHECK(c, hipMemcpy(s.d_spins, bits.data(), sizeof(uint64_t) * s.W, hipMemcpyHostToDevice));
+  }
+  int32_t zero = 0;
+  HIPCHECK(c, hipMemcpy(s.d_flag, &zero, sizeof zero, hipMemcpyHostToDevice));
+  return SADMC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,268 @@
+"""Python host side of the sadmc walker engine (ctypes over include/sadmc.h).
+
+Mirrors the reference's driver-facing API: a `WalkerSet` is a batch of
+independent `EnergyMC` walkers (src/mc/energy.rs:167-210) whose
+`move_once` (energy.rs:904-974) runs on the GPU, K moves per launch.  The
+product path is the compiled HIP library `libsadmc_amd.so` next to this file;
+there is no CPU fallback: importing works anywhere, but creating a context
+raises unless a gfx950 device is present and the library is built.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsadmc_amd.so")
+_LIB = None
+
+ISING, LJ, WCA = 0, 1, 2
+SAD, SAMC, WL, INV_T_WL, CANONICAL = 0, 1, 2, 3, 4
+TRANSLATION_SCALE, ACCEPTANCE_RATE = 0, 1
+FP64, FP32_PAIRS = 0, 1
+MOVE_CUBE, MOVE_LEARNED = 0, 1
+
+WCA_RC2 = 2.0 ** (1.0 / 3.0)  # (2^(1/6) sigma)^2, wca.rs:61-63
+
+ERRORS = {
+    -1: "argument", -2: "HIP", -3: "capacity", -4: "verify", -5: "init", -6: "state", -7: "no device",
+    -8: "index",
+}
+
+
+class SadmcError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"sadmc error {code} ({ERRORS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+class SetParams(C.Structure):
+    _fields_ = [
+        ("system", C.c_int32), ("n", C.c_int32), ("method", C.c_int32), ("move_plan", C.c_int32),
+        ("method_param", C.c_double), ("has_min_gamma", C.c_int32), ("precision", C.c_int32),
+        ("move_param", C.c_double), ("energy_bin", C.c_double),
+        ("has_min_e", C.c_int32), ("has_max_e", C.c_int32), ("min_e", C.c_double), ("max_e", C.c_double),
+        ("rc2", C.c_double), ("refresh", C.c_int32), ("ising_init", C.c_int32),
+        ("collect_pressure", C.c_int32), ("lanes", C.c_int32),
+        ("e_lo_bound", C.c_double), ("e_hi_bound", C.c_double),
+        ("move_kind", C.c_int32), ("reserved0", C.c_int32), ("reserved", C.c_double * 4),
+    ]
+
+
+WALKER_STATE_DTYPE = np.dtype([
+    ("rng", "<u8", 2), ("E", "<f8"), ("error", "<f8"), ("moves", "<u8"), ("accepted_moves", "<u8"),
+    ("acceptance_rate", "<f8"), ("translation_scale", "<f8"), ("bins_min", "<f8"), ("bins_width", "<f8"),
+    ("base", "<i8"), ("len", "<i8"), ("max_S", "<f8"), ("max_S_index", "<i8"), ("epoch", "<u8"),
+    ("bulk_visited", "<u8"), ("method", "<i4"), ("status", "<i4"),
+    ("too_lo", "<f8"), ("too_hi", "<f8"), ("latest_parameter", "<f8"), ("min_T", "<f8"),
+    ("tL", "<u8"), ("tF", "<u8"), ("num_states", "<u8"), ("highest_hist", "<u8"), ("tmax", "<u8"),
+    ("t0", "<f8"), ("wl_gamma", "<f8"), ("wl_num_states", "<f8"), ("wl_min_energy", "<f8"),
+    ("wl_lowest_hist", "<u8"), ("wl_highest_hist", "<u8"), ("wl_total_hist", "<u8"), ("wl_len", "<i8"),
+    ("has_pressure", "<i4"), ("reserved0", "<i4"), ("geom", "<f8"), ("cap", "<i8"), ("reserved", "<u8", 3),
+])
+
+BIN_DTYPE = np.dtype([
+    ("lnw", "<f8"), ("energy_total", "<f8"), ("energy_squared_total", "<f8"), ("histogram", "<u8"),
+    ("t_found", "<u8"), ("round_trips", "<u8"), ("visit_epoch", "<u8"), ("wl_hist", "<u8"),
+])
+
+
+class WalkerStateC(C.Structure):
+    _fields_ = [(name, C.c_uint8 * WALKER_STATE_DTYPE.itemsize) for name in ("raw",)]
+
+
+# Every entry point the header declares (checked against the library by the
+# CPU test suite).
+EXPORTS = [
+    "sadmc_abi_version", "sadmc_create", "sadmc_destroy", "sadmc_last_error", "sadmc_add_set", "sadmc_init",
+    "sadmc_run", "sadmc_sync", "sadmc_last_run_ms", "sadmc_plan_move", "sadmc_confirm", "sadmc_walker_count",
+    "sadmc_get_walker_state", "sadmc_walker_bins", "sadmc_walker_coords", "sadmc_walker_spins",
+    "sadmc_download_raw", "sadmc_upload_raw", "sadmc_init_coords", "sadmc_init_spins", "sadmc_bin_window",
+]
+
+
+def lib():
+    """Load libsadmc_amd.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, i64, u64, d = C.c_void_p, C.c_int32, C.c_int64, C.c_uint64, C.c_double
+    L.sadmc_abi_version.restype = C.c_int
+    L.sadmc_create.argtypes = [C.POINTER(vp), C.c_int]
+    L.sadmc_destroy.argtypes = [vp]
+    L.sadmc_last_error.argtypes = [vp]; L.sadmc_last_error.restype = C.c_char_p
+    L.sadmc_add_set.argtypes = [vp, C.POINTER(SetParams), i32, vp, vp, vp, vp, C.POINTER(i32)]
+    L.sadmc_init.argtypes = [vp]
+    L.sadmc_run.argtypes = [vp, u64]
+    L.sadmc_sync.argtypes = [vp]
+    L.sadmc_last_run_ms.argtypes = [vp, C.POINTER(d)]
+    L.sadmc_plan_move.argtypes = [vp, i32, i32, d, C.POINTER(d), C.POINTER(i32)]
+    L.sadmc_confirm.argtypes = [vp, i32, i32]
+    L.sadmc_walker_count.argtypes = [vp, i32, C.POINTER(i32), C.POINTER(i64)]
+    L.sadmc_get_walker_state.argtypes = [vp, i32, i32, vp]
+    L.sadmc_walker_bins.argtypes = [vp, i32, i32, i64] + [vp] * 10
+    L.sadmc_walker_coords.argtypes = [vp, i32, i32, vp]
+    L.sadmc_walker_spins.argtypes = [vp, i32, i32, vp]
+    L.sadmc_download_raw.argtypes = [vp, i32, vp, vp]
+    L.sadmc_upload_raw.argtypes = [vp, i32, vp, vp, vp, vp]
+    L.sadmc_init_coords.argtypes = [i32, i32, d, u64, vp]
+    L.sadmc_init_spins.argtypes = [i32, i32, u64, vp]
+    L.sadmc_bin_window.argtypes = [C.POINTER(SetParams), d, C.POINTER(d), C.POINTER(d), C.POINTER(i64)]
+    for name in EXPORTS:
+        if name not in ("sadmc_last_error", "sadmc_destroy", "sadmc_abi_version"):
+            getattr(L, name).restype = C.c_int
+    _LIB = L
+    return L
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def make_params(**kw):
+    p = SetParams()
+    defaults = dict(rc2=WCA_RC2, move_param=0.05)
+    defaults.update(kw)
+    for k, v in defaults.items():
+        setattr(p, k, v)
+    return p
+
+
+def init_coords(system, n, geom, seed):
+    c = np.zeros(3 * n)
+    rc = lib().sadmc_init_coords(system, n, geom, seed, _p(c))
+    if rc:
+        raise SadmcError(rc, "init_coords")
+    return c
+
+
+def init_spins(n, rule, seed):
+    s = np.zeros(n * n, np.int8)
+    rc = lib().sadmc_init_spins(n, rule, seed, _p(s))
+    if rc:
+        raise SadmcError(rc, "init_spins")
+    return s
+
+
+def bin_window(params, geom=0.0):
+    lo, hi, cap = C.c_double(), C.c_double(), C.c_int64()
+    lib().sadmc_bin_window(C.byref(params), geom, C.byref(lo), C.byref(hi), C.byref(cap))
+    return lo.value, hi.value, cap.value
+
+
+class Context:
+    """One device context (one HIP stream) holding walker sets."""
+
+    def __init__(self, device=0):
+        L = lib()
+        h = C.c_void_p()
+        rc = L.sadmc_create(C.byref(h), device)
+        if rc:
+            raise SadmcError(rc, "sadmc_create: no gfx950 device or HIP failure")
+        self.h = h
+        self.sets = []
+
+    def _check(self, rc, what=""):
+        if rc:
+            raise SadmcError(rc, f"{what}: {lib().sadmc_last_error(self.h).decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sadmc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def add_set(self, params, seeds, geom=None, coords=None, spins=None):
+        seeds = np.ascontiguousarray(seeds, dtype=np.uint64)
+        g = None if geom is None else np.ascontiguousarray(np.broadcast_to(geom, seeds.shape), dtype=np.float64)
+        c = None if coords is None else np.ascontiguousarray(coords, dtype=np.float64)
+        s = None if spins is None else np.ascontiguousarray(spins, dtype=np.int8)
+        sid = C.c_int32()
+        self._check(lib().sadmc_add_set(self.h, C.byref(params), len(seeds), _p(seeds), _p(g), _p(c), _p(s),
+                                        C.byref(sid)), "add_set")
+        self.sets.append((params, len(seeds)))
+        return sid.value
+
+    def init(self):
+        self._check(lib().sadmc_init(self.h), "init")
+
+    def run(self, moves, sync=True):
+        self._check(lib().sadmc_run(self.h, moves), "run")
+        if sync:
+            self.sync()
+
+    def sync(self):
+        self._check(lib().sadmc_sync(self.h), "sync")
+
+    def last_run_ms(self):
+        ms = C.c_double()
+        self._check(lib().sadmc_last_run_ms(self.h, C.byref(ms)), "last_run_ms")
+        return ms.value
+
+    def plan_move(self, s, w, scale):
+        e, some = C.c_double(), C.c_int32()
+        self._check(lib().sadmc_plan_move(self.h, s, w, scale, C.byref(e), C.byref(some)), "plan_move")
+        return e.value if some.value else None
+
+    def confirm(self, s, w):
+        self._check(lib().sadmc_confirm(self.h, s, w), "confirm")
+
+    def count(self, s):
+        n, cap = C.c_int32(), C.c_int64()
+        self._check(lib().sadmc_walker_count(self.h, s, C.byref(n), C.byref(cap)), "count")
+        return n.value, cap.value
+
+    def state(self, s, w):
+        a = np.zeros(1, WALKER_STATE_DTYPE)
+        self._check(lib().sadmc_get_walker_state(self.h, s, w, _p(a)), "state")
+        return a[0]
+
+    def bins(self, s, w):
+        n = int(self.state(s, w)["len"])
+        out = dict(
+            lnw=np.zeros(n), histogram=np.zeros(n, np.uint64), t_found=np.zeros(n, np.uint64),
+            energy_total=np.zeros(n), energy_squared_total=np.zeros(n), visited=np.zeros(n, np.uint8),
+            round_trips=np.zeros(n, np.uint64), wl_hist=np.zeros(n, np.uint64),
+            pressure_count=np.zeros(n, np.uint64), pressure_total=np.zeros(n),
+        )
+        self._check(lib().sadmc_walker_bins(self.h, s, w, n, *[_p(out[k]) for k in (
+            "lnw", "histogram", "t_found", "energy_total", "energy_squared_total", "visited", "round_trips",
+            "wl_hist", "pressure_count", "pressure_total")]), "bins")
+        return out
+
+    def coords(self, s, w):
+        n = self.sets[s][0].n
+        c = np.zeros(3 * n)
+        self._check(lib().sadmc_walker_coords(self.h, s, w, _p(c)), "coords")
+        return c
+
+    def spins(self, s, w):
+        n = self.sets[s][0].n
+        a = np.zeros(n * n, np.int8)
+        self._check(lib().sadmc_walker_spins(self.h, s, w, _p(a)), "spins")
+        return a
+
+    def download_raw(self, s, with_bins=True):
+        W, cap = self.count(s)
+        st = np.zeros(W, WALKER_STATE_DTYPE)
+        b = np.zeros(W * cap, BIN_DTYPE) if with_bins else None
+        self._check(lib().sadmc_download_raw(self.h, s, _p(st), _p(b)), "download_raw")
+        return st, (b.reshape(W, cap) if with_bins else None)
+
+    def upload_raw(self, s, states, bins, coords=None, spins=None):
+        self._check(lib().sadmc_upload_raw(self.h, s, _p(np.ascontiguousarray(states)),
+                                           _p(np.ascontiguousarray(bins)),
+                                           _p(None if coords is None else np.ascontiguousarray(coords, np.float64)),
+                                           _p(None if spins is None else np.ascontiguousarray(spins, np.int8))),
+                    "upload_raw")
+
+
+def logical_bins(state, bins_row):
+    """Slice one walker's window to its logical bins [0, len)."""
+    b, n = int(state["base"]), int(state["len"])
+    return bins_row[b:b + n]
