@@ -1,0 +1,189 @@
+"""Benchmark: LJ31 SAD moves/s on one MI355X.
+
+Workload (the LJ31 precision configuration): 16384 independent SAD walkers,
+each an LJ cluster of N = 31 atoms in a hard sphere of radius 3.0 sigma,
+started from a random configuration with all pairs >= 0.8 sigma; single-atom
+moves uniform in a cube of half-width 0.1 sigma; SAD with min_T = 0.05
+epsilon, energy bins of 0.01 epsilon, --max-allowed-energy 0; FP32 pair terms
+accumulated in FP64, FP64 full-energy refresh every 1024 moves.
+
+A step is one launch that advances every walker by --moves-per-step moves.
+The walkers first run --burn-in moves untimed (SAD's start-up phase, when the
+energy range is still being discovered), then --warmup untimed steps, then
+exactly --steps timed steps bracketed by stream synchronisation.  `value` is
+total moves / wall seconds of the timed steps; the kernel time per launch
+comes from HIP events on the library's stream.
+
+The same line carries the FP64 variant's throughput, the relative energy
+error of a sample of walkers against an FP64 recompute, the roofline figures
+of the run kernel, and a CPU baseline: the C oracle (the same algorithm and
+configuration, one walker per thread) timed on this host.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "sad-monte-carlo_amd"))
+
+import sadmc_amd as S  # noqa: E402
+
+N_ATOMS = 31
+RADIUS = 3.0
+FLOP_PER_MOVE = 30 * (N_ATOMS - 1)  # 2(N-1) pair terms x 15 flop (division counted as one)
+FP32_VECTOR_PEAK = 157.3            # TFLOP/s, MI355X
+FP64_VECTOR_PEAK = 78.6
+
+
+def log(msg):
+    sys.stderr.write(msg + "\n")
+    sys.stderr.flush()
+
+
+def lj_params(precision, lanes):
+    return S.make_params(system=S.LJ, n=N_ATOMS, lanes=lanes, method=S.SAD, method_param=0.05,
+                         move_param=0.1, energy_bin=0.01, has_max_e=1, max_e=0.0, precision=precision,
+                         refresh=1024 if precision == S.FP32_PAIRS else 0)
+
+
+def lj_energy_fp64(c):
+    x = c.reshape(3, -1)
+    d = x[:, :, None] - x[:, None, :]
+    r2 = (d * d).sum(0)
+    iu = np.triu_indices(x.shape[1], 1)
+    s = 1.0 / r2[iu]
+    s3 = s * s * s
+    return float(np.sum(4.0 * (s3 * s3 - s3)))
+
+
+def run_gpu(precision, walkers, lanes, burn_in, warmup, steps, mps, label):
+    ctx = S.Context(0)
+    sid = ctx.add_set(lj_params(precision, lanes), np.arange(walkers, dtype=np.uint64), geom=RADIUS)
+    ctx.init()
+    log(f"[{label}] init done; burn-in {burn_in} moves")
+    left = burn_in
+    while left > 0:
+        m = min(left, 10000)
+        ctx.run(m)
+        left -= m
+    for _ in range(warmup):
+        ctx.run(mps)
+    log(f"[{label}] timing {steps} steps of {mps} moves x {walkers} walkers")
+    ctx.sync()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.run(mps, sync=False)
+        ctx.sync()
+        kernel_ms.append(ctx.last_run_ms())
+    wall = time.perf_counter() - t0
+    moves = steps * mps * walkers
+    # relative energy error of a sample of walkers against an FP64 recompute
+    st, _ = ctx.download_raw(sid, with_bins=False)
+    rel = []
+    for w in range(0, walkers, max(1, walkers // 256)):
+        e_ref = lj_energy_fp64(ctx.coords(sid, w))
+        rel.append(abs(st[w]["E"] - e_ref) / abs(e_ref))
+    ctx.close()
+    log(f"[{label}] {moves / wall / 1e9:.3f} G moves/s; kernel {np.mean(kernel_ms):.2f} ms/launch")
+    return dict(value=moves / wall, wall=wall, kernel_ms=float(np.mean(kernel_ms)),
+                max_rel_energy_error=float(np.max(rel)), mean_rel_energy_error=float(np.mean(rel)))
+
+
+def cpu_baseline(precision, seconds_target=10.0):
+    """The C oracle (same algorithm and configuration), one walker per thread,
+    built here with -O3 -march=native."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import ctypes as C
+    import oracle as O
+    src = os.path.join(ROOT, "oracle", "sadmc_oracle.c")
+    so = os.path.join("/tmp", f"liboracle_native_{os.getpid()}.so")
+    subprocess.run(["gcc", "-O3", "-march=native", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-shared", "-o", so,
+                    src, "-lm", "-lpthread"], check=True)
+    lib = C.CDLL(so)
+    lib.or_bench.argtypes = [C.POINTER(O.Params), C.c_int32, C.c_uint64, C.c_uint64]
+    lib.or_bench.restype = C.c_double
+    p = O.make_params(system=O.LJ, n=N_ATOMS, radius=RADIUS, method=O.SAD, method_param=0.05, move_param=0.1,
+                      energy_bin=0.01, has_max_e=1, max_e=0.0, precision=precision,
+                      refresh=1024 if precision == 1 else 0, order_g=0)
+    threads = len(os.sched_getaffinity(0))
+    # calibrate on one core, then size the sample to ~seconds_target of wall time
+    t1 = lib.or_bench(C.byref(p), 1, 400000, 1000)
+    per_core = 400000 / t1
+    moves = int(per_core * seconds_target)
+    wall = lib.or_bench(C.byref(p), threads, moves, 0)
+    os.remove(so)
+    cpu = subprocess.run(["bash", "-c", "grep -m1 'model name' /proc/cpuinfo | cut -d: -f2"], capture_output=True,
+                         text=True).stdout.strip()
+    return dict(value=threads * moves / wall, unit="moves/s", cores=threads, kind="port",
+                sample=f"{threads} walkers x {moves} moves (C oracle, reference order, -O3 -march=native) "
+                       f"in {wall:.1f} s on {cpu}; single-core calibration {per_core / 1e6:.2f} M moves/s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--moves-per-step", type=int, default=2000)
+    ap.add_argument("--walkers", type=int, default=16384)
+    ap.add_argument("--lanes", type=int, default=8)
+    ap.add_argument("--burn-in", type=int, default=30000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only (walkers are independent; see DESIGN.md for sharding); "
+              f"--gpus {a.gpus} is not supported", file=sys.stderr)
+        sys.exit(2)
+    main_run = run_gpu(S.FP32_PAIRS, a.walkers, a.lanes, a.burn_in, a.warmup, a.steps, a.moves_per_step,
+                       "lj31 fp32 pairs")
+    moves_per_launch = a.walkers * a.moves_per_step
+    achieved = FLOP_PER_MOVE * moves_per_launch / (main_run["kernel_ms"] * 1e-3) / 1e12
+    variants = {}
+    if not a.no_fp64:
+        r = run_gpu(S.FP64, a.walkers, a.lanes, a.burn_in, a.warmup, max(5, a.steps // 2), a.moves_per_step,
+                    "lj31 fp64")
+        variants["fp64"] = dict(value=r["value"], ms_per_step=r["wall"] * 1e3 / max(5, a.steps // 2),
+                                kernel_ms=r["kernel_ms"], max_rel_energy_error=r["max_rel_energy_error"],
+                                roofline_frac=FLOP_PER_MOVE * moves_per_launch / (r["kernel_ms"] * 1e-3) / 1e12
+                                / FP64_VECTOR_PEAK)
+    out = {
+        "metric": "LJ31 SAD moves/s on 1 GPU",
+        "value": main_run["value"],
+        "unit": "moves/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": main_run["wall"] * 1e3 / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 pair terms, f64 sums",
+        "data": "synthetic",
+        "config": {
+            "workload": "LJ31 SAD precision configuration: 16384 walkers, N=31, sphere R=3.0, cube moves 0.1, "
+                        "min_T=0.05, bins 0.01, max_allowed_energy 0, FP32 pairs + FP64 sums, FP64 refresh/1024",
+            "walkers": a.walkers, "lanes_per_walker": a.lanes, "moves_per_step": a.moves_per_step,
+            "burn_in_moves": a.burn_in,
+        },
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_VECTOR_PEAK, "unit": "TFLOP/s",
+                     "frac": achieved / FP32_VECTOR_PEAK, "traffic": None,
+                     "flop_per_move": FLOP_PER_MOVE, "kernel_ms_per_launch": main_run["kernel_ms"]},
+        "energy_error": {"max_rel": main_run["max_rel_energy_error"], "mean_rel": main_run["mean_rel_energy_error"]},
+        "variants": variants,
+    }
+    if not a.no_cpu:
+        log("cpu baseline: C oracle, one walker per thread")
+        out["cpu_baseline"] = cpu_baseline(1)
+        out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

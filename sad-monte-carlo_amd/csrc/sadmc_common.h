@@ -9,6 +9,7 @@
 #include <stdint.h>
 
 #define SADMC_HD __host__ __device__ __forceinline__
+#define SADMC_DF __device__ __forceinline__
 
 namespace sadmc {
 

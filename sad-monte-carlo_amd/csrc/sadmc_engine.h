@@ -2,21 +2,27 @@
 // (src/mc/energy.rs:904-974) per call, for a group of G lanes that share one
 // walker.
 //
-// Layout.  Every lane of a group holds the walker's scalar state in registers
-// (identical values; all lanes execute the same bookkeeping, so no broadcast
-// is ever needed).  For LJ/WCA, lane l of the group owns atoms l, l+G, l+2G,
-// ... (K slots, SoA registers); the group also keeps an LDS mirror of all N
-// positions so any lane can read the moved atom.  The energy difference of a
-// move is a per-lane partial sum over the lane's atoms followed by an xor
-// butterfly across the G lanes.  Ising uses one lane per walker (G = 1) with
-// the N*N <= 64 spins bit-packed in one 64-bit register.
+// Layout.  Every lane of a group holds the walker's hot scalar state in
+// registers (identical values: all lanes execute the same bookkeeping, so no
+// broadcast is ever needed).  For LJ/WCA, lane l of the group owns atoms l,
+// l+G, l+2G, ... (K slots, SoA registers); the group also keeps an LDS mirror
+// of all N positions so any lane can read the moved atom.  The energy
+// difference of a move is a per-lane partial sum over the lane's atoms
+// followed by an xor butterfly across the G lanes.  Ising uses one lane per
+// walker (G = 1) with the N*N <= 64 spins bit-packed in one 64-bit register.
 //
 // Bins.  A walker's bins are a fixed window of `cap` 64-byte records in HBM.
 // Logical bin i lives in physical slot base + i; the reference's front
 // insertion (energy.rs:400-420) becomes base -= 1, and the slots outside the
 // window are zero from allocation, which is exactly the value the reference
-// inserts.  The record of the current bin is cached in registers and written
-// through, so a move that stays in its bin touches no bin memory.
+// inserts.  The current bin's record is held in registers; the record of a
+// proposed bin is loaded once (one dependent memory round trip per move) and
+// becomes the current record if the move lands there.  Indices that only
+// change when the window or the SAD range changes (the current bin, the bins
+// of too_lo and too_hi, and ln w at those two bins) are cached and refreshed
+// exactly when their inputs change, so each move computes one bin index.
+// Periodic checks (verify_energy, the fixed refresh, WCA pressure) use
+// next-trigger counters instead of a 64-bit modulo per move.
 #pragma once
 
 #include "sadmc_common.h"
@@ -25,25 +31,31 @@
 namespace sadmc {
 
 enum { SYS_ISING = 0, SYS_LJ = 1, SYS_WCA = 2 };
+// Method families compiled separately: SAD; SAMC; WL and 1/t-WL (which may
+// switch to SAMC, energy.rs:747-756); canonical.
+enum { M_SAD = 0, M_SAMC = 1, M_WL = 2, M_CAN = 3 };
 
 struct PExtra {
   uint64_t count;
   double total;
 };
 
-// Arguments of one walker set, passed by value to the kernels.
+// Arguments of one walker set, passed by value to the kernels (they live in
+// scalar registers).
 struct SetArgs {
   sadmc_walker_state *ws;
   double *pos;       // [W][3][npad]
   uint64_t *spins;   // [W]
   sadmc_bin *bins;   // [W][cap]
   PExtra *extra;     // [W][cap] (WCA pressure) or null
+  int32_t *flag;     // set to 1 when any walker of the set stops with a status
   int64_t cap;
+  uint64_t zone_n;     // Uniform::new(0, n) acceptance zone
+  uint64_t zone_site;  // gen_range(0, N) acceptance zone (Ising)
   int32_t W, n, npad, G, K;
   int32_t method, move_plan, has_min_e, has_max_e, has_min_gamma;
   int32_t refresh, collect_pressure, precision, move_kind;
-  double method_param, move_param, min_e, max_e, rc2;
-  int32_t *flag;     // set to 1 when any walker of the set stops with a status
+  double method_param, move_param, min_e, max_e, rc2, width;
 };
 
 // Pending change of the single-walker plan/confirm entry points.
@@ -57,31 +69,78 @@ __device__ __forceinline__ double group_sum(double v, int G) {
   return v;
 }
 
-template <int SYS, int KMAX, int PREC>
+// rand 0.7 UniformInt sampling with a precomputed zone (lj.rs:368 /
+// ising.rs:105 semantics; the zone is the only part that depends on n).
+__device__ __forceinline__ uint64_t sample_zone(uint64_t &s0, uint64_t &s1, uint64_t range, uint64_t zone) {
+  for (;;) {
+    uint64_t v = rng_next(s0, s1);
+    uint64_t lo = v * range;
+    if (lo <= zone) return __umul64hi(v, range);
+  }
+}
+
+
+// Field-wise copies of a bin record (no aggregate memcpy, so the record stays
+// in registers).
+SADMC_HD void copy_bin(sadmc_bin &d, const sadmc_bin &s) {
+  d.lnw = s.lnw;
+  d.energy_total = s.energy_total;
+  d.energy_squared_total = s.energy_squared_total;
+  d.histogram = s.histogram;
+  d.t_found = s.t_found;
+  d.round_trips = s.round_trips;
+  d.visit_epoch = s.visit_epoch;
+  d.wl_hist = s.wl_hist;
+}
+
+template <int SYS, int KMAX, int PREC, int METH>
 struct Engine {
   SetArgs A;
   int lane, G, K, n;
-  sadmc_walker_state st;
   sadmc_bin *B;
   PExtra *X;
+  // system
   double *lx, *ly, *lz;
   double x[KMAX], y[KMAX], z[KMAX];
   uint64_t spins;
   double R2, L, halfL;
-  int64_t cur_p;
-  sadmc_bin cur;
+  // EnergyMC scalars
+  uint64_t s0, s1;
+  double E, err;
+  uint64_t moves, accepted;
+  double tm;  // (double)moves, kept exact by +1.0 steps
+  double acc_rate, tscale, usc;
+  double bmin;
+  int32_t base, len;
+  double max_S;
+  int32_t max_S_index;
+  uint64_t epoch;
+  int32_t bulk, status, method;
+  // SAD
+  double too_lo, too_hi, lam, min_T, tF_d, Ns_d, lnw_lo, lnw_hi;
+  uint64_t tL, tF, Ns, highest, tmax;
+  int32_t ilo, ihi;
+  // SAMC / WL
+  double t0, wl_gamma, wl_Ns, wl_min_e;
+  uint64_t wl_lowest, wl_highest, wl_total;
+  int32_t wl_len;
+  int32_t has_pressure;
+  // current bin (logical cur_i, physical cur_p) and its record
+  int32_t cur_i, cur_p;
+  sadmc_bin c;
+  // periodic triggers
+  uint64_t next_check, next_press;
   // pending move
   int32_t pw;
   double ptx, pty, ptz, pe, pdabse;
 
   // ---------------------------------------------------------------- setup
-  __device__ void load(const SetArgs &a, int walker, int lane_in_group, double *lds_group) {
+  SADMC_DF void load_system(const SetArgs &a, int walker, int lane_in_group, double *lds_group) {
     A = a;
     lane = lane_in_group;
     G = a.G;
     K = a.K;
     n = a.n;
-    st = a.ws[walker];
     B = a.bins + (int64_t)walker * a.cap;
     X = a.extra ? a.extra + (int64_t)walker * a.cap : nullptr;
     if (SYS == SYS_ISING) {
@@ -104,17 +163,113 @@ struct Engine {
         }
       }
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      if (SYS == SYS_LJ) R2 = st.geom * st.geom;
-      L = st.geom;
-      halfL = 0.5 * st.geom;
     }
-    cur_p = -1;
   }
 
-  __device__ void store(int walker) {
+  SADMC_DF void load(const SetArgs &a, int walker, int lane_in_group, double *lds_group) {
+    load_system(a, walker, lane_in_group, lds_group);
+    const sadmc_walker_state &w = a.ws[walker];
+    s0 = w.rng[0];
+    s1 = w.rng[1];
+    E = w.E;
+    err = w.error;
+    moves = w.moves;
+    tm = (double)moves;
+    accepted = w.accepted_moves;
+    acc_rate = w.acceptance_rate;
+    tscale = w.translation_scale;
+    bmin = w.bins_min;
+    base = (int32_t)w.base;
+    len = (int32_t)w.len;
+    max_S = w.max_S;
+    max_S_index = (int32_t)w.max_S_index;
+    epoch = w.epoch;
+    bulk = (int32_t)w.bulk_visited;
+    status = w.status;
+    method = w.method;
+    if (METH == M_SAD) {
+      too_lo = w.too_lo;
+      too_hi = w.too_hi;
+      lam = w.latest_parameter;
+      min_T = w.min_T;
+      tL = w.tL;
+      tF = w.tF;
+      Ns = w.num_states;
+      highest = w.highest_hist;
+      tmax = w.tmax;
+    }
+    if (METH == M_SAMC || METH == M_WL) t0 = w.t0;
+    if (METH == M_WL) {
+      wl_gamma = w.wl_gamma;
+      wl_Ns = w.wl_num_states;
+      wl_min_e = w.wl_min_energy;
+      wl_lowest = w.wl_lowest_hist;
+      wl_highest = w.wl_highest_hist;
+      wl_total = w.wl_total_hist;
+      wl_len = (int32_t)w.wl_len;
+    }
+    if (SYS == SYS_WCA) has_pressure = w.has_pressure;
+    if (SYS != SYS_ISING) {
+      if (SYS == SYS_LJ) R2 = w.geom * w.geom;
+      L = w.geom;
+      halfL = 0.5 * w.geom;
+    }
+    tF_d = (double)tF;
+    Ns_d = (double)Ns;
+    usc = SYS == SYS_ISING ? 0.0 : uniform_scale(-tscale, tscale);
+    cur_p = -1;
+    if (!status) {
+      refresh_indices();
+      reset_checks();
+    }
+  }
+
+  // Write back the walker state; `all` also writes the fields a method
+  // family does not use (only from_params needs that).
+  SADMC_DF void store(int walker, bool all = false) {
     if (lane == 0) {
-      A.ws[walker] = st;
-      if (st.status) *A.flag = 1;
+      sadmc_walker_state &w = A.ws[walker];
+      w.rng[0] = s0;
+      w.rng[1] = s1;
+      w.E = E;
+      w.error = err;
+      w.moves = moves;
+      w.accepted_moves = accepted;
+      w.acceptance_rate = acc_rate;
+      w.translation_scale = tscale;
+      w.bins_min = bmin;
+      w.bins_width = A.width;
+      w.base = base;
+      w.len = len;
+      w.max_S = max_S;
+      w.max_S_index = max_S_index;
+      w.epoch = epoch;
+      w.bulk_visited = (uint64_t)bulk;
+      w.status = status;
+      w.method = method;
+      if (all || METH == M_SAD) {
+        w.too_lo = too_lo;
+        w.too_hi = too_hi;
+        w.latest_parameter = lam;
+        w.min_T = min_T;
+        w.tL = tL;
+        w.tF = tF;
+        w.num_states = Ns;
+        w.highest_hist = highest;
+        w.tmax = tmax;
+      }
+      if (all || METH == M_SAMC || METH == M_WL) w.t0 = t0;
+      if (all || METH == M_WL) {
+        w.wl_gamma = wl_gamma;
+        w.wl_num_states = wl_Ns;
+        w.wl_min_energy = wl_min_e;
+        w.wl_lowest_hist = wl_lowest;
+        w.wl_highest_hist = wl_highest;
+        w.wl_total_hist = wl_total;
+        w.wl_len = wl_len;
+      }
+      if (all || SYS == SYS_WCA) w.has_pressure = has_pressure;
+      if (status) *A.flag = 1;
     }
     if (SYS == SYS_ISING) {
       A.spins[walker] = spins;
@@ -133,54 +288,108 @@ struct Engine {
   }
 
   // --------------------------------------------------------------- bins
-  __device__ int64_t e_index(double e) const {
-    double v = (e - st.bins_min) / st.bins_width;
+  // Bins::state_to_index (energy.rs:371-373); `as usize` saturates.
+  SADMC_DF int32_t e_index(double e) const {
+    double v = (e - bmin) / A.width;
     if (!(v > 0.0)) return 0;
-    if (v >= 9.2e18) return INT64_MAX;
-    return (int64_t)v;
+    if (v >= 2.0e9) return 0x7fffffff;
+    return (int32_t)v;
   }
-  __device__ double i_energy(int64_t i) const { return st.bins_min + ((double)i + 0.5) * st.bins_width; }
-  __device__ bool idx_ok(int64_t i) {
-    if (i < 0 || i >= st.len) {
-      st.status = SADMC_WS_INDEX;
+  SADMC_DF double i_energy(int32_t i) const { return bmin + ((double)i + 0.5) * A.width; }
+  SADMC_DF bool idx_ok(int32_t i) {
+    if (i < 0 || i >= len) {
+      status = SADMC_WS_INDEX;
       return false;
     }
     return true;
   }
-  __device__ double bin_lnw(int64_t p) const { return p == cur_p ? cur.lnw : B[p].lnw; }
-  __device__ uint64_t bin_hist(int64_t p) const { return p == cur_p ? cur.histogram : B[p].histogram; }
+  SADMC_DF double lnw_at(int32_t p) const { return p == cur_p ? c.lnw : B[p].lnw; }
+  SADMC_DF uint64_t hist_at(int32_t p) const { return p == cur_p ? c.histogram : B[p].histogram; }
+  SADMC_DF uint64_t wlh_at(int32_t p) const { return p == cur_p ? c.wl_hist : B[p].wl_hist; }
 
-  // energy.rs:400-434.  New slots get have_visited = true (visit_epoch =
-  // epoch) and round_trips = 1; every other field is already zero.
-  __device__ bool prepare_for_state(double e) {
-    while (e < st.bins_min) {
-      if (st.base == 0) {
-        st.status = SADMC_WS_CAPACITY;
-        return false;
+  // Re-derive every cached index from its inputs (after init, resume or a
+  // window change) and reload the values cached at those indices.  An index
+  // past the bins is left for move_once to report where the reference would
+  // index out of bounds.
+  SADMC_DF void refresh_indices() {
+    int32_t i = e_index(E);
+    cur_i = i;
+    if (i < len) {
+      int32_t p = base + i;
+      if (p != cur_p) {
+        copy_bin(c, B[p]);
+        cur_p = p;
       }
-      st.base -= 1;
-      st.len += 1;
-      B[st.base].round_trips = 1;
-      B[st.base].visit_epoch = st.epoch;
-      st.bins_min -= st.bins_width;
+    } else {
+      cur_p = -1;
     }
-    while (e >= st.bins_min + st.bins_width * (double)st.len) {
-      int64_t q = st.base + st.len;
-      if (q >= A.cap) {
-        st.status = SADMC_WS_CAPACITY;
-        return false;
-      }
-      B[q].round_trips = 1;
-      B[q].visit_epoch = st.epoch;
-      st.len += 1;
+    if (METH == M_SAD) {
+      ilo = e_index(too_lo);
+      ihi = e_index(too_hi);
+      lnw_lo = ilo < len ? lnw_at(base + ilo) : 0.0;
+      lnw_hi = ihi < len ? lnw_at(base + ihi) : 0.0;
     }
+  }
+
+  // True when every cached index is inside the bins.
+  SADMC_DF bool indices_valid() const {
+    if (cur_p < 0) return false;
+    if (METH == M_SAD && (ilo >= len || ihi >= len)) return false;
     return true;
   }
 
+  SADMC_DF void reset_checks() {
+    if (SYS != SYS_ISING) {
+      uint64_t P = A.refresh > 0 ? (uint64_t)A.refresh : (uint64_t)len * (uint64_t)len * 1000ULL;
+      next_check = (moves / P + 1) * P;
+      if (SYS == SYS_WCA) {
+        uint64_t Q = (uint64_t)n * (uint64_t)n;
+        next_press = (moves / Q + 1) * Q;
+      }
+    }
+  }
+
+  // energy.rs:400-434.  New slots get have_visited = true (visit_epoch =
+  // epoch) and round_trips = 1; every other field is already zero.
+  SADMC_DF bool prepare_for_state(double e) {
+    bool front = false, grew = false;
+    while (e < bmin) {
+      if (base == 0) {
+        status = SADMC_WS_CAPACITY;
+        return false;
+      }
+      base -= 1;
+      len += 1;
+      B[base].round_trips = 1;
+      B[base].visit_epoch = epoch;
+      bmin -= A.width;
+      front = true;
+    }
+    while (e >= bmin + A.width * (double)len) {
+      int32_t q = base + len;
+      if (q >= A.cap) {
+        status = SADMC_WS_CAPACITY;
+        return false;
+      }
+      B[q].round_trips = 1;
+      B[q].visit_epoch = epoch;
+      len += 1;
+      grew = true;
+    }
+    if (front || (grew && !indices_valid())) refresh_indices();
+    if ((front || grew) && SYS != SYS_ISING && A.refresh <= 0) {
+      uint64_t P = (uint64_t)len * (uint64_t)len * 1000ULL;
+      next_check = (moves / P + 1) * P;
+    }
+    return status == 0;
+  }
+
   // ------------------------------------------------------------ systems
+  SADMC_DF int spin(int b) const { return ((spins >> b) & 1ULL) ? 1 : -1; }
+
   // Full FP64 energy in lane-group order: lane l sums pairs (i, j<i) for its
   // atoms i; the G partial sums meet in the butterfly.
-  __device__ double full_energy() {
+  SADMC_DF double full_energy() {
     if (SYS == SYS_ISING) {
       int N = n;
       double e = 0.0;
@@ -215,7 +424,7 @@ struct Engine {
   }
 
   // wca.rs:202-218 virial, ordered pairs, lane-group order.
-  __device__ double pressure() {
+  SADMC_DF double pressure() {
     double acc = 0.0;
 #pragma unroll
     for (int k = 0; k < KMAX; k++) {
@@ -235,33 +444,31 @@ struct Engine {
     return p / (3.0 * (L * L * L));
   }
 
-  __device__ int spin(int b) const { return ((spins >> b) & 1ULL) ? 1 : -1; }
-
   // ising.rs:104-119.
-  __device__ bool ising_plan(double &e2) {
-    int N = n;
-    int i = (int)gen_range(st.rng[0], st.rng[1], 0, (uint64_t)N);
-    int j = (int)gen_range(st.rng[0], st.rng[1], 0, (uint64_t)N);
+  SADMC_DF bool ising_plan(double &e2) {
+    const int N = n;
+    int i = (int)sample_zone(s0, s1, (uint64_t)N, A.zone_site);
+    int j = (int)sample_zone(s0, s1, (uint64_t)N, A.zone_site);
     int jp = j + 1 == N ? 0 : j + 1;
     int jm = j == 0 ? N - 1 : j - 1;
     int ip = i + 1 == N ? 0 : i + 1;
     int im = i == 0 ? N - 1 : i - 1;
     int nt = spin(i + jp * N) + spin(i + jm * N) + spin(ip + j * N) + spin(im + j * N);
-    double e = st.E - (double)nt * (double)spin(i + j * N) * 2.0;
+    double e = E - (double)nt * (double)spin(i + j * N) * 2.0;
     pw = i + j * N;
     pe = e;
     e2 = e;
     return true;
   }
 
-  // lj.rs:365-374 / wca.rs:341-353 with the cube move; lj.rs:86-105 and
-  // wca.rs:119-140 for the energy difference.
-  __device__ bool particle_plan(double d, double &e2) {
-    int w = (int)uniform_index(st.rng[0], st.rng[1], (uint64_t)n);
-    double sc = uniform_scale(-d, d);
-    double ux = uniform_sample(st.rng[0], st.rng[1], -d, sc);
-    double uy = uniform_sample(st.rng[0], st.rng[1], -d, sc);
-    double uz = uniform_sample(st.rng[0], st.rng[1], -d, sc);
+  // lj.rs:365-374 / wca.rs:341-353 with the cube move (half-width d,
+  // Uniform::new(-d, d) scale sc); lj.rs:86-105 and wca.rs:119-140 for the
+  // energy difference.
+  SADMC_DF bool particle_plan(double d, double sc, double &e2) {
+    int w = (int)sample_zone(s0, s1, (uint64_t)n, A.zone_n);
+    double ux = uniform_sample(s0, s1, -d, sc);
+    double uy = uniform_sample(s0, s1, -d, sc);
+    double uz = uniform_sample(s0, s1, -d, sc);
     double fx = lx[w], fy = ly[w], fz = lz[w];
     double tx, ty, tz;
     if (SYS == SYS_LJ) {
@@ -304,7 +511,7 @@ struct Engine {
         }
       }
     }
-    double e = st.E + group_sum(acc, G);
+    double e = E + group_sum(acc, G);
     if (SYS == SYS_WCA) pdabse = group_sum(accabs, G);
     pw = w;
     ptx = tx;
@@ -315,42 +522,42 @@ struct Engine {
     return true;
   }
 
-  __device__ bool plan_move(double d, double &e2) {
+  SADMC_DF bool plan_move(double d, double sc, double &e2) {
     if (SYS == SYS_ISING) return ising_plan(e2);
-    return particle_plan(d, e2);
+    return particle_plan(d, sc, e2);
   }
 
   // lj.rs:106-123.
-  __device__ void lj_set_energy(double new_e) {
+  SADMC_DF void lj_set_energy(double new_e) {
     double nn = (double)n;
-    double new_error = fabs(new_e) > fabs(st.E) ? fabs(new_e) * 1e-15 * nn : fabs(st.E) * 1e-15 * nn;
-    st.error = new_error + st.error;
-    if (st.error > fabs(new_e) * 1e-14 * nn * nn) {
-      st.error *= 0.0;
-      st.E = full_energy();
+    double new_error = fabs(new_e) > fabs(E) ? fabs(new_e) * 1e-15 * nn : fabs(E) * 1e-15 * nn;
+    err = new_error + err;
+    if (err > fabs(new_e) * 1e-14 * nn * nn) {
+      err *= 0.0;
+      E = full_energy();
     } else {
-      st.E = new_e;
+      E = new_e;
     }
   }
 
   // wca.rs:164-180.
-  __device__ void wca_set_energy(double new_e, double dabse) {
+  SADMC_DF void wca_set_energy(double new_e, double dabse) {
     double nn = (double)n;
     double single = dabse > fabs(new_e) ? 1e-14 * dabse * nn : 1e-14 * fabs(new_e) * nn;
-    st.error += single * nn;
-    if (st.error > fabs(new_e) * 1e-13 * nn * nn) {
-      st.E = full_energy();
-      st.error = 1e-15 * st.E * nn;
+    err += single * nn;
+    if (err > fabs(new_e) * 1e-13 * nn * nn) {
+      E = full_energy();
+      err = 1e-15 * E * nn;
     } else {
-      st.E = new_e;
+      E = new_e;
     }
   }
 
   // ConfirmSystem::confirm: ising.rs:94-101, lj.rs:338-358, wca.rs:276-301.
-  __device__ void confirm() {
+  SADMC_DF void confirm() {
     if (SYS == SYS_ISING) {
       spins ^= (1ULL << pw);
-      st.E = pe;
+      E = pe;
       return;
     }
     int owner = pw & (G - 1);
@@ -370,7 +577,7 @@ struct Engine {
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     if (A.refresh > 0) {
-      st.E = pe;
+      E = pe;
     } else if (SYS == SYS_LJ) {
       lj_set_energy(pe);
     } else {
@@ -379,392 +586,420 @@ struct Engine {
   }
 
   // lj.rs:249-261, wca.rs:237-250.
-  __device__ void verify_energy() {
+  SADMC_DF void verify_energy() {
     double egood = full_energy();
     double nn = (double)n;
-    double expected = SYS == SYS_LJ ? fabs(st.E) * 1e-14 * nn * nn : fabs(st.E) * 1e-13 * nn * nn;
-    if (fabs(egood - st.E) > expected) st.status = SADMC_WS_VERIFY;
+    double expected = SYS == SYS_LJ ? fabs(E) * 1e-14 * nn * nn : fabs(E) * 1e-13 * nn * nn;
+    if (fabs(egood - E) > expected) status = SADMC_WS_VERIFY;
   }
 
   // ------------------------------------------------------------- method
-  // energy.rs:22-38, 799-824.
-  __device__ double gamma() const {
-    switch (st.method) {
-      case SADMC_CANONICAL:
-        return 0.0;
-      case SADMC_SAD: {
-        double lam = st.latest_parameter, t = (double)st.moves, tF = (double)st.tF, ns = (double)st.num_states;
-        if (lam * tF * ns == 0.0) return 0.0;
-        return (lam + t / tF) / (lam + t / ns * (t / tF));
+  // energy.rs:22-38 with t = moves, tF, N_s (energy.rs:799-813).
+  SADMC_DF double sad_gamma() const {
+    if (lam * tF_d * Ns_d == 0.0) return 0.0;
+    return (lam + tm / tF_d) / (lam + tm / Ns_d * (tm / tF_d));
+  }
+
+  // energy.rs:440-512.  `r2` is the record of the proposed bin.
+  SADMC_DF bool reject_move(double e1, double e2, const sadmc_bin &r2) {
+    if (METH == M_SAD) {
+      double lnw1 = e1 < too_lo ? lnw_lo + (e1 - too_lo) / min_T : (e1 > too_hi ? lnw_hi : c.lnw);
+      double lnw2 = e2 < too_lo ? lnw_lo + (e2 - too_lo) / min_T : (e2 > too_hi ? lnw_hi : r2.lnw);
+      bool rejected = lnw2 > lnw1 && gen_f64(s0, s1) > exp(lnw1 - lnw2);
+      if (!rejected && r2.histogram == 0 && e2 < too_hi && e2 > too_lo) {
+        lam = (too_hi - too_lo) / min_T;
+        Ns += 1;
+        Ns_d = (double)Ns;
+        tL = moves;
       }
-      case SADMC_SAMC: {
-        double t = (double)st.moves;
-        return t > st.t0 ? st.t0 / t : 1.0;
-      }
-      default:
-        return st.wl_gamma;
+      return rejected;
+    } else if (METH == M_SAMC) {
+      return r2.lnw > c.lnw && gen_f64(s0, s1) > exp(c.lnw - r2.lnw);
+    } else if (METH == M_WL) {
+      bool rejected = r2.lnw > c.lnw && gen_f64(s0, s1) > exp(c.lnw - r2.lnw);
+      if (method != SADMC_SAMC && !rejected && r2.histogram == 0 && wl_lowest > 0) wl_Ns += 1.0;
+      return rejected;
+    } else {
+      if (e1 >= e2) return false;
+      return gen_f64(s0, s1) > exp((e1 - e2) / A.method_param);
     }
   }
 
-  // energy.rs:440-512.
-  __device__ bool reject_move(double e1, double e2) {
-    int64_t i1 = e_index(e1), i2 = e_index(e2);
-    if (!idx_ok(i1) || !idx_ok(i2)) return true;
-    int64_t p1 = st.base + i1, p2 = st.base + i2;
-    switch (st.method) {
-      case SADMC_SAD: {
-        int64_t ilo = e_index(st.too_lo), ihi = e_index(st.too_hi);
-        if (!idx_ok(ilo) || !idx_ok(ihi)) return true;
-        int64_t q1 = e1 < st.too_lo ? st.base + ilo : (e1 > st.too_hi ? st.base + ihi : p1);
-        int64_t q2 = e2 < st.too_lo ? st.base + ilo : (e2 > st.too_hi ? st.base + ihi : p2);
-        double w1 = bin_lnw(q1), w2 = bin_lnw(q2);
-        uint64_t h2 = bin_hist(p2);
-        double lnw1 = e1 < st.too_lo ? w1 + (e1 - st.too_lo) / st.min_T : w1;
-        double lnw2 = e2 < st.too_lo ? w2 + (e2 - st.too_lo) / st.min_T : w2;
-        bool rejected = lnw2 > lnw1 && gen_f64(st.rng[0], st.rng[1]) > exp(lnw1 - lnw2);
-        if (!rejected && h2 == 0 && e2 < st.too_hi && e2 > st.too_lo) {
-          st.latest_parameter = (st.too_hi - st.too_lo) / st.min_T;
-          st.num_states += 1;
-          st.tL = st.moves;
-        }
-        return rejected;
-      }
-      case SADMC_SAMC: {
-        double lnw1 = bin_lnw(p1), lnw2 = bin_lnw(p2);
-        return lnw2 > lnw1 && gen_f64(st.rng[0], st.rng[1]) > exp(lnw1 - lnw2);
-      }
-      case SADMC_WL:
-      case SADMC_INV_T_WL: {
-        double lnw1 = bin_lnw(p1), lnw2 = bin_lnw(p2);
-        uint64_t h2 = bin_hist(p2);
-        bool rejected = lnw2 > lnw1 && gen_f64(st.rng[0], st.rng[1]) > exp(lnw1 - lnw2);
-        if (!rejected && h2 == 0 && st.wl_lowest_hist > 0) st.wl_num_states += 1.0;
-        return rejected;
-      }
-      default:
-        if (e1 >= e2) return false;
-        return gen_f64(st.rng[0], st.rng[1]) > exp((e1 - e2) / A.method_param);
+  // Running max of t_found over logical bins [a, b].
+  SADMC_DF void tmax_scan(int32_t a, int32_t b) {
+    for (int32_t j = a; j <= b; j++) {
+      uint64_t t = B[base + j].t_found;
+      if (t > tmax) tmax = t;
     }
   }
 
-  // Running max of t_found over logical bins [a, b] (physical base + a ..).
-  __device__ void tmax_scan(int64_t a, int64_t b) {
-    for (int64_t j = a; j <= b; j++) {
-      uint64_t t = B[st.base + j].t_found;
-      if (t > st.tmax) st.tmax = t;
-    }
+  SADMC_DF void set_lnw(int32_t p, double v) {
+    B[p].lnw = v;
+    if (p == cur_p) c.lnw = v;
   }
 
-  // energy.rs:514-761 for the current bin (cur, physical cur_p, logical i).
-  __device__ void update_weights(double E, int64_t i) {
-    double gam = gamma();
-    double old = cur.lnw;
-    cur.lnw += gam;
-    const int64_t p = cur_p;
-    bool to_samc = false;
-    double samc_t0 = 0.0;
-    if (st.method == SADMC_SAD) {
-      if (st.too_lo > st.too_hi || E < st.too_lo || E > st.too_hi) cur.lnw = old;
-      B[p].lnw = cur.lnw;
-      if (cur.histogram > st.highest_hist) {
-        st.highest_hist = cur.histogram;
-        if (E > st.too_hi) {
-          int64_t ihi = e_index(st.too_hi);
-          if (!idx_ok(ihi)) return;
-          double vhi = bin_lnw(st.base + ihi);
-          int64_t jlo = ihi > 0 ? ihi - 1 : 0, jhi = i + 1 < st.len ? i + 1 : st.len - 1;
-          for (int64_t j = jlo; j <= jhi; j++) {
-            double ej = i_energy(j);
-            if (ej > st.too_hi && ej <= E) {
-              int64_t pj = st.base + j;
-              double v;
-              if (bin_hist(pj) != 0) {
-                v = vhi;
-                st.num_states += 1;
-              } else {
-                v = 0.0;
-              }
-              B[pj].lnw = v;
-              if (pj == p) cur.lnw = v;
-            }
-          }
-          st.latest_parameter = (E - st.too_lo) / st.min_T;
-          st.tL = st.moves;
-          double nh = i_energy(e_index(E));
-          int64_t ihi2 = e_index(nh);
-          if (ihi2 > ihi) tmax_scan(ihi + 1, ihi2 < st.len ? ihi2 : st.len - 1);
-          st.too_hi = nh;
-        } else if (E < st.too_lo) {
-          int64_t ilo = e_index(st.too_lo);
-          if (!idx_ok(ilo)) return;
-          int64_t jlo = i > 0 ? i - 1 : 0, jhi = ilo + 1 < st.len ? ilo + 1 : st.len - 1;
-          for (int64_t j = jlo; j <= jhi; j++) {
-            double ej = i_energy(j);
-            if (ej < st.too_lo && ej >= E) {
-              int64_t pj = st.base + j;
-              double v;
-              if (bin_hist(pj) != 0) {
-                v = bin_lnw(st.base + ilo) + (ej - st.too_lo) / st.min_T;
-                if (v < 0.0) v = 0.0;
-                st.num_states += 1;
-              } else {
-                v = 0.0;
-              }
-              B[pj].lnw = v;
-              if (pj == p) cur.lnw = v;
-            }
-          }
-          st.latest_parameter = (st.too_hi - E) / st.min_T;
-          st.tL = st.moves;
-          double nl = i_energy(e_index(E));
-          int64_t ilo2 = e_index(nl);
-          if (ilo2 < ilo) tmax_scan(ilo2, ilo - 1);
-          st.too_lo = nl;
-        }
-      }
-      if (st.tL == st.moves) {
-        uint64_t old_tF = st.tF;
-        st.tF = st.tmax;
-        if (old_tF != st.tF && A.move_plan == SADMC_ACCEPTANCE_RATE) {
-          double s = st.acceptance_rate / A.move_param;
-          s = s < 0.8 ? 0.8 : (s > 1.2 ? 1.2 : s);
-          st.translation_scale *= s;
-        }
-      }
-    } else if (st.method == SADMC_WL || st.method == SADMC_INV_T_WL) {
-      B[p].lnw = cur.lnw;
-      const bool inv_t = st.method == SADMC_INV_T_WL;
-      const bool has_mg = !inv_t && A.has_min_gamma;
-      if (has_mg && st.wl_gamma < A.method_param) {
-        if (i >= st.wl_len) {
-          st.status = SADMC_WS_INDEX;
+  // energy.rs:514-636 for the current bin (logical i = cur_i).
+  SADMC_DF void update_sad(double E_, int32_t i) {
+    const double E = E_;
+    if (!(too_lo > too_hi || E < too_lo || E > too_hi)) {
+      c.lnw += sad_gamma();
+      if (cur_p == base + ilo) lnw_lo = c.lnw;
+      if (cur_p == base + ihi) lnw_hi = c.lnw;
+    }
+    if (c.histogram > highest) {
+      highest = c.histogram;
+      if (E > too_hi) {
+        if (ihi >= len) {
+          status = SADMC_WS_INDEX;
           return;
         }
-        cur.wl_hist += 1;
-        B[p].wl_hist = cur.wl_hist;
-        return;
-      }
-      if (st.wl_len != st.len) {
-        if (st.wl_len == 0 || (st.wl_gamma != 1.0 && st.wl_lowest_hist > 0)) {
-          st.wl_gamma = 1.0;
-          st.wl_lowest_hist = 0;
-          st.wl_highest_hist = 0;
-          st.wl_total_hist = 0;
-          for (int64_t q = 0; q < st.len; q++) B[st.base + q].wl_hist = 0;
-          cur.wl_hist = 0;
-          st.wl_len = st.len;
-          st.wl_min_energy = st.bins_min;
-        } else {
-          while (st.wl_min_energy > st.bins_min) {
-            st.wl_min_energy -= st.bins_width;
-            st.wl_len += 1;
-          }
-          while (st.wl_len < st.len) st.wl_len += 1;
-          uint64_t mn = ~0ULL;
-          for (int64_t q = 0; q < st.wl_len; q++) {
-            uint64_t h = B[st.base + q].wl_hist;
-            if (h < mn) mn = h;
-          }
-          st.wl_lowest_hist = mn;
-        }
-      }
-      if (i >= st.wl_len) {
-        st.status = SADMC_WS_INDEX;
-        return;
-      }
-      cur.wl_hist += 1;
-      B[p].wl_hist = cur.wl_hist;
-      if (cur.wl_hist > st.wl_highest_hist) st.wl_highest_hist = cur.wl_hist;
-      st.wl_total_hist += 1;
-      double max_energy = st.wl_min_energy + (double)st.wl_len * st.bins_width;
-      if (cur.wl_hist == st.wl_lowest_hist + 1 && st.wl_len > 1 && (!A.has_min_e || A.min_e >= st.wl_min_energy) &&
-          (!A.has_max_e || A.max_e <= max_energy)) {
-        bool found = false;
-        uint64_t mn = ~0ULL;
-        for (int64_t q = 0; q < st.wl_len; q++) {
-          if (B[st.base + q].histogram != 0) {
-            found = true;
-            uint64_t h = B[st.base + q].wl_hist;
-            if (h < mn) mn = h;
+        const double vhi = lnw_hi;
+        int32_t jlo = ihi > 0 ? ihi - 1 : 0, jhi = i + 1 < len ? i + 1 : len - 1;
+        for (int32_t j = jlo; j <= jhi; j++) {
+          double ej = i_energy(j);
+          if (ej > too_hi && ej <= E) {
+            int32_t pj = base + j;
+            if (hist_at(pj) != 0) {
+              set_lnw(pj, vhi);
+              Ns += 1;
+            } else {
+              set_lnw(pj, 0.0);
+            }
           }
         }
-        if (found && mn == st.wl_lowest_hist + 1) {
-          st.wl_lowest_hist = cur.wl_hist;
-          if ((inv_t && st.wl_lowest_hist > 0) ||
-              (double)st.wl_lowest_hist >= 0.8 * (double)st.wl_total_hist / st.wl_num_states) {
-            st.wl_gamma *= 0.5;
-            for (int64_t q = 0; q < st.wl_len; q++) B[st.base + q].wl_hist = 0;
-            cur.wl_hist = 0;
-            st.wl_total_hist = 0;
-            st.wl_lowest_hist = 0;
-            st.wl_highest_hist = 0;
-            if (has_mg && st.wl_gamma < A.method_param) st.wl_gamma = 0.0;
-          }
-          if (inv_t && st.wl_gamma < st.wl_num_states / (double)st.moves) {
-            to_samc = true;
-            samc_t0 = st.wl_num_states;
+        lam = (E - too_lo) / min_T;
+        tL = moves;
+        double nh = i_energy(i);
+        int32_t ihi2 = e_index(nh);
+        if (ihi2 > ihi) tmax_scan(ihi + 1, ihi2 < len ? ihi2 : len - 1);
+        too_hi = nh;
+        ihi = ihi2;
+        if (ihi >= len) {
+          status = SADMC_WS_INDEX;
+          return;
+        }
+      } else if (E < too_lo) {
+        if (ilo >= len) {
+          status = SADMC_WS_INDEX;
+          return;
+        }
+        const int32_t plo = base + ilo;
+        int32_t jlo = i > 0 ? i - 1 : 0, jhi = ilo + 1 < len ? ilo + 1 : len - 1;
+        for (int32_t j = jlo; j <= jhi; j++) {
+          double ej = i_energy(j);
+          if (ej < too_lo && ej >= E) {
+            int32_t pj = base + j;
+            double v;
+            if (hist_at(pj) != 0) {
+              v = lnw_lo + (ej - too_lo) / min_T;
+              if (v < 0.0) v = 0.0;
+              Ns += 1;
+            } else {
+              v = 0.0;
+            }
+            set_lnw(pj, v);
+            if (pj == plo) lnw_lo = v;
           }
         }
+        lam = (too_hi - E) / min_T;
+        tL = moves;
+        double nl = i_energy(i);
+        int32_t ilo2 = e_index(nl);
+        if (ilo2 < ilo) tmax_scan(ilo2, ilo - 1);
+        too_lo = nl;
+        ilo = ilo2;
       }
-    } else {
-      B[p].lnw = cur.lnw;
+      Ns_d = (double)Ns;
+      lnw_lo = lnw_at(base + ilo);
+      lnw_hi = lnw_at(base + ihi);
     }
-    if (to_samc) {
-      st.method = SADMC_SAMC;
-      st.t0 = samc_t0;
+    if (tL == moves) {
+      if (ilo >= len || ihi >= len) {
+        status = SADMC_WS_INDEX;
+        return;
+      }
+      uint64_t old_tF = tF;
+      tF = tmax;
+      tF_d = (double)tF;
+      if (old_tF != tF && A.move_plan == SADMC_ACCEPTANCE_RATE) {
+        double s = acc_rate / A.move_param;
+        s = s < 0.8 ? 0.8 : (s > 1.2 ? 1.2 : s);
+        tscale *= s;
+        if (SYS != SYS_ISING) usc = uniform_scale(-tscale, tscale);
+      }
+    }
+  }
+
+  // energy.rs:638-756 (WL and 1/t-WL); SAMC after the 1/t switch.
+  SADMC_DF void update_wl(int32_t i) {
+    if (method == SADMC_SAMC) {
+      double t = tm;
+      c.lnw += t > t0 ? t0 / t : 1.0;
+      return;
+    }
+    c.lnw += wl_gamma;
+    const bool inv_t = method == SADMC_INV_T_WL;
+    const bool has_mg = !inv_t && A.has_min_gamma;
+    if (has_mg && wl_gamma < A.method_param) {
+      if (i >= wl_len) {
+        status = SADMC_WS_INDEX;
+        return;
+      }
+      c.wl_hist += 1;
+      B[cur_p].wl_hist = c.wl_hist;
+      return;
+    }
+    if (wl_len != len) {
+      if (wl_len == 0 || (wl_gamma != 1.0 && wl_lowest > 0)) {
+        wl_gamma = 1.0;
+        wl_lowest = 0;
+        wl_highest = 0;
+        wl_total = 0;
+        for (int32_t q = 0; q < len; q++) B[base + q].wl_hist = 0;
+        c.wl_hist = 0;
+        wl_len = len;
+        wl_min_e = bmin;
+      } else {
+        while (wl_min_e > bmin) {
+          wl_min_e -= A.width;
+          wl_len += 1;
+        }
+        while (wl_len < len) wl_len += 1;
+        uint64_t mn = ~0ULL;
+        for (int32_t q = 0; q < wl_len; q++) {
+          uint64_t h = wlh_at(base + q);
+          if (h < mn) mn = h;
+        }
+        wl_lowest = mn;
+      }
+    }
+    if (i >= wl_len) {
+      status = SADMC_WS_INDEX;
+      return;
+    }
+    c.wl_hist += 1;
+    B[cur_p].wl_hist = c.wl_hist;
+    if (c.wl_hist > wl_highest) wl_highest = c.wl_hist;
+    wl_total += 1;
+    double max_energy = wl_min_e + (double)wl_len * A.width;
+    if (c.wl_hist == wl_lowest + 1 && wl_len > 1 && (!A.has_min_e || A.min_e >= wl_min_e) &&
+        (!A.has_max_e || A.max_e <= max_energy)) {
+      bool found = false;
+      uint64_t mn = ~0ULL;
+      for (int32_t q = 0; q < wl_len; q++) {
+        if (hist_at(base + q) != 0) {
+          found = true;
+          uint64_t h = wlh_at(base + q);
+          if (h < mn) mn = h;
+        }
+      }
+      if (found && mn == wl_lowest + 1) {
+        wl_lowest = c.wl_hist;
+        if ((inv_t && wl_lowest > 0) || (double)wl_lowest >= 0.8 * (double)wl_total / wl_Ns) {
+          wl_gamma *= 0.5;
+          for (int32_t q = 0; q < wl_len; q++) B[base + q].wl_hist = 0;
+          c.wl_hist = 0;
+          wl_total = 0;
+          wl_lowest = 0;
+          wl_highest = 0;
+          if (has_mg && wl_gamma < A.method_param) wl_gamma = 0.0;
+        }
+        if (inv_t && wl_gamma < wl_Ns / tm) {
+          method = SADMC_SAMC;
+          t0 = wl_Ns;
+        }
+      }
     }
   }
 
   // energy.rs:904-974.
-  __device__ void move_once() {
-    st.moves += 1;
-    if (SYS != SYS_ISING) {
+  SADMC_DF void move_once() {
+    moves += 1;
+    tm += 1.0;
+    if (SYS != SYS_ISING && moves == next_check) {
       if (A.refresh > 0) {
-        if (st.moves % (uint64_t)A.refresh == 0) st.E = full_energy();
+        E = full_energy();
+        next_check += (uint64_t)A.refresh;
       } else {
-        uint64_t Lb = (uint64_t)st.len;
-        if (st.moves % (Lb * Lb * 1000ULL) == 0) verify_energy();
+        verify_energy();
+        next_check += (uint64_t)len * (uint64_t)len * 1000ULL;
       }
     }
-    double e1 = st.E;
-    double rs = sqrt(1.0 / (double)st.moves);
-    st.acceptance_rate *= 1.0 - rs;
+    const double e1 = E;
+    int32_t i1 = cur_i;
+    const double rs = sqrt(1.0 / tm);
+    acc_rate *= 1.0 - rs;
     double e2;
-    if (plan_move(st.translation_scale, e2)) {
+    if (plan_move(tscale, usc, e2)) {
       bool oob = false;
       if (A.has_max_e) oob = e2 > A.max_e && e2 > e1;
       if (A.has_min_e) oob = oob || (e2 < A.min_e && e2 < e1);
       if (!oob) {
         if (!prepare_for_state(e2)) return;
-        if (!reject_move(e1, e2)) {
-          st.accepted_moves += 1;
-          st.acceptance_rate += rs;
+        i1 = cur_i;
+        int32_t i2 = e_index(e2);
+        if (!indices_valid() || !idx_ok(i2)) {
+          status = SADMC_WS_INDEX;
+          return;
+        }
+        const int32_t p2 = base + i2;
+        sadmc_bin r2;
+        if (p2 == cur_p)
+          copy_bin(r2, c);
+        else
+          copy_bin(r2, B[p2]);
+        if (!reject_move(e1, e2, r2)) {
+          accepted += 1;
+          acc_rate += rs;
           confirm();
+          int32_t i = (E == e2) ? i2 : e_index(E);
+          if (!idx_ok(i)) return;
+          const int32_t p = base + i;
+          if (p != cur_p) {
+            if (p == p2)
+              copy_bin(c, r2);
+            else
+              copy_bin(c, B[p]);
+            cur_p = p;
+          }
+          cur_i = i;
         }
       }
     }
-    if (st.status) return;
-    double E = st.E;
-    int64_t i = e_index(E);
-    if (!idx_ok(i)) return;
-    int64_t p = st.base + i;
-    if (p != cur_p) {
-      cur = B[p];
-      cur_p = p;
+    if (status) return;
+    if (cur_p < 0) {
+      status = SADMC_WS_INDEX;
+      return;
     }
-    if (cur.histogram == 0) {
-      cur.t_found = st.moves;
-      B[p].t_found = st.moves;
-      if (st.method == SADMC_SAD) {
-        int64_t ilo = e_index(st.too_lo), ihi = e_index(st.too_hi);
-        if (i >= ilo && i <= ihi && st.moves > st.tmax) st.tmax = st.moves;
-      }
+    const double En = E;
+    const int32_t i = cur_i;
+    const int32_t p = cur_p;
+    if (c.histogram == 0) {
+      c.t_found = moves;
+      B[p].t_found = moves;
+      if (METH == M_SAD && i >= ilo && i <= ihi) tmax = moves;
     }
-    cur.histogram += 1;
-    cur.energy_total += E;
-    cur.energy_squared_total += E * E;
-    B[p].histogram = cur.histogram;
-    B[p].energy_total = cur.energy_total;
-    B[p].energy_squared_total = cur.energy_squared_total;
-    if (SYS == SYS_WCA && A.collect_pressure) {
-      if (st.moves % ((uint64_t)n * (uint64_t)n) == 0) {
-        double pv = pressure();
-        st.has_pressure = 1;
-        X[p].count += 1;
-        X[p].total += pv;
-      }
+    c.histogram += 1;
+    c.energy_total += En;
+    c.energy_squared_total += En * En;
+    if (SYS == SYS_WCA && A.collect_pressure && moves == next_press) {
+      double pv = pressure();
+      has_pressure = 1;
+      X[p].count += 1;
+      X[p].total += pv;
+      next_press += (uint64_t)n * (uint64_t)n;
     }
-    update_weights(E, i);
-    if (st.status) return;
-    double li = cur.lnw;
-    if (li > st.max_S) {
-      st.max_S = li;
-      st.max_S_index = i;
-      st.epoch += 1;
-      st.bulk_visited = 1;
-    } else if (i == st.max_S_index) {
-      if (e_index(e1) != i) {
-        st.epoch += 1;
-        st.bulk_visited = 0;
+    if (METH == M_SAD) {
+      update_sad(En, i);
+    } else if (METH == M_SAMC) {
+      c.lnw += tm > t0 ? t0 / tm : 1.0;
+    } else if (METH == M_WL) {
+      update_wl(i);
+    }
+    // hot part of the current record: lnw, energy_total, energy_squared_total, histogram
+    B[p].lnw = c.lnw;
+    B[p].energy_total = c.energy_total;
+    B[p].energy_squared_total = c.energy_squared_total;
+    B[p].histogram = c.histogram;
+    if (status) return;
+    const double li = c.lnw;
+    if (li > max_S) {
+      max_S = li;
+      max_S_index = i;
+      epoch += 1;
+      bulk = 1;
+    } else if (i == max_S_index) {
+      if (i1 != i) {
+        epoch += 1;
+        bulk = 0;
       }
-    } else if (!(st.bulk_visited || cur.visit_epoch == st.epoch)) {
-      cur.visit_epoch = st.epoch;
-      cur.round_trips += 1;
-      B[p].visit_epoch = cur.visit_epoch;
-      B[p].round_trips = cur.round_trips;
+    } else if (!(bulk || c.visit_epoch == epoch)) {
+      c.visit_epoch = epoch;
+      c.round_trips += 1;
+      B[p].visit_epoch = c.visit_epoch;
+      B[p].round_trips = c.round_trips;
     }
   }
 
-  __device__ void run(uint64_t moves) {
-    for (uint64_t k = 0; k < moves; k++) {
-      if (st.status) break;
+  SADMC_DF void run(uint64_t nmoves) {
+    for (uint64_t k = 0; k < nmoves; k++) {
+      if (status) break;
       move_once();
     }
   }
 
   // energy.rs:830-898 after the system exists (initial energy, greedy
   // descent below max_e, one-bin histogram, method state).
-  __device__ void from_params(double e_lo, double width) {
-    st.E = full_energy();
-    st.error = 0.0;
+  SADMC_DF void from_params(int walker, double e_lo) {
+    const sadmc_walker_state &w = A.ws[walker];
+    s0 = w.rng[0];
+    s1 = w.rng[1];
+    if (SYS != SYS_ISING) {
+      if (SYS == SYS_LJ) R2 = w.geom * w.geom;
+      L = w.geom;
+      halfL = 0.5 * w.geom;
+    }
+    status = 0;
+    E = full_energy();
+    err = 0.0;
     if (A.has_max_e) {
+      const double sc = SYS == SYS_ISING ? 0.0 : uniform_scale(-0.05, 0.05);
       for (uint64_t k = 0; k < 100000000ULL; k++) {
         double newe;
-        if (plan_move(0.05, newe)) {
-          if (newe < st.E) confirm();
-          if (st.E < A.max_e) break;
+        if (plan_move(0.05, sc, newe)) {
+          if (newe < E) confirm();
+          if (E < A.max_e) break;
         }
       }
     }
-    double E = st.E;
+    const double width = A.width;
     double emin = (round(E / width) - 0.5) * width;
-    st.bins_width = width;
-    st.bins_min = emin;
-    st.len = 1;
+    bmin = emin;
+    len = 1;
     double b = floor((emin - e_lo) / width) + 2.0;
     if (!(b >= 0.0) || b >= (double)(A.cap - 2)) {
-      st.status = SADMC_WS_CAPACITY;
-      st.base = 0;
-      return;
+      status = SADMC_WS_CAPACITY;
+      base = 0;
+    } else {
+      base = (int32_t)b;
+      sadmc_bin r;
+      r.lnw = 0.0;
+      r.energy_total = E;
+      r.energy_squared_total = E * E;
+      r.histogram = 1;
+      r.t_found = 0;
+      r.round_trips = 1;
+      r.visit_epoch = 0;
+      r.wl_hist = 0;
+      copy_bin(B[base], r);
     }
-    st.base = (int64_t)b;
-    sadmc_bin r;
-    r.lnw = 0.0;
-    r.energy_total = E;
-    r.energy_squared_total = E * E;
-    r.histogram = 1;
-    r.t_found = 0;
-    r.round_trips = 1;
-    r.visit_epoch = 0;
-    r.wl_hist = 0;
-    B[st.base] = r;
-    st.max_S = 0.0;
-    st.max_S_index = 0;
-    st.epoch = 1;
-    st.bulk_visited = 0;
-    st.moves = 0;
-    st.accepted_moves = 0;
-    st.acceptance_rate = 0.5;
-    st.translation_scale = A.move_plan == SADMC_TRANSLATION_SCALE ? A.move_param : 0.05;
-    st.method = A.method;
-    st.min_T = A.method == SADMC_SAD ? A.method_param : 0.0;
-    st.t0 = A.method == SADMC_SAMC ? A.method_param : 0.0;
-    st.too_lo = E;
-    st.too_hi = E;
-    st.tL = 0;
-    st.tF = 0;
-    st.num_states = 1;
-    st.highest_hist = 1;
-    st.latest_parameter = 0.0;
-    st.tmax = 0;
-    st.wl_gamma = 1.0;
-    st.wl_lowest_hist = (A.has_min_e && A.has_max_e) ? 0 : 1;
-    st.wl_highest_hist = 1;
-    st.wl_total_hist = 0;
-    st.wl_num_states = (A.has_min_e && A.has_max_e) ? (A.max_e - A.min_e) / width : 1.0;
-    st.wl_len = 0;
-    st.wl_min_energy = E;
-    st.has_pressure = 0;
+    max_S = 0.0;
+    max_S_index = 0;
+    epoch = 1;
+    bulk = 0;
+    moves = 0;
+    accepted = 0;
+    acc_rate = 0.5;
+    tscale = A.move_plan == SADMC_TRANSLATION_SCALE ? A.move_param : 0.05;
+    method = A.method;
+    min_T = A.method == SADMC_SAD ? A.method_param : 0.0;
+    t0 = A.method == SADMC_SAMC ? A.method_param : 0.0;
+    too_lo = E;
+    too_hi = E;
+    tL = 0;
+    tF = 0;
+    Ns = 1;
+    highest = 1;
+    lam = 0.0;
+    tmax = 0;
+    wl_gamma = 1.0;
+    wl_lowest = (A.has_min_e && A.has_max_e) ? 0 : 1;
+    wl_highest = 1;
+    wl_total = 0;
+    wl_Ns = (A.has_min_e && A.has_max_e) ? (A.max_e - A.min_e) / width : 1.0;
+    wl_len = 0;
+    wl_min_e = E;
+    has_pressure = 0;
   }
 };
 

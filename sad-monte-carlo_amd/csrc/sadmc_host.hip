@@ -1,11 +1,5 @@
-// gfx950 kernels and the C ABI (include/sadmc.h) of the walker engine.
-//
-// Launch shape: one 64-lane wavefront per workgroup; a workgroup holds
-// 64/G walkers of one set, G lanes each (G = 1 for Ising).  Each workgroup
-// carries the LDS mirror of its walkers' positions: (64/G) * 3 * npad doubles.
-// The run kernel is persistent over the K moves of a batch: walker state is
-// read from HBM once, kept in registers for the whole batch, and written back
-// once.
+// Host side of the C ABI (include/sadmc.h): walker sets, device buffers,
+// initial configurations, kernel dispatch and state transfer.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -15,139 +9,29 @@
 #include <string>
 #include <vector>
 
-#include "sadmc_engine.h"
+#include "sadmc_kernels.h"
 
 using namespace sadmc;
 
-// ============================================================== kernels
-
-template <int SYS, int KMAX, int PREC>
-__global__ __launch_bounds__(64) void k_run(SetArgs a, uint64_t moves) {
-  extern __shared__ double lds[];
-  const int G = a.G;
-  const int group = threadIdx.x / G;
-  const int lane = threadIdx.x & (G - 1);
-  const int walker = blockIdx.x * (64 / G) + group;
-  if (walker >= a.W) return;
-  Engine<SYS, KMAX, PREC> e;
-  e.load(a, walker, lane, lds + group * 3 * a.npad);
-  e.run(moves);
-  e.store(walker);
-}
-
-template <int SYS, int KMAX, int PREC>
-__global__ __launch_bounds__(64) void k_init(SetArgs a, double e_lo, double width) {
-  extern __shared__ double lds[];
-  const int G = a.G;
-  const int group = threadIdx.x / G;
-  const int lane = threadIdx.x & (G - 1);
-  const int walker = blockIdx.x * (64 / G) + group;
-  if (walker >= a.W) return;
-  Engine<SYS, KMAX, PREC> e;
-  e.load(a, walker, lane, lds + group * 3 * a.npad);
-  e.from_params(e_lo, width);
-  e.store(walker);
-}
-
-template <int SYS, int KMAX, int PREC>
-__global__ __launch_bounds__(64) void k_plan(SetArgs a, int walker, double scale, Pending *out) {
-  extern __shared__ double lds[];
-  const int G = a.G;
-  if ((int)threadIdx.x >= G) return;
-  Engine<SYS, KMAX, PREC> e;
-  e.load(a, walker, threadIdx.x, lds);
-  double e2 = 0.0;
-  bool some = e.plan_move(scale, e2);
-  if (threadIdx.x == 0) {
-    Pending p;
-    p.some = some ? 1 : 0;
-    p.which = e.pw;
-    p.tx = e.ptx;
-    p.ty = e.pty;
-    p.tz = e.ptz;
-    p.e = e2;
-    p.dabse = e.pdabse;
-    *out = p;
-  }
-  e.store(walker);
-}
-
-template <int SYS, int KMAX, int PREC>
-__global__ __launch_bounds__(64) void k_confirm(SetArgs a, int walker, const Pending *pend) {
-  extern __shared__ double lds[];
-  const int G = a.G;
-  if ((int)threadIdx.x >= G) return;
-  Engine<SYS, KMAX, PREC> e;
-  e.load(a, walker, threadIdx.x, lds);
-  Pending p = *pend;
-  if (p.some) {
-    e.pw = p.which;
-    e.ptx = p.tx;
-    e.pty = p.ty;
-    e.ptz = p.tz;
-    e.pe = p.e;
-    e.pdabse = p.dabse;
-    e.confirm();
-  }
-  e.store(walker);
-}
-
-// ============================================================ dispatch
-
-enum KernelKind { KK_RUN, KK_INIT, KK_PLAN, KK_CONFIRM };
-
-struct LaunchArgs {
-  uint64_t moves;
-  double e_lo, width;
-  int walker;
-  double scale;
-  Pending *pending;
-};
-
-template <int SYS, int KMAX, int PREC>
-static hipError_t launch_t(KernelKind kk, const SetArgs &a, const LaunchArgs &la, hipStream_t s) {
-  const int G = a.G;
-  const int per_block = 64 / G;
-  const size_t lds = SYS == SYS_ISING ? 0 : (size_t)per_block * 3 * a.npad * sizeof(double);
-  const dim3 grid((a.W + per_block - 1) / per_block), block(64);
-  switch (kk) {
-    case KK_RUN:
-      hipLaunchKernelGGL((k_run<SYS, KMAX, PREC>), grid, block, lds, s, a, la.moves);
-      break;
-    case KK_INIT:
-      hipLaunchKernelGGL((k_init<SYS, KMAX, PREC>), grid, block, lds, s, a, la.e_lo, la.width);
-      break;
-    case KK_PLAN:
-      hipLaunchKernelGGL((k_plan<SYS, KMAX, PREC>), dim3(1), block, (size_t)3 * a.npad * sizeof(double), s, a,
-                         la.walker, la.scale, la.pending);
-      break;
-    case KK_CONFIRM:
-      hipLaunchKernelGGL((k_confirm<SYS, KMAX, PREC>), dim3(1), block, (size_t)3 * a.npad * sizeof(double), s, a,
-                         la.walker, la.pending);
-      break;
-  }
-  return hipGetLastError();
-}
-
-template <int SYS, int PREC>
-static hipError_t launch_k(int kmax, KernelKind kk, const SetArgs &a, const LaunchArgs &la, hipStream_t s) {
-  switch (kmax) {
-    case 1: return launch_t<SYS, 1, PREC>(kk, a, la, s);
-    case 2: return launch_t<SYS, 2, PREC>(kk, a, la, s);
-    case 4: return launch_t<SYS, 4, PREC>(kk, a, la, s);
-    case 8: return launch_t<SYS, 8, PREC>(kk, a, la, s);
-    default: return launch_t<SYS, 16, PREC>(kk, a, la, s);
+static int method_family(int method) {
+  switch (method) {
+    case SADMC_SAD: return M_SAD;
+    case SADMC_SAMC: return M_SAMC;
+    case SADMC_WL:
+    case SADMC_INV_T_WL: return M_WL;
+    default: return M_CAN;
   }
 }
 
-static hipError_t launch(int sys, int prec, int kmax, KernelKind kk, const SetArgs &a, const LaunchArgs &la,
-                         hipStream_t s) {
-  if (sys == SYS_ISING) return launch_t<SYS_ISING, 1, 0>(kk, a, la, s);
+static hipError_t launch(int sys, int prec, int method, int kmax, KernelKind kk, const SetArgs &a,
+                         const LaunchArgs &la, hipStream_t s) {
+  const int m = method_family(method);
+  if (sys == SYS_ISING) return launch_ising(m, kk, a, la, s);
   if (sys == SYS_LJ) {
-    if (prec == SADMC_FP32_PAIRS) return launch_k<SYS_LJ, 1>(kmax, kk, a, la, s);
-    return launch_k<SYS_LJ, 0>(kmax, kk, a, la, s);
+    if (prec == SADMC_FP32_PAIRS) return launch_lj32(m, kmax, kk, a, la, s);
+    return launch_lj64(m, kmax, kk, a, la, s);
   }
-  return launch_k<SYS_WCA, 0>(kmax, kk, a, la, s);
+  return launch_wca(m, kmax, kk, a, la, s);
 }
 
 // ============================================================ host side
@@ -480,7 +364,11 @@ static void fill_args(SetHost &s) {
   a.min_e = s.p.min_e;
   a.max_e = s.p.max_e;
   a.rc2 = s.p.rc2;
+  a.width = s.width;
   a.flag = s.d_flag;
+  const uint64_t nn = (uint64_t)s.p.n;
+  a.zone_n = ~0ULL - ((~0ULL - nn + 1) % nn);
+  a.zone_site = (nn << __builtin_clzll(nn)) - 1;
 }
 
 static int check_flags(sadmc_ctx *c) {
@@ -550,8 +438,7 @@ int sadmc_init(sadmc_ctx *c) {
     fill_args(s);
     LaunchArgs la{};
     la.e_lo = s.e_lo;
-    la.width = s.width;
-    HIPCHECK(c, launch(s.p.system, s.p.precision, s.KMAX, KK_INIT, s.args, la, c->stream));
+    HIPCHECK(c, launch(s.p.system, s.p.precision, s.p.method, s.KMAX, KK_INIT, s.args, la, c->stream));
   }
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   c->inited = true;
@@ -566,7 +453,7 @@ int sadmc_run(sadmc_ctx *c, uint64_t moves) {
   for (auto &s : c->sets) {
     LaunchArgs la{};
     la.moves = moves;
-    HIPCHECK(c, launch(s.p.system, s.p.precision, s.KMAX, KK_RUN, s.args, la, c->stream));
+    HIPCHECK(c, launch(s.p.system, s.p.precision, s.p.method, s.KMAX, KK_RUN, s.args, la, c->stream));
   }
   HIPCHECK(c, hipEventRecord(c->ev1, c->stream));
   c->timed = true;
@@ -601,7 +488,7 @@ int sadmc_plan_move(sadmc_ctx *c, int32_t set, int32_t walker, double scale, dou
   la.walker = walker;
   la.scale = scale;
   la.pending = s->d_pending;
-  HIPCHECK(c, launch(s->p.system, s->p.precision, s->KMAX, KK_PLAN, s->args, la, c->stream));
+  HIPCHECK(c, launch(s->p.system, s->p.precision, s->p.method, s->KMAX, KK_PLAN, s->args, la, c->stream));
   Pending p;
   HIPCHECK(c, hipMemcpyAsync(&p, s->d_pending, sizeof p, hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(c, hipStreamSynchronize(c->stream));
@@ -616,7 +503,7 @@ int sadmc_confirm(sadmc_ctx *c, int32_t set, int32_t walker) {
   LaunchArgs la{};
   la.walker = walker;
   la.pending = s->d_pending;
-  HIPCHECK(c, launch(s->p.system, s->p.precision, s->KMAX, KK_CONFIRM, s->args, la, c->stream));
+  HIPCHECK(c, launch(s->p.system, s->p.precision, s->p.method, s->KMAX, KK_CONFIRM, s->args, la, c->stream));
   Pending none{};
   HIPCHECK(c, hipMemcpyAsync(s->d_pending, &none, sizeof none, hipMemcpyHostToDevice, c->stream));
   HIPCHECK(c, hipStreamSynchronize(c->stream));

@@ -1,0 +1,151 @@
+// Kernel templates and per-system launch entry points.
+//
+// Launch shape: one 64-lane wavefront per workgroup; a workgroup holds 64/G
+// walkers of one set, G lanes each (G = 1 for Ising), and the LDS mirror of
+// their positions: (64/G) * 3 * npad doubles.  The run kernel is persistent
+// over the K moves of a batch: walker state is read from HBM once, kept in
+// registers for the whole batch, and written back once.
+#pragma once
+
+#include "sadmc_engine.h"
+
+namespace sadmc {
+
+enum KernelKind { KK_RUN = 0, KK_INIT = 1, KK_PLAN = 2, KK_CONFIRM = 3 };
+
+struct LaunchArgs {
+  uint64_t moves;
+  double e_lo;
+  int walker;
+  double scale;
+  Pending *pending;
+};
+
+// One entry point per system kind; each lives in its own translation unit.
+hipError_t launch_ising(int meth, KernelKind kk, const SetArgs &a, const LaunchArgs &la, hipStream_t s);
+hipError_t launch_lj64(int meth, int kmax, KernelKind kk, const SetArgs &a, const LaunchArgs &la, hipStream_t s);
+hipError_t launch_lj32(int meth, int kmax, KernelKind kk, const SetArgs &a, const LaunchArgs &la, hipStream_t s);
+hipError_t launch_wca(int meth, int kmax, KernelKind kk, const SetArgs &a, const LaunchArgs &la, hipStream_t s);
+
+#ifdef SADMC_KERNEL_TU
+
+template <int SYS, int KMAX, int PREC, int METH>
+__global__ __launch_bounds__(64) void k_run(SetArgs a, uint64_t moves) {
+  extern __shared__ double lds[];
+  const int G = a.G;
+  const int group = threadIdx.x / G;
+  const int lane = threadIdx.x & (G - 1);
+  const int walker = blockIdx.x * (64 / G) + group;
+  if (walker >= a.W) return;
+  Engine<SYS, KMAX, PREC, METH> e;
+  e.load(a, walker, lane, lds + group * 3 * a.npad);
+  e.run(moves);
+  e.store(walker);
+}
+
+template <int SYS, int KMAX, int PREC, int METH>
+__global__ __launch_bounds__(64) void k_init(SetArgs a, double e_lo) {
+  extern __shared__ double lds[];
+  const int G = a.G;
+  const int group = threadIdx.x / G;
+  const int lane = threadIdx.x & (G - 1);
+  const int walker = blockIdx.x * (64 / G) + group;
+  if (walker >= a.W) return;
+  Engine<SYS, KMAX, PREC, METH> e;
+  e.load_system(a, walker, lane, lds + group * 3 * a.npad);
+  e.from_params(walker, e_lo);
+  e.store(walker, true);
+}
+
+template <int SYS, int KMAX, int PREC, int METH>
+__global__ __launch_bounds__(64) void k_plan(SetArgs a, int walker, double scale, Pending *out) {
+  extern __shared__ double lds[];
+  if ((int)threadIdx.x >= a.G) return;
+  Engine<SYS, KMAX, PREC, METH> e;
+  e.load(a, walker, threadIdx.x, lds);
+  double e2 = 0.0;
+  const double sc = SYS == SYS_ISING ? 0.0 : uniform_scale(-scale, scale);
+  bool some = e.plan_move(scale, sc, e2);
+  if (threadIdx.x == 0) {
+    Pending p;
+    p.some = some ? 1 : 0;
+    p.which = e.pw;
+    p.tx = e.ptx;
+    p.ty = e.pty;
+    p.tz = e.ptz;
+    p.e = e2;
+    p.dabse = e.pdabse;
+    *out = p;
+  }
+  e.store(walker);
+}
+
+template <int SYS, int KMAX, int PREC, int METH>
+__global__ __launch_bounds__(64) void k_confirm(SetArgs a, int walker, const Pending *pend) {
+  extern __shared__ double lds[];
+  if ((int)threadIdx.x >= a.G) return;
+  Engine<SYS, KMAX, PREC, METH> e;
+  e.load(a, walker, threadIdx.x, lds);
+  Pending p = *pend;
+  if (p.some) {
+    e.pw = p.which;
+    e.ptx = p.tx;
+    e.pty = p.ty;
+    e.ptz = p.tz;
+    e.pe = p.e;
+    e.pdabse = p.dabse;
+    e.confirm();
+  }
+  e.store(walker);
+}
+
+template <int SYS, int KMAX, int PREC, int METH>
+static hipError_t launch_t(KernelKind kk, const SetArgs &a, const LaunchArgs &la, hipStream_t s) {
+  const int G = a.G;
+  const int per_block = 64 / G;
+  const size_t lds = SYS == SYS_ISING ? 0 : (size_t)per_block * 3 * a.npad * sizeof(double);
+  const size_t lds1 = SYS == SYS_ISING ? 0 : (size_t)3 * a.npad * sizeof(double);
+  const dim3 grid((a.W + per_block - 1) / per_block), block(64);
+  switch (kk) {
+    case KK_RUN:
+      hipLaunchKernelGGL((k_run<SYS, KMAX, PREC, METH>), grid, block, lds, s, a, la.moves);
+      break;
+    case KK_INIT:
+      hipLaunchKernelGGL((k_init<SYS, KMAX, PREC, METH>), grid, block, lds, s, a, la.e_lo);
+      break;
+    case KK_PLAN:
+      hipLaunchKernelGGL((k_plan<SYS, KMAX, PREC, METH>), dim3(1), block, lds1, s, a, la.walker, la.scale,
+                         la.pending);
+      break;
+    case KK_CONFIRM:
+      hipLaunchKernelGGL((k_confirm<SYS, KMAX, PREC, METH>), dim3(1), block, lds1, s, a, la.walker, la.pending);
+      break;
+  }
+  return hipGetLastError();
+}
+
+template <int SYS, int KMAX, int PREC>
+static hipError_t launch_m(int meth, KernelKind kk, const SetArgs &a, const LaunchArgs &la, hipStream_t s) {
+  switch (meth) {
+    case M_SAD: return launch_t<SYS, KMAX, PREC, M_SAD>(kk, a, la, s);
+    case M_SAMC: return launch_t<SYS, KMAX, PREC, M_SAMC>(kk, a, la, s);
+    case M_WL: return launch_t<SYS, KMAX, PREC, M_WL>(kk, a, la, s);
+    default: return launch_t<SYS, KMAX, PREC, M_CAN>(kk, a, la, s);
+  }
+}
+
+template <int SYS, int PREC>
+static hipError_t launch_km(int meth, int kmax, KernelKind kk, const SetArgs &a, const LaunchArgs &la,
+                            hipStream_t s) {
+  switch (kmax) {
+    case 1: return launch_m<SYS, 1, PREC>(meth, kk, a, la, s);
+    case 2: return launch_m<SYS, 2, PREC>(meth, kk, a, la, s);
+    case 4: return launch_m<SYS, 4, PREC>(meth, kk, a, la, s);
+    case 8: return launch_m<SYS, 8, PREC>(meth, kk, a, la, s);
+    default: return launch_m<SYS, 16, PREC>(meth, kk, a, la, s);
+  }
+}
+
+#endif  // SADMC_KERNEL_TU
+
+}  // namespace sadmc
