@@ -87,6 +87,9 @@ def lib():
     L.or_mc_spins.argtypes = [C.c_void_p, C.c_void_p]
     L.or_mc_compute_energy.argtypes = [C.c_void_p]; L.or_mc_compute_energy.restype = C.c_double
     L.or_mc_reference_energy.argtypes = [C.c_void_p]; L.or_mc_reference_energy.restype = C.c_double
+    L.or_mc_plan_move.argtypes = [C.c_void_p, C.c_double, C.POINTER(C.c_double)]; L.or_mc_plan_move.restype = C.c_int
+    L.or_mc_confirm.argtypes = [C.c_void_p]
+    L.or_mc_energy.argtypes = [C.c_void_p]; L.or_mc_energy.restype = C.c_double
     L.or_mc_trace.argtypes = [C.c_void_p, C.c_uint64]
     L.or_mc_trace_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]; L.or_mc_trace_copy.restype = C.c_uint64
     L.or_ising_init_spins.argtypes = [C.c_int32, C.c_int32, C.c_uint64, C.c_void_p]
@@ -208,6 +211,16 @@ class Walker:
         s = np.zeros(self.p.n * self.p.n, np.int8)
         lib().or_mc_spins(self.h, _ptr(s))
         return s
+
+    def plan_move(self, scale):
+        e = C.c_double()
+        return e.value if lib().or_mc_plan_move(self.h, scale, C.byref(e)) else None
+
+    def confirm(self):
+        lib().or_mc_confirm(self.h)
+
+    def energy(self):
+        return lib().or_mc_energy(self.h)
 
     def compute_energy(self):
         return lib().or_mc_compute_energy(self.h)

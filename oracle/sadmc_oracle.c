@@ -1191,6 +1191,12 @@ void or_mc_spins(const or_mc *m, int8_t *s) {
 }
 
 double or_mc_compute_energy(const or_mc *m) { return sys_compute_energy(m, m->G); }
+
+/* MovableSystem::plan_move / ConfirmSystem::confirm on the walker's own
+ * system and RNG (system/mod.rs:101-120). */
+int or_mc_plan_move(or_mc *m, double scale, double *e) { return sys_plan_move(m, scale, e); }
+void or_mc_confirm(or_mc *m) { sys_confirm(m); }
+double or_mc_energy(const or_mc *m) { return m->E; }
 double or_mc_reference_energy(const or_mc *m) { return sys_compute_energy(m, 0); }
 
 /* Trace accessors live on the struct; expose copies. */

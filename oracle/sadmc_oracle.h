@@ -121,6 +121,9 @@ void or_mc_coords(const or_mc *mc, double *coords);
 void or_mc_spins(const or_mc *mc, int8_t *spins);
 double or_mc_compute_energy(const or_mc *mc);     /* order-aware, fp64 */
 double or_mc_reference_energy(const or_mc *mc);   /* reference order */
+int or_mc_plan_move(or_mc *mc, double scale, double *e_new);
+void or_mc_confirm(or_mc *mc);
+double or_mc_energy(const or_mc *mc);
 
 /* Per-move trace: energy after each move and whether it was accepted.
  * or_mc_trace arms a buffer for the next cap moves; or_mc_trace_copy

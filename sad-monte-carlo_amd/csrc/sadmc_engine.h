@@ -40,13 +40,29 @@ struct PExtra {
   double total;
 };
 
+// Bin records are split by when they are needed.  The hot part (16 bytes)
+// is all the accept/reject decision reads; the cold part is read-modify-
+// written one move later, so its load overlaps the next move's pair sums.
+// WL's own histogram lives in a third array that only WL kernels touch.
+struct HotBin {
+  double lnw;
+  uint64_t hist;
+};
+struct ColdBin {
+  double etot, esq;
+  uint64_t rt, vep;
+  uint64_t tfound, pad;
+};
+
 // Arguments of one walker set, passed by value to the kernels (they live in
 // scalar registers).
 struct SetArgs {
   sadmc_walker_state *ws;
   double *pos;       // [W][3][npad]
   uint64_t *spins;   // [W]
-  sadmc_bin *bins;   // [W][cap]
+  HotBin *hot;       // [W][cap]
+  ColdBin *cold;     // [W][cap]
+  uint64_t *wlh;     // [W][cap] WL histogram, or null
   PExtra *extra;     // [W][cap] (WCA pressure) or null
   int32_t *flag;     // set to 1 when any walker of the set stops with a status
   int64_t cap;
@@ -64,8 +80,29 @@ struct Pending {
   double tx, ty, tz, e, dabse;
 };
 
+// Cross-lane move of a double with a DPP pattern (both 32-bit halves).
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  uint64_t b = __builtin_bit_cast(uint64_t, v);
+  int lo = __builtin_amdgcn_mov_dpp((int)(uint32_t)b, CTRL, 0xF, 0xF, false);
+  int hi = __builtin_amdgcn_mov_dpp((int)(uint32_t)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+// Sum over the G lanes of a group, every lane ending with the same value.
+// Steps 1-4 are DPP: quad_perm [1,0,3,2] (xor 1), quad_perm [2,3,0,1]
+// (xor 2), row_half_mirror and row_mirror.  After steps 1-2 the four lanes of
+// a quad hold one value, after step 3 the eight lanes of a half row do, so
+// the mirrors pair each lane with a lane holding exactly the partial sum the
+// xor-4 / xor-8 partner would hold: the result is bit-identical to an xor
+// butterfly.  Wider groups finish with cross-row shuffles.
 __device__ __forceinline__ double group_sum(double v, int G) {
-  for (int off = 1; off < G; off <<= 1) v = v + __shfl_xor(v, off, 64);
+  if (G >= 2) v = v + dpp_f64<0xB1>(v);
+  if (G >= 4) v = v + dpp_f64<0x4E>(v);
+  if (G >= 8) v = v + dpp_f64<0x141>(v);
+  if (G >= 16) v = v + dpp_f64<0x140>(v);
+  if (G >= 32) v = v + __shfl_xor(v, 16, 64);
+  if (G >= 64) v = v + __shfl_xor(v, 32, 64);
   return v;
 }
 
@@ -80,24 +117,15 @@ __device__ __forceinline__ uint64_t sample_zone(uint64_t &s0, uint64_t &s1, uint
 }
 
 
-// Field-wise copies of a bin record (no aggregate memcpy, so the record stays
-// in registers).
-SADMC_HD void copy_bin(sadmc_bin &d, const sadmc_bin &s) {
-  d.lnw = s.lnw;
-  d.energy_total = s.energy_total;
-  d.energy_squared_total = s.energy_squared_total;
-  d.histogram = s.histogram;
-  d.t_found = s.t_found;
-  d.round_trips = s.round_trips;
-  d.visit_epoch = s.visit_epoch;
-  d.wl_hist = s.wl_hist;
-}
 
 template <int SYS, int KMAX, int PREC, int METH>
 struct Engine {
-  SetArgs A;
+  const SetArgs *Ap;  // kernel argument segment (uniform; read with scalar loads)
+  SADMC_DF const SetArgs &args() const { return *Ap; }
   int lane, G, K, n;
-  sadmc_bin *B;
+  HotBin *H;
+  ColdBin *Cb;
+  uint64_t *WH;
   PExtra *X;
   // system
   double *lx, *ly, *lz;
@@ -125,9 +153,17 @@ struct Engine {
   uint64_t wl_lowest, wl_highest, wl_total;
   int32_t wl_len;
   int32_t has_pressure;
-  // current bin (logical cur_i, physical cur_p) and its record
+  // current bin (logical cur_i, physical cur_p): hot record (and WL count)
   int32_t cur_i, cur_p;
-  sadmc_bin c;
+  double c_lnw;
+  uint64_t c_hist, c_wlh;
+  // cold record of bin kp, and the previous move's cold work on it
+  int32_t kp;
+  double k_etot, k_esq;
+  uint64_t k_rt, k_vep;
+  int32_t pend, pend_rt, pend_bulk;
+  double pend_E;
+  uint64_t pend_epoch;
   // periodic triggers
   uint64_t next_check, next_press;
   // pending move
@@ -136,13 +172,17 @@ struct Engine {
 
   // ---------------------------------------------------------------- setup
   SADMC_DF void load_system(const SetArgs &a, int walker, int lane_in_group, double *lds_group) {
-    A = a;
+    Ap = &a;
     lane = lane_in_group;
     G = a.G;
     K = a.K;
     n = a.n;
-    B = a.bins + (int64_t)walker * a.cap;
+    H = a.hot + (int64_t)walker * a.cap;
+    Cb = a.cold + (int64_t)walker * a.cap;
+    WH = a.wlh ? a.wlh + (int64_t)walker * a.cap : nullptr;
     X = a.extra ? a.extra + (int64_t)walker * a.cap : nullptr;
+    kp = -1;
+    pend = 0;
     if (SYS == SYS_ISING) {
       spins = a.spins[walker];
     } else {
@@ -227,8 +267,10 @@ struct Engine {
   // Write back the walker state; `all` also writes the fields a method
   // family does not use (only from_params needs that).
   SADMC_DF void store(int walker, bool all = false) {
+    apply_pending();
+    write_cold();
     if (lane == 0) {
-      sadmc_walker_state &w = A.ws[walker];
+      sadmc_walker_state &w = args().ws[walker];
       w.rng[0] = s0;
       w.rng[1] = s1;
       w.E = E;
@@ -238,7 +280,7 @@ struct Engine {
       w.acceptance_rate = acc_rate;
       w.translation_scale = tscale;
       w.bins_min = bmin;
-      w.bins_width = A.width;
+      w.bins_width = args().width;
       w.base = base;
       w.len = len;
       w.max_S = max_S;
@@ -269,19 +311,19 @@ struct Engine {
         w.wl_len = wl_len;
       }
       if (all || SYS == SYS_WCA) w.has_pressure = has_pressure;
-      if (status) *A.flag = 1;
+      if (status) *args().flag = 1;
     }
     if (SYS == SYS_ISING) {
-      A.spins[walker] = spins;
+      args().spins[walker] = spins;
     } else {
-      double *P = A.pos + (int64_t)walker * 3 * A.npad;
+      double *P = args().pos + (int64_t)walker * 3 * args().npad;
 #pragma unroll
       for (int k = 0; k < KMAX; k++) {
         if (k < K) {
           int j = lane + G * k;
           P[j] = x[k];
-          P[A.npad + j] = y[k];
-          P[2 * A.npad + j] = z[k];
+          P[args().npad + j] = y[k];
+          P[2 * args().npad + j] = z[k];
         }
       }
     }
@@ -290,12 +332,12 @@ struct Engine {
   // --------------------------------------------------------------- bins
   // Bins::state_to_index (energy.rs:371-373); `as usize` saturates.
   SADMC_DF int32_t e_index(double e) const {
-    double v = (e - bmin) / A.width;
+    double v = (e - bmin) / args().width;
     if (!(v > 0.0)) return 0;
     if (v >= 2.0e9) return 0x7fffffff;
     return (int32_t)v;
   }
-  SADMC_DF double i_energy(int32_t i) const { return bmin + ((double)i + 0.5) * A.width; }
+  SADMC_DF double i_energy(int32_t i) const { return bmin + ((double)i + 0.5) * args().width; }
   SADMC_DF bool idx_ok(int32_t i) {
     if (i < 0 || i >= len) {
       status = SADMC_WS_INDEX;
@@ -303,9 +345,49 @@ struct Engine {
     }
     return true;
   }
-  SADMC_DF double lnw_at(int32_t p) const { return p == cur_p ? c.lnw : B[p].lnw; }
-  SADMC_DF uint64_t hist_at(int32_t p) const { return p == cur_p ? c.histogram : B[p].histogram; }
-  SADMC_DF uint64_t wlh_at(int32_t p) const { return p == cur_p ? c.wl_hist : B[p].wl_hist; }
+  SADMC_DF double lnw_at(int32_t p) const { return p == cur_p ? c_lnw : H[p].lnw; }
+  SADMC_DF uint64_t hist_at(int32_t p) const { return p == cur_p ? c_hist : H[p].hist; }
+  SADMC_DF uint64_t wlh_at(int32_t p) const { return p == cur_p ? c_wlh : WH[p]; }
+
+  // Cold record of the current bin: written back when the walker leaves the
+  // bin, loaded for the new bin, and first used one move later.
+  SADMC_DF void write_cold() {
+    if (kp >= 0) {
+      Cb[kp].etot = k_etot;
+      Cb[kp].esq = k_esq;
+      Cb[kp].rt = k_rt;
+      Cb[kp].vep = k_vep;
+    }
+  }
+  SADMC_DF void switch_cold(int32_t p) {
+    write_cold();
+    const ColdBin &q = Cb[p];
+    k_etot = q.etot;
+    k_esq = q.esq;
+    k_rt = q.rt;
+    k_vep = q.vep;
+    kp = p;
+  }
+  // energy.rs:941-943 and 950-965 for the previous move: the energy sums and,
+  // when the round-trip branch reached have_visited, its test and update.
+  SADMC_DF void apply_pending() {
+    if (pend) {
+      k_etot += pend_E;
+      k_esq += pend_E * pend_E;
+      if (pend_rt && !(pend_bulk || k_vep == pend_epoch)) {
+        k_vep = pend_epoch;
+        k_rt += 1;
+      }
+      pend = 0;
+    }
+  }
+  SADMC_DF void set_current(int32_t p, double lnw, uint64_t hist) {
+    c_lnw = lnw;
+    c_hist = hist;
+    if (METH == M_WL) c_wlh = WH[p];
+    cur_p = p;
+    switch_cold(p);
+  }
 
   // Re-derive every cached index from its inputs (after init, resume or a
   // window change) and reload the values cached at those indices.  An index
@@ -317,8 +399,8 @@ struct Engine {
     if (i < len) {
       int32_t p = base + i;
       if (p != cur_p) {
-        copy_bin(c, B[p]);
-        cur_p = p;
+        HotBin hb = H[p];
+        set_current(p, hb.lnw, hb.hist);
       }
     } else {
       cur_p = -1;
@@ -340,7 +422,7 @@ struct Engine {
 
   SADMC_DF void reset_checks() {
     if (SYS != SYS_ISING) {
-      uint64_t P = A.refresh > 0 ? (uint64_t)A.refresh : (uint64_t)len * (uint64_t)len * 1000ULL;
+      uint64_t P = args().refresh > 0 ? (uint64_t)args().refresh : (uint64_t)len * (uint64_t)len * 1000ULL;
       next_check = (moves / P + 1) * P;
       if (SYS == SYS_WCA) {
         uint64_t Q = (uint64_t)n * (uint64_t)n;
@@ -360,24 +442,24 @@ struct Engine {
       }
       base -= 1;
       len += 1;
-      B[base].round_trips = 1;
-      B[base].visit_epoch = epoch;
-      bmin -= A.width;
+      Cb[base].rt = 1;
+      Cb[base].vep = epoch;
+      bmin -= args().width;
       front = true;
     }
-    while (e >= bmin + A.width * (double)len) {
+    while (e >= bmin + args().width * (double)len) {
       int32_t q = base + len;
-      if (q >= A.cap) {
+      if (q >= args().cap) {
         status = SADMC_WS_CAPACITY;
         return false;
       }
-      B[q].round_trips = 1;
-      B[q].visit_epoch = epoch;
+      Cb[q].rt = 1;
+      Cb[q].vep = epoch;
       len += 1;
       grew = true;
     }
     if (front || (grew && !indices_valid())) refresh_indices();
-    if ((front || grew) && SYS != SYS_ISING && A.refresh <= 0) {
+    if ((front || grew) && SYS != SYS_ISING && args().refresh <= 0) {
       uint64_t P = (uint64_t)len * (uint64_t)len * 1000ULL;
       next_check = (moves / P + 1) * P;
     }
@@ -414,7 +496,7 @@ struct Engine {
             } else {
               acc += wca_pair(norm2(min_image(x[k] - lx[j], L, halfL), min_image(y[k] - ly[j], L, halfL),
                                     min_image(z[k] - lz[j], L, halfL)),
-                              A.rc2);
+                              args().rc2);
             }
           }
         }
@@ -435,7 +517,7 @@ struct Engine {
             if (j == i) continue;
             acc += wca_pair_pressure(norm2(min_image(x[k] - lx[j], L, halfL), min_image(y[k] - ly[j], L, halfL),
                                            min_image(z[k] - lz[j], L, halfL)),
-                                     A.rc2);
+                                     args().rc2);
           }
         }
       }
@@ -447,8 +529,8 @@ struct Engine {
   // ising.rs:104-119.
   SADMC_DF bool ising_plan(double &e2) {
     const int N = n;
-    int i = (int)sample_zone(s0, s1, (uint64_t)N, A.zone_site);
-    int j = (int)sample_zone(s0, s1, (uint64_t)N, A.zone_site);
+    int i = (int)sample_zone(s0, s1, (uint64_t)N, args().zone_site);
+    int j = (int)sample_zone(s0, s1, (uint64_t)N, args().zone_site);
     int jp = j + 1 == N ? 0 : j + 1;
     int jm = j == 0 ? N - 1 : j - 1;
     int ip = i + 1 == N ? 0 : i + 1;
@@ -465,7 +547,7 @@ struct Engine {
   // Uniform::new(-d, d) scale sc); lj.rs:86-105 and wca.rs:119-140 for the
   // energy difference.
   SADMC_DF bool particle_plan(double d, double sc, double &e2) {
-    int w = (int)sample_zone(s0, s1, (uint64_t)n, A.zone_n);
+    int w = (int)sample_zone(s0, s1, (uint64_t)n, args().zone_n);
     double ux = uniform_sample(s0, s1, -d, sc);
     double uy = uniform_sample(s0, s1, -d, sc);
     double uz = uniform_sample(s0, s1, -d, sc);
@@ -501,10 +583,10 @@ struct Engine {
           } else {
             double vn = wca_pair(norm2(min_image(x[k] - tx, L, halfL), min_image(y[k] - ty, L, halfL),
                                        min_image(z[k] - tz, L, halfL)),
-                                 A.rc2);
+                                 args().rc2);
             double vo = wca_pair(norm2(min_image(x[k] - fx, L, halfL), min_image(y[k] - fy, L, halfL),
                                        min_image(z[k] - fz, L, halfL)),
-                                 A.rc2);
+                                 args().rc2);
             acc += vn - vo;
             accabs += vn + vo;
           }
@@ -576,7 +658,7 @@ struct Engine {
       lz[pw] = ptz;
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    if (A.refresh > 0) {
+    if (args().refresh > 0) {
       E = pe;
     } else if (SYS == SYS_LJ) {
       lj_set_energy(pe);
@@ -601,12 +683,12 @@ struct Engine {
   }
 
   // energy.rs:440-512.  `r2` is the record of the proposed bin.
-  SADMC_DF bool reject_move(double e1, double e2, const sadmc_bin &r2) {
+  SADMC_DF bool reject_move(double e1, double e2, double r2_lnw, uint64_t r2_hist) {
     if (METH == M_SAD) {
-      double lnw1 = e1 < too_lo ? lnw_lo + (e1 - too_lo) / min_T : (e1 > too_hi ? lnw_hi : c.lnw);
-      double lnw2 = e2 < too_lo ? lnw_lo + (e2 - too_lo) / min_T : (e2 > too_hi ? lnw_hi : r2.lnw);
+      double lnw1 = e1 < too_lo ? lnw_lo + (e1 - too_lo) / min_T : (e1 > too_hi ? lnw_hi : c_lnw);
+      double lnw2 = e2 < too_lo ? lnw_lo + (e2 - too_lo) / min_T : (e2 > too_hi ? lnw_hi : r2_lnw);
       bool rejected = lnw2 > lnw1 && gen_f64(s0, s1) > exp(lnw1 - lnw2);
-      if (!rejected && r2.histogram == 0 && e2 < too_hi && e2 > too_lo) {
+      if (!rejected && r2_hist == 0 && e2 < too_hi && e2 > too_lo) {
         lam = (too_hi - too_lo) / min_T;
         Ns += 1;
         Ns_d = (double)Ns;
@@ -614,40 +696,40 @@ struct Engine {
       }
       return rejected;
     } else if (METH == M_SAMC) {
-      return r2.lnw > c.lnw && gen_f64(s0, s1) > exp(c.lnw - r2.lnw);
+      return r2_lnw > c_lnw && gen_f64(s0, s1) > exp(c_lnw - r2_lnw);
     } else if (METH == M_WL) {
-      bool rejected = r2.lnw > c.lnw && gen_f64(s0, s1) > exp(c.lnw - r2.lnw);
-      if (method != SADMC_SAMC && !rejected && r2.histogram == 0 && wl_lowest > 0) wl_Ns += 1.0;
+      bool rejected = r2_lnw > c_lnw && gen_f64(s0, s1) > exp(c_lnw - r2_lnw);
+      if (method != SADMC_SAMC && !rejected && r2_hist == 0 && wl_lowest > 0) wl_Ns += 1.0;
       return rejected;
     } else {
       if (e1 >= e2) return false;
-      return gen_f64(s0, s1) > exp((e1 - e2) / A.method_param);
+      return gen_f64(s0, s1) > exp((e1 - e2) / args().method_param);
     }
   }
 
   // Running max of t_found over logical bins [a, b].
   SADMC_DF void tmax_scan(int32_t a, int32_t b) {
     for (int32_t j = a; j <= b; j++) {
-      uint64_t t = B[base + j].t_found;
+      uint64_t t = Cb[base + j].tfound;
       if (t > tmax) tmax = t;
     }
   }
 
   SADMC_DF void set_lnw(int32_t p, double v) {
-    B[p].lnw = v;
-    if (p == cur_p) c.lnw = v;
+    H[p].lnw = v;
+    if (p == cur_p) c_lnw = v;
   }
 
   // energy.rs:514-636 for the current bin (logical i = cur_i).
   SADMC_DF void update_sad(double E_, int32_t i) {
     const double E = E_;
     if (!(too_lo > too_hi || E < too_lo || E > too_hi)) {
-      c.lnw += sad_gamma();
-      if (cur_p == base + ilo) lnw_lo = c.lnw;
-      if (cur_p == base + ihi) lnw_hi = c.lnw;
+      c_lnw += sad_gamma();
+      if (cur_p == base + ilo) lnw_lo = c_lnw;
+      if (cur_p == base + ihi) lnw_hi = c_lnw;
     }
-    if (c.histogram > highest) {
-      highest = c.histogram;
+    if (c_hist > highest) {
+      highest = c_hist;
       if (E > too_hi) {
         if (ihi >= len) {
           status = SADMC_WS_INDEX;
@@ -721,8 +803,8 @@ struct Engine {
       uint64_t old_tF = tF;
       tF = tmax;
       tF_d = (double)tF;
-      if (old_tF != tF && A.move_plan == SADMC_ACCEPTANCE_RATE) {
-        double s = acc_rate / A.move_param;
+      if (old_tF != tF && args().move_plan == SADMC_ACCEPTANCE_RATE) {
+        double s = acc_rate / args().move_param;
         s = s < 0.8 ? 0.8 : (s > 1.2 ? 1.2 : s);
         tscale *= s;
         if (SYS != SYS_ISING) usc = uniform_scale(-tscale, tscale);
@@ -734,19 +816,19 @@ struct Engine {
   SADMC_DF void update_wl(int32_t i) {
     if (method == SADMC_SAMC) {
       double t = tm;
-      c.lnw += t > t0 ? t0 / t : 1.0;
+      c_lnw += t > t0 ? t0 / t : 1.0;
       return;
     }
-    c.lnw += wl_gamma;
+    c_lnw += wl_gamma;
     const bool inv_t = method == SADMC_INV_T_WL;
-    const bool has_mg = !inv_t && A.has_min_gamma;
-    if (has_mg && wl_gamma < A.method_param) {
+    const bool has_mg = !inv_t && args().has_min_gamma;
+    if (has_mg && wl_gamma < args().method_param) {
       if (i >= wl_len) {
         status = SADMC_WS_INDEX;
         return;
       }
-      c.wl_hist += 1;
-      B[cur_p].wl_hist = c.wl_hist;
+      c_wlh += 1;
+      WH[cur_p] = c_wlh;
       return;
     }
     if (wl_len != len) {
@@ -755,13 +837,13 @@ struct Engine {
         wl_lowest = 0;
         wl_highest = 0;
         wl_total = 0;
-        for (int32_t q = 0; q < len; q++) B[base + q].wl_hist = 0;
-        c.wl_hist = 0;
+        for (int32_t q = 0; q < len; q++) WH[base + q] = 0;
+        c_wlh = 0;
         wl_len = len;
         wl_min_e = bmin;
       } else {
         while (wl_min_e > bmin) {
-          wl_min_e -= A.width;
+          wl_min_e -= args().width;
           wl_len += 1;
         }
         while (wl_len < len) wl_len += 1;
@@ -777,13 +859,13 @@ struct Engine {
       status = SADMC_WS_INDEX;
       return;
     }
-    c.wl_hist += 1;
-    B[cur_p].wl_hist = c.wl_hist;
-    if (c.wl_hist > wl_highest) wl_highest = c.wl_hist;
+    c_wlh += 1;
+    WH[cur_p] = c_wlh;
+    if (c_wlh > wl_highest) wl_highest = c_wlh;
     wl_total += 1;
-    double max_energy = wl_min_e + (double)wl_len * A.width;
-    if (c.wl_hist == wl_lowest + 1 && wl_len > 1 && (!A.has_min_e || A.min_e >= wl_min_e) &&
-        (!A.has_max_e || A.max_e <= max_energy)) {
+    double max_energy = wl_min_e + (double)wl_len * args().width;
+    if (c_wlh == wl_lowest + 1 && wl_len > 1 && (!args().has_min_e || args().min_e >= wl_min_e) &&
+        (!args().has_max_e || args().max_e <= max_energy)) {
       bool found = false;
       uint64_t mn = ~0ULL;
       for (int32_t q = 0; q < wl_len; q++) {
@@ -794,15 +876,15 @@ struct Engine {
         }
       }
       if (found && mn == wl_lowest + 1) {
-        wl_lowest = c.wl_hist;
+        wl_lowest = c_wlh;
         if ((inv_t && wl_lowest > 0) || (double)wl_lowest >= 0.8 * (double)wl_total / wl_Ns) {
           wl_gamma *= 0.5;
-          for (int32_t q = 0; q < wl_len; q++) B[base + q].wl_hist = 0;
-          c.wl_hist = 0;
+          for (int32_t q = 0; q < wl_len; q++) WH[base + q] = 0;
+          c_wlh = 0;
           wl_total = 0;
           wl_lowest = 0;
           wl_highest = 0;
-          if (has_mg && wl_gamma < A.method_param) wl_gamma = 0.0;
+          if (has_mg && wl_gamma < args().method_param) wl_gamma = 0.0;
         }
         if (inv_t && wl_gamma < wl_Ns / tm) {
           method = SADMC_SAMC;
@@ -817,9 +899,9 @@ struct Engine {
     moves += 1;
     tm += 1.0;
     if (SYS != SYS_ISING && moves == next_check) {
-      if (A.refresh > 0) {
+      if (args().refresh > 0) {
         E = full_energy();
-        next_check += (uint64_t)A.refresh;
+        next_check += (uint64_t)args().refresh;
       } else {
         verify_energy();
         next_check += (uint64_t)len * (uint64_t)len * 1000ULL;
@@ -830,10 +912,12 @@ struct Engine {
     const double rs = sqrt(1.0 / tm);
     acc_rate *= 1.0 - rs;
     double e2;
-    if (plan_move(tscale, usc, e2)) {
+    const bool some = plan_move(tscale, usc, e2);
+    apply_pending();
+    if (some) {
       bool oob = false;
-      if (A.has_max_e) oob = e2 > A.max_e && e2 > e1;
-      if (A.has_min_e) oob = oob || (e2 < A.min_e && e2 < e1);
+      if (args().has_max_e) oob = e2 > args().max_e && e2 > e1;
+      if (args().has_min_e) oob = oob || (e2 < args().min_e && e2 < e1);
       if (!oob) {
         if (!prepare_for_state(e2)) return;
         i1 = cur_i;
@@ -843,12 +927,17 @@ struct Engine {
           return;
         }
         const int32_t p2 = base + i2;
-        sadmc_bin r2;
-        if (p2 == cur_p)
-          copy_bin(r2, c);
-        else
-          copy_bin(r2, B[p2]);
-        if (!reject_move(e1, e2, r2)) {
+        double w2;
+        uint64_t h2;
+        if (p2 == cur_p) {
+          w2 = c_lnw;
+          h2 = c_hist;
+        } else {
+          HotBin hb = H[p2];
+          w2 = hb.lnw;
+          h2 = hb.hist;
+        }
+        if (!reject_move(e1, e2, w2, h2)) {
           accepted += 1;
           acc_rate += rs;
           confirm();
@@ -856,11 +945,12 @@ struct Engine {
           if (!idx_ok(i)) return;
           const int32_t p = base + i;
           if (p != cur_p) {
-            if (p == p2)
-              copy_bin(c, r2);
-            else
-              copy_bin(c, B[p]);
-            cur_p = p;
+            if (p == p2) {
+              set_current(p, w2, h2);
+            } else {
+              HotBin hb = H[p];
+              set_current(p, hb.lnw, hb.hist);
+            }
           }
           cur_i = i;
         }
@@ -874,15 +964,12 @@ struct Engine {
     const double En = E;
     const int32_t i = cur_i;
     const int32_t p = cur_p;
-    if (c.histogram == 0) {
-      c.t_found = moves;
-      B[p].t_found = moves;
+    if (c_hist == 0) {
+      Cb[p].tfound = moves;
       if (METH == M_SAD && i >= ilo && i <= ihi) tmax = moves;
     }
-    c.histogram += 1;
-    c.energy_total += En;
-    c.energy_squared_total += En * En;
-    if (SYS == SYS_WCA && A.collect_pressure && moves == next_press) {
+    c_hist += 1;
+    if (SYS == SYS_WCA && args().collect_pressure && moves == next_press) {
       double pv = pressure();
       has_pressure = 1;
       X[p].count += 1;
@@ -892,19 +979,20 @@ struct Engine {
     if (METH == M_SAD) {
       update_sad(En, i);
     } else if (METH == M_SAMC) {
-      c.lnw += tm > t0 ? t0 / tm : 1.0;
+      c_lnw += tm > t0 ? t0 / tm : 1.0;
     } else if (METH == M_WL) {
       update_wl(i);
     }
-    // hot part of the current record: lnw, energy_total, energy_squared_total, histogram
-    B[p].lnw = c.lnw;
-    B[p].energy_total = c.energy_total;
-    B[p].energy_squared_total = c.energy_squared_total;
-    B[p].histogram = c.histogram;
+    {
+      HotBin hb;
+      hb.lnw = c_lnw;
+      hb.hist = c_hist;
+      H[p] = hb;
+    }
     if (status) return;
-    const double li = c.lnw;
-    if (li > max_S) {
-      max_S = li;
+    int32_t rt_cand = 0;
+    if (c_lnw > max_S) {
+      max_S = c_lnw;
       max_S_index = i;
       epoch += 1;
       bulk = 1;
@@ -913,12 +1001,14 @@ struct Engine {
         epoch += 1;
         bulk = 0;
       }
-    } else if (!(bulk || c.visit_epoch == epoch)) {
-      c.visit_epoch = epoch;
-      c.round_trips += 1;
-      B[p].visit_epoch = c.visit_epoch;
-      B[p].round_trips = c.round_trips;
+    } else {
+      rt_cand = 1;
     }
+    pend = 1;
+    pend_E = En;
+    pend_rt = rt_cand;
+    pend_epoch = epoch;
+    pend_bulk = bulk;
   }
 
   SADMC_DF void run(uint64_t nmoves) {
@@ -931,7 +1021,7 @@ struct Engine {
   // energy.rs:830-898 after the system exists (initial energy, greedy
   // descent below max_e, one-bin histogram, method state).
   SADMC_DF void from_params(int walker, double e_lo) {
-    const sadmc_walker_state &w = A.ws[walker];
+    const sadmc_walker_state &w = args().ws[walker];
     s0 = w.rng[0];
     s1 = w.rng[1];
     if (SYS != SYS_ISING) {
@@ -942,36 +1032,38 @@ struct Engine {
     status = 0;
     E = full_energy();
     err = 0.0;
-    if (A.has_max_e) {
+    if (args().has_max_e) {
       const double sc = SYS == SYS_ISING ? 0.0 : uniform_scale(-0.05, 0.05);
       for (uint64_t k = 0; k < 100000000ULL; k++) {
         double newe;
         if (plan_move(0.05, sc, newe)) {
           if (newe < E) confirm();
-          if (E < A.max_e) break;
+          if (E < args().max_e) break;
         }
       }
     }
-    const double width = A.width;
+    const double width = args().width;
     double emin = (round(E / width) - 0.5) * width;
     bmin = emin;
     len = 1;
     double b = floor((emin - e_lo) / width) + 2.0;
-    if (!(b >= 0.0) || b >= (double)(A.cap - 2)) {
+    if (!(b >= 0.0) || b >= (double)(args().cap - 2)) {
       status = SADMC_WS_CAPACITY;
       base = 0;
     } else {
       base = (int32_t)b;
-      sadmc_bin r;
-      r.lnw = 0.0;
-      r.energy_total = E;
-      r.energy_squared_total = E * E;
-      r.histogram = 1;
-      r.t_found = 0;
-      r.round_trips = 1;
-      r.visit_epoch = 0;
-      r.wl_hist = 0;
-      copy_bin(B[base], r);
+      HotBin hb;
+      hb.lnw = 0.0;
+      hb.hist = 1;
+      H[base] = hb;
+      ColdBin cb;
+      cb.etot = E;
+      cb.esq = E * E;
+      cb.rt = 1;
+      cb.vep = 0;
+      cb.tfound = 0;
+      cb.pad = 0;
+      Cb[base] = cb;
     }
     max_S = 0.0;
     max_S_index = 0;
@@ -980,10 +1072,10 @@ struct Engine {
     moves = 0;
     accepted = 0;
     acc_rate = 0.5;
-    tscale = A.move_plan == SADMC_TRANSLATION_SCALE ? A.move_param : 0.05;
-    method = A.method;
-    min_T = A.method == SADMC_SAD ? A.method_param : 0.0;
-    t0 = A.method == SADMC_SAMC ? A.method_param : 0.0;
+    tscale = args().move_plan == SADMC_TRANSLATION_SCALE ? args().move_param : 0.05;
+    method = args().method;
+    min_T = args().method == SADMC_SAD ? args().method_param : 0.0;
+    t0 = args().method == SADMC_SAMC ? args().method_param : 0.0;
     too_lo = E;
     too_hi = E;
     tL = 0;
@@ -993,10 +1085,10 @@ struct Engine {
     lam = 0.0;
     tmax = 0;
     wl_gamma = 1.0;
-    wl_lowest = (A.has_min_e && A.has_max_e) ? 0 : 1;
+    wl_lowest = (args().has_min_e && args().has_max_e) ? 0 : 1;
     wl_highest = 1;
     wl_total = 0;
-    wl_Ns = (A.has_min_e && A.has_max_e) ? (A.max_e - A.min_e) / width : 1.0;
+    wl_Ns = (args().has_min_e && args().has_max_e) ? (args().max_e - args().min_e) / width : 1.0;
     wl_len = 0;
     wl_min_e = E;
     has_pressure = 0;

@@ -51,7 +51,9 @@ struct SetHost {
   sadmc_walker_state *d_ws = nullptr;
   double *d_pos = nullptr;
   uint64_t *d_spins = nullptr;
-  sadmc_bin *d_bins = nullptr;
+  HotBin *d_hot = nullptr;
+  ColdBin *d_cold = nullptr;
+  uint64_t *d_wlh = nullptr;
   PExtra *d_extra = nullptr;
   Pending *d_pending = nullptr;
   int32_t *d_flag = nullptr;
@@ -177,6 +179,29 @@ static double lj_lowest(int n) {
   }
 }
 
+// Assemble the public 64-byte records of walker w's physical slots
+// [first, first + count) from the hot, cold and WL arrays.
+static int gather_bins(sadmc_ctx *c, const SetHost &s, int w, int64_t first, int64_t count, sadmc_bin *out) {
+  std::vector<HotBin> h(count);
+  std::vector<ColdBin> cb(count);
+  std::vector<uint64_t> wl(count, 0);
+  const size_t off = (size_t)w * s.cap + first;
+  HIPCHECK(c, hipMemcpy(h.data(), s.d_hot + off, sizeof(HotBin) * count, hipMemcpyDeviceToHost));
+  HIPCHECK(c, hipMemcpy(cb.data(), s.d_cold + off, sizeof(ColdBin) * count, hipMemcpyDeviceToHost));
+  if (s.d_wlh) HIPCHECK(c, hipMemcpy(wl.data(), s.d_wlh + off, sizeof(uint64_t) * count, hipMemcpyDeviceToHost));
+  for (int64_t k = 0; k < count; k++) {
+    out[k].lnw = h[k].lnw;
+    out[k].histogram = h[k].hist;
+    out[k].energy_total = cb[k].etot;
+    out[k].energy_squared_total = cb[k].esq;
+    out[k].round_trips = cb[k].rt;
+    out[k].visit_epoch = cb[k].vep;
+    out[k].t_found = cb[k].tfound;
+    out[k].wl_hist = wl[k];
+  }
+  return SADMC_OK;
+}
+
 extern "C" {
 
 int sadmc_abi_version(void) { return SADMC_ABI_VERSION; }
@@ -241,14 +266,18 @@ static void free_set(SetHost &s) {
   (void)hipFree(s.d_ws);
   (void)hipFree(s.d_pos);
   (void)hipFree(s.d_spins);
-  (void)hipFree(s.d_bins);
+  (void)hipFree(s.d_hot);
+  (void)hipFree(s.d_cold);
+  (void)hipFree(s.d_wlh);
   (void)hipFree(s.d_extra);
   (void)hipFree(s.d_pending);
   (void)hipFree(s.d_flag);
   s.d_ws = nullptr;
   s.d_pos = nullptr;
   s.d_spins = nullptr;
-  s.d_bins = nullptr;
+  s.d_hot = nullptr;
+  s.d_cold = nullptr;
+  s.d_wlh = nullptr;
   s.d_extra = nullptr;
   s.d_pending = nullptr;
   s.d_flag = nullptr;
@@ -342,7 +371,9 @@ static void fill_args(SetHost &s) {
   a.ws = s.d_ws;
   a.pos = s.d_pos;
   a.spins = s.d_spins;
-  a.bins = s.d_bins;
+  a.hot = s.d_hot;
+  a.cold = s.d_cold;
+  a.wlh = s.d_wlh;
   a.extra = s.d_extra;
   a.cap = s.cap;
   a.W = s.W;
@@ -403,8 +434,15 @@ int sadmc_init(sadmc_ctx *c) {
   for (auto &s : c->sets) {
     const int W = s.W, n = s.p.n;
     HIPCHECK(c, hipMalloc(&s.d_ws, sizeof(sadmc_walker_state) * W));
-    HIPCHECK(c, hipMalloc(&s.d_bins, sizeof(sadmc_bin) * (size_t)W * s.cap));
-    HIPCHECK(c, hipMemsetAsync(s.d_bins, 0, sizeof(sadmc_bin) * (size_t)W * s.cap, c->stream));
+    const size_t nb = (size_t)W * s.cap;
+    HIPCHECK(c, hipMalloc(&s.d_hot, sizeof(HotBin) * nb));
+    HIPCHECK(c, hipMemsetAsync(s.d_hot, 0, sizeof(HotBin) * nb, c->stream));
+    HIPCHECK(c, hipMalloc(&s.d_cold, sizeof(ColdBin) * nb));
+    HIPCHECK(c, hipMemsetAsync(s.d_cold, 0, sizeof(ColdBin) * nb, c->stream));
+    if (s.p.method == SADMC_WL || s.p.method == SADMC_INV_T_WL) {
+      HIPCHECK(c, hipMalloc(&s.d_wlh, sizeof(uint64_t) * nb));
+      HIPCHECK(c, hipMemsetAsync(s.d_wlh, 0, sizeof(uint64_t) * nb, c->stream));
+    }
     HIPCHECK(c, hipMalloc(&s.d_pending, sizeof(Pending)));
     HIPCHECK(c, hipMalloc(&s.d_flag, sizeof(int32_t)));
     HIPCHECK(c, hipMemsetAsync(s.d_flag, 0, sizeof(int32_t), c->stream));
@@ -535,8 +573,10 @@ int sadmc_walker_bins(sadmc_ctx *c, int32_t set, int32_t walker, int64_t cap, do
   if (rc) return rc;
   if (st.len > cap) return fail(c, SADMC_E_ARG, "output arrays too short for len");
   std::vector<sadmc_bin> b(st.len);
-  HIPCHECK(c, hipMemcpy(b.data(), s->d_bins + (size_t)walker * s->cap + st.base, sizeof(sadmc_bin) * st.len,
-                        hipMemcpyDeviceToHost));
+  {
+    int rc2 = gather_bins(c, *s, walker, st.base, st.len, b.data());
+    if (rc2) return rc2;
+  }
   std::vector<PExtra> x;
   if (s->d_extra) {
     x.resize(st.len);
@@ -588,8 +628,12 @@ int sadmc_download_raw(sadmc_ctx *c, int32_t set, sadmc_walker_state *states, sa
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   if (states)
     HIPCHECK(c, hipMemcpy(states, s.d_ws, sizeof(sadmc_walker_state) * s.W, hipMemcpyDeviceToHost));
-  if (bins)
-    HIPCHECK(c, hipMemcpy(bins, s.d_bins, sizeof(sadmc_bin) * (size_t)s.W * s.cap, hipMemcpyDeviceToHost));
+  if (bins) {
+    for (int w = 0; w < s.W; w++) {
+      int rc = gather_bins(c, s, w, 0, s.cap, bins + (size_t)w * s.cap);
+      if (rc) return rc;
+    }
+  }
   return SADMC_OK;
 }
 
@@ -601,8 +645,26 @@ int sadmc_upload_raw(sadmc_ctx *c, int32_t set, const sadmc_walker_state *states
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   if (states)
     HIPCHECK(c, hipMemcpy(s.d_ws, states, sizeof(sadmc_walker_state) * s.W, hipMemcpyHostToDevice));
-  if (bins)
-    HIPCHECK(c, hipMemcpy(s.d_bins, bins, sizeof(sadmc_bin) * (size_t)s.W * s.cap, hipMemcpyHostToDevice));
+  if (bins) {
+    const size_t nb = (size_t)s.W * s.cap;
+    std::vector<HotBin> h(nb);
+    std::vector<ColdBin> cb(nb);
+    std::vector<uint64_t> wl(nb);
+    for (size_t k = 0; k < nb; k++) {
+      h[k].lnw = bins[k].lnw;
+      h[k].hist = bins[k].histogram;
+      cb[k].etot = bins[k].energy_total;
+      cb[k].esq = bins[k].energy_squared_total;
+      cb[k].rt = bins[k].round_trips;
+      cb[k].vep = bins[k].visit_epoch;
+      cb[k].tfound = bins[k].t_found;
+      cb[k].pad = 0;
+      wl[k] = bins[k].wl_hist;
+    }
+    HIPCHECK(c, hipMemcpy(s.d_hot, h.data(), sizeof(HotBin) * nb, hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(s.d_cold, cb.data(), sizeof(ColdBin) * nb, hipMemcpyHostToDevice));
+    if (s.d_wlh) HIPCHECK(c, hipMemcpy(s.d_wlh, wl.data(), sizeof(uint64_t) * nb, hipMemcpyHostToDevice));
+  }
   if (coords && s.p.system != SADMC_ISING) {
     std::vector<double> pos((size_t)s.W * 3 * s.npad, 0.0);
     for (int w = 0; w < s.W; w++)

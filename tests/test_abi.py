@@ -1,0 +1,94 @@
+"""The C-ABI library on the CPU: it loads without a GPU, exports every
+function include/sadmc.h declares, its struct layouts match the Python
+mirrors, its host-side initial configurations equal the oracle's, and it
+refuses to run without a gfx950 device instead of falling back to the CPU."""
+import os
+import re
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import oracle as O
+import sadmc_amd as S
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "sadmc.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(sadmc_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_function():
+    lib = S.lib()
+    names = declared_functions()
+    assert len(names) >= 20
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(names) == set(S.EXPORTS)
+    assert lib.sadmc_abi_version() == 1
+
+
+def test_struct_layouts_match_python_mirrors():
+    src = r"""
+#include <stdio.h>
+#include <stddef.h>
+#include "sadmc.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu\n", sizeof(sadmc_walker_state), sizeof(sadmc_bin), sizeof(sadmc_set_params),
+         offsetof(sadmc_walker_state, cap), offsetof(sadmc_set_params, move_kind));
+  return 0;
+}
+"""
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "sizes.c")
+        exe = os.path.join(d, "sizes")
+        open(c, "w").write(src)
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", exe, c], check=True)
+        out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()
+    ws, b, sp, cap_off, mk_off = map(int, out)
+    assert ws == S.WALKER_STATE_DTYPE.itemsize
+    assert b == S.BIN_DTYPE.itemsize == 64
+    assert sp == __import__("ctypes").sizeof(S.SetParams)
+    assert cap_off == S.WALKER_STATE_DTYPE.fields["cap"][1]
+    assert mk_off == S.SetParams.move_kind.offset
+
+
+@pytest.mark.parametrize("system,n,geom", [(S.LJ, 13, 2.5), (S.LJ, 31, 3.0), (S.LJ, 38, 3.2),
+                                           (S.WCA, 32, 3.5), (S.WCA, 64, 4.2), (S.WCA, 128, 5.5),
+                                           (S.WCA, 256, 6.8)])
+def test_host_initial_configurations_equal_oracle(system, n, geom):
+    for seed in (0, 1, 17, 2 ** 40 + 3):
+        got = S.init_coords(system, n, geom, seed)
+        want = O.lj_init(n, geom, seed) if system == S.LJ else O.wca_init(n, geom, seed)
+        np.testing.assert_array_equal(got, want)
+
+
+def test_host_initial_spins_equal_oracle():
+    for rule in (0, 1):
+        for seed in (0, 5, 99):
+            np.testing.assert_array_equal(S.init_spins(8, rule, seed), O.ising_spins(8, rule, seed))
+
+
+def test_bin_window_covers_the_configurations():
+    lo, hi, cap = S.bin_window(S.make_params(system=S.ISING, n=8, energy_bin=0.0))
+    assert lo == -128 and hi == 128 and cap >= 64
+    lo, hi, cap = S.bin_window(S.make_params(system=S.LJ, n=31, energy_bin=0.01, has_max_e=1, max_e=0.0))
+    assert lo < -133.586422 and hi == 0.0 and cap >= (hi - lo) / 0.01
+
+
+def test_no_cpu_fallback_without_device():
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU node is present")
+    with pytest.raises(S.SadmcError) as e:
+        S.Context(0)
+    assert e.value.code == -7
+
+
+def test_bench_refuses_more_than_one_gpu():
+    r = subprocess.run(["python3", os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True)
+    assert r.returncode == 2 and "one GPU" in r.stderr
