@@ -158,6 +158,22 @@ int sadmc_sync(sadmc_ctx *ctx);
  * stream around the kernel). */
 int sadmc_last_run_ms(const sadmc_ctx *ctx, double *ms);
 
+/* ---- mixed batches (walker sets of different systems and sizes) ------
+ * One persistent launch runs every set (SAD, LJ/WCA, FP64 pair terms) for
+ * `moves` moves: workgroups map to (set, walker range) tiles, heaviest
+ * first.  Results are bit-identical to sadmc_run.  No reference counterpart
+ * (the reference runs one walker per process, src/bin/histogram.rs:6-11). */
+int sadmc_run_batched(sadmc_ctx *ctx, uint64_t moves);
+/* The same launch captured once per move count in a HIP graph and replayed. */
+int sadmc_graph_run(sadmc_ctx *ctx, uint64_t moves);
+/* Lanes per walker: times `trial_moves` moves (after `warm_moves`) of a
+ * scratch set for each power-of-two lane count that fits (<= 16 atoms per
+ * lane) and returns the fastest.  rates[k] = moves/s at 2^k lanes (k = 0..6,
+ * 0 where not tried); may be NULL. */
+int sadmc_autotune_lanes(int device, const sadmc_set_params *p, int32_t n_walkers, const uint64_t *seeds,
+                         const double *geom, uint64_t warm_moves, uint64_t trial_moves, int32_t *best_lanes,
+                         double *rates);
+
 /* ---- single-walker pair for parity tests: MovableSystem::plan_move and
  * ConfirmSystem::confirm (system/mod.rs:101-120).  plan_move consumes the
  * walker's RNG exactly as inside move_once; *some = 0 means None. */

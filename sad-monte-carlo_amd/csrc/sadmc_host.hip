@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -70,6 +71,13 @@ struct sadmc_ctx {
   bool inited = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool timed = false;
+  // batched launch: per-set arguments and the block -> tile table on the device
+  SetArgs *d_sets = nullptr;
+  BatchTile *d_tiles = nullptr;
+  int ntiles = 0;
+  size_t batch_lds = 0;
+  std::map<uint64_t, hipGraphExec_t> graphs;  // captured batched launch per move count
+  std::vector<hipGraph_t> graph_defs;
 };
 
 static int fail(sadmc_ctx *c, int code, const std::string &msg) {
@@ -288,6 +296,10 @@ void sadmc_destroy(sadmc_ctx *c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto &s : c->sets) free_set(s);
+  for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.second);
+  for (auto &g : c->graph_defs) (void)hipGraphDestroy(g);
+  (void)hipFree(c->d_sets);
+  (void)hipFree(c->d_tiles);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -509,6 +521,129 @@ int sadmc_last_run_ms(const sadmc_ctx *c, double *ms) {
   float f = 0.0f;
   if (hipEventElapsedTime(&f, c->ev0, c->ev1) != hipSuccess) return SADMC_E_HIP;
   *ms = f;
+  return SADMC_OK;
+}
+
+// Tiles of the batched launch, heaviest first: a tile's cost is its walkers
+// times atoms (the pair work of one move).
+static int build_batch(sadmc_ctx *c) {
+  if (c->d_tiles) return SADMC_OK;
+  struct T {
+    double cost;
+    BatchTile t;
+  };
+  std::vector<T> tiles;
+  size_t lds = 0;
+  for (size_t k = 0; k < c->sets.size(); k++) {
+    const SetHost &s = c->sets[k];
+    if (s.p.method != SADMC_SAD || s.p.precision != SADMC_FP64 || s.p.system == SADMC_ISING)
+      return fail(c, SADMC_E_ARG, "the batched launch runs SAD walkers of LJ/WCA sets with FP64 pair terms");
+    const int per_block = 64 / s.G;
+    lds = std::max(lds, (size_t)per_block * 3 * s.npad * sizeof(double));
+    for (int w = 0; w < s.W; w += per_block) {
+      T t;
+      t.cost = (double)std::min(per_block, s.W - w) * s.p.n * (s.p.system == SADMC_WCA ? 1.6 : 1.0);
+      t.t.set = (int32_t)k;
+      t.t.first_walker = w;
+      t.t.sys = s.p.system;
+      t.t.pad = 0;
+      tiles.push_back(t);
+    }
+  }
+  std::stable_sort(tiles.begin(), tiles.end(), [](const T &a, const T &b) { return a.cost > b.cost; });
+  std::vector<BatchTile> bt(tiles.size());
+  for (size_t k = 0; k < tiles.size(); k++) bt[k] = tiles[k].t;
+  std::vector<SetArgs> sa(c->sets.size());
+  for (size_t k = 0; k < c->sets.size(); k++) sa[k] = c->sets[k].args;
+  HIPCHECK(c, hipMalloc(&c->d_sets, sizeof(SetArgs) * sa.size()));
+  HIPCHECK(c, hipMemcpy(c->d_sets, sa.data(), sizeof(SetArgs) * sa.size(), hipMemcpyHostToDevice));
+  HIPCHECK(c, hipMalloc(&c->d_tiles, sizeof(BatchTile) * bt.size()));
+  HIPCHECK(c, hipMemcpy(c->d_tiles, bt.data(), sizeof(BatchTile) * bt.size(), hipMemcpyHostToDevice));
+  c->ntiles = (int)bt.size();
+  c->batch_lds = lds;
+  return SADMC_OK;
+}
+
+int sadmc_run_batched(sadmc_ctx *c, uint64_t moves) {
+  if (!c) return SADMC_E_ARG;
+  if (!c->inited) return fail(c, SADMC_E_STATE, "sadmc_init first");
+  HIPCHECK(c, hipSetDevice(c->device));
+  int rc = build_batch(c);
+  if (rc) return rc;
+  HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
+  HIPCHECK(c, launch_batched(c->d_sets, c->d_tiles, c->ntiles, c->batch_lds, moves, c->stream));
+  HIPCHECK(c, hipEventRecord(c->ev1, c->stream));
+  c->timed = true;
+  return SADMC_OK;
+}
+
+int sadmc_graph_run(sadmc_ctx *c, uint64_t moves) {
+  if (!c) return SADMC_E_ARG;
+  if (!c->inited) return fail(c, SADMC_E_STATE, "sadmc_init first");
+  HIPCHECK(c, hipSetDevice(c->device));
+  int rc = build_batch(c);
+  if (rc) return rc;
+  auto it = c->graphs.find(moves);
+  if (it == c->graphs.end()) {
+    hipGraph_t g;
+    hipGraphExec_t ge;
+    HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t le = launch_batched(c->d_sets, c->d_tiles, c->ntiles, c->batch_lds, moves, c->stream);
+    HIPCHECK(c, hipStreamEndCapture(c->stream, &g));
+    HIPCHECK(c, le);
+    HIPCHECK(c, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+    c->graph_defs.push_back(g);
+    it = c->graphs.emplace(moves, ge).first;
+  }
+  HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
+  HIPCHECK(c, hipGraphLaunch(it->second, c->stream));
+  HIPCHECK(c, hipEventRecord(c->ev1, c->stream));
+  c->timed = true;
+  return SADMC_OK;
+}
+
+int sadmc_autotune_lanes(int device, const sadmc_set_params *p, int32_t n_walkers, const uint64_t *seeds,
+                         const double *geom, uint64_t warm_moves, uint64_t trial_moves, int32_t *best_lanes,
+                         double *rates) {
+  if (!p || !seeds || !best_lanes || n_walkers <= 0) return SADMC_E_ARG;
+  if (rates)
+    for (int k = 0; k < 7; k++) rates[k] = 0.0;
+  if (p->system == SADMC_ISING) {
+    *best_lanes = 1;
+    return SADMC_OK;
+  }
+  double best = -1.0;
+  int bestg = 0, last = SADMC_OK;
+  for (int k = 1; k < 7; k++) {
+    const int G = 1 << k;
+    if ((p->n + G - 1) / G > 16) continue;
+    sadmc_set_params q = *p;
+    q.lanes = G;
+    sadmc_ctx *c = nullptr;
+    int rc = sadmc_create(&c, device);
+    if (rc) return rc;
+    rc = sadmc_add_set(c, &q, n_walkers, seeds, geom, nullptr, nullptr, nullptr);
+    if (!rc) rc = sadmc_init(c);
+    if (!rc && warm_moves) rc = sadmc_run(c, warm_moves);
+    if (!rc) rc = sadmc_sync(c);
+    if (!rc) rc = sadmc_run(c, trial_moves);
+    if (!rc) rc = sadmc_sync(c);
+    double ms = 0.0;
+    if (!rc) rc = sadmc_last_run_ms(c, &ms);
+    sadmc_destroy(c);
+    if (rc) {
+      last = rc;
+      continue;
+    }
+    const double rate = (double)n_walkers * (double)trial_moves / (ms * 1e-3);
+    if (rates) rates[k] = rate;
+    if (rate > best) {
+      best = rate;
+      bestg = G;
+    }
+  }
+  if (!bestg) return last ? last : SADMC_E_ARG;
+  *best_lanes = bestg;
   return SADMC_OK;
 }
 

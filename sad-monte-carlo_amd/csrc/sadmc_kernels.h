@@ -21,7 +21,15 @@ struct LaunchArgs {
   Pending *pending;
 };
 
+// One workgroup of the batched launch: walkers [first_walker, first_walker +
+// 64/G) of set `set`, whose system is `sys`.
+struct BatchTile {
+  int32_t set, first_walker, sys, pad;
+};
+
 // One entry point per system kind; each lives in its own translation unit.
+hipError_t launch_batched(const SetArgs *d_sets, const BatchTile *d_tiles, int ntiles, size_t lds_bytes,
+                          uint64_t moves, hipStream_t s);
 hipError_t launch_ising(int meth, KernelKind kk, const SetArgs &a, const LaunchArgs &la, hipStream_t s);
 hipError_t launch_lj64(int meth, int kmax, KernelKind kk, const SetArgs &a, const LaunchArgs &la, hipStream_t s);
 hipError_t launch_lj32(int meth, int kmax, KernelKind kk, const SetArgs &a, const LaunchArgs &la, hipStream_t s);

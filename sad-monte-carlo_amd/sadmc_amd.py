@@ -78,6 +78,7 @@ EXPORTS = [
     "sadmc_run", "sadmc_sync", "sadmc_last_run_ms", "sadmc_plan_move", "sadmc_confirm", "sadmc_walker_count",
     "sadmc_get_walker_state", "sadmc_walker_bins", "sadmc_walker_coords", "sadmc_walker_spins",
     "sadmc_download_raw", "sadmc_upload_raw", "sadmc_init_coords", "sadmc_init_spins", "sadmc_bin_window",
+    "sadmc_run_batched", "sadmc_graph_run", "sadmc_autotune_lanes",
 ]
 
 
@@ -111,6 +112,10 @@ def lib():
     L.sadmc_init_coords.argtypes = [i32, i32, d, u64, vp]
     L.sadmc_init_spins.argtypes = [i32, i32, u64, vp]
     L.sadmc_bin_window.argtypes = [C.POINTER(SetParams), d, C.POINTER(d), C.POINTER(d), C.POINTER(i64)]
+    L.sadmc_run_batched.argtypes = [vp, u64]
+    L.sadmc_graph_run.argtypes = [vp, u64]
+    L.sadmc_autotune_lanes.argtypes = [C.c_int, C.POINTER(SetParams), i32, vp, vp, u64, u64, C.POINTER(i32),
+                                       C.POINTER(d)]
     for name in EXPORTS:
         if name not in ("sadmc_last_error", "sadmc_destroy", "sadmc_abi_version"):
             getattr(L, name).restype = C.c_int
@@ -145,6 +150,19 @@ def init_spins(n, rule, seed):
     if rc:
         raise SadmcError(rc, "init_spins")
     return s
+
+
+def autotune_lanes(params, seeds, geom, warm_moves=500, trial_moves=500, device=0):
+    """Fastest lanes-per-walker for a walker set (see sadmc_autotune_lanes)."""
+    seeds = np.ascontiguousarray(seeds, dtype=np.uint64)
+    g = np.ascontiguousarray(np.broadcast_to(geom, seeds.shape), dtype=np.float64)
+    best = C.c_int32()
+    rates = (C.c_double * 7)()
+    rc = lib().sadmc_autotune_lanes(device, C.byref(params), len(seeds), _p(seeds), _p(g), warm_moves,
+                                    trial_moves, C.byref(best), rates)
+    if rc:
+        raise SadmcError(rc, "autotune_lanes")
+    return best.value, {1 << k: rates[k] for k in range(7) if rates[k] > 0}
 
 
 def bin_window(params, geom=0.0):
@@ -193,6 +211,12 @@ class Context:
 
     def run(self, moves, sync=True):
         self._check(lib().sadmc_run(self.h, moves), "run")
+        if sync:
+            self.sync()
+
+    def run_batched(self, moves, sync=True, graph=False):
+        fn = lib().sadmc_graph_run if graph else lib().sadmc_run_batched
+        self._check(fn(self.h, moves), "graph_run" if graph else "run_batched")
         if sync:
             self.sync()
 

@@ -1,0 +1,110 @@
+"""GPU: the mixed-batch launch (one persistent kernel over walker sets of
+different systems and sizes) and its HIP-graph replay give the same walkers,
+bit for bit, as the per-set launches, which are themselves bit-identical to
+the oracle (test_gpu_parity.py); one walker per system is also checked
+against the oracle directly here."""
+import numpy as np
+import pytest
+
+import sadmc_amd as S
+from conftest import compare_walker, oracle_params
+
+pytestmark = pytest.mark.gpu
+
+
+def _mixed_sets():
+    """LJ clusters of 13 and 19 atoms in spheres, WCA fluids of 32 and 64
+    atoms with the packing fraction drawn per walker from U[0.3, 0.5]."""
+    rng = np.random.default_rng(4)
+    out = []
+    for n, R, lanes in ((13, 2.5, 4), (19, 2.7, 8)):
+        sp = S.make_params(system=S.LJ, n=n, lanes=lanes, method=S.SAD, method_param=0.1, move_param=0.05,
+                           energy_bin=0.01, has_max_e=1, max_e=0.0)
+        out.append((sp, np.arange(40, dtype=np.uint64) + 1000 * n, np.full(40, R)))
+    for n, lanes in ((32, 8), (64, 16)):
+        eta = rng.uniform(0.3, 0.5, 24)
+        box = np.cbrt(n / (6.0 * eta / np.pi))
+        sp = S.make_params(system=S.WCA, n=n, lanes=lanes, method=S.SAD, method_param=0.1, move_param=0.05,
+                           energy_bin=0.05, has_max_e=1, max_e=2.5 * n)
+        out.append((sp, np.arange(24, dtype=np.uint64) + 1000 * n, box))
+    return out
+
+
+def _build(ctx_factory, sets):
+    ctx = ctx_factory()
+    ids = [ctx.add_set(sp, seeds, geom=geom) for sp, seeds, geom in sets]
+    ctx.init()
+    return ctx, ids
+
+
+def _dump(ctx, ids, sets):
+    out = []
+    for sid, (_, seeds, _) in zip(ids, sets):
+        st, bins = ctx.download_raw(sid)
+        coords = np.concatenate([ctx.coords(sid, w) for w in range(len(seeds))])
+        out.append((st, bins, coords))
+    return out
+
+
+def test_batched_and_graph_equal_per_set_launches(gpu_ctx_factory):
+    print("case batched vs per-set", flush=True)
+    sets = _mixed_sets()
+    moves = 4000
+    a, ia = _build(gpu_ctx_factory, sets)
+    a.run(moves)
+    want = _dump(a, ia, sets)
+
+    b, ib = _build(gpu_ctx_factory, sets)
+    b.run_batched(moves // 2)
+    b.run_batched(moves // 2)
+    got_b = _dump(b, ib, sets)
+
+    c, ic = _build(gpu_ctx_factory, sets)
+    for k in range(4):
+        c.run_batched(moves // 4, graph=True)
+        # eager work between replays: read state back, then replay again
+        c.state(ic[k % len(ic)], 0)
+    got_c = _dump(c, ic, sets)
+
+    for k, (w, gb, gc) in enumerate(zip(want, got_b, got_c)):
+        for name, x, y, z in zip(("states", "bins", "coords"), w, gb, gc):
+            assert x.tobytes() == y.tobytes(), f"set {k} {name}: batched differs from per-set"
+            assert x.tobytes() == z.tobytes(), f"set {k} {name}: graph replay differs from per-set"
+        assert np.all(w[0]["status"] == 0)
+        assert np.all(w[0]["moves"] == moves)
+
+
+def test_batched_walkers_match_oracle(gpu_ctx_factory):
+    import oracle as O
+    print("case batched vs oracle", flush=True)
+    sets = _mixed_sets()
+    moves = 3000
+    b, ib = _build(gpu_ctx_factory, sets)
+    b.run_batched(moves)
+    for sid, (sp, seeds, geom) in zip(ib, sets):
+        for w in (0, len(seeds) - 1):
+            orc = O.Walker(oracle_params(sp, seeds[w], float(geom[w])))
+            orc.run(moves)
+            compare_walker(b.state(sid, w), b.bins(sid, w), orc, sp.method, label=f"set {sid} walker {w}")
+            np.testing.assert_array_equal(b.coords(sid, w), orc.coords())
+
+
+def test_batched_refuses_other_methods(gpu_ctx_factory):
+    print("case batched refusal", flush=True)
+    ctx = gpu_ctx_factory()
+    sp = S.make_params(system=S.LJ, n=13, method=S.SAMC, method_param=100.0, energy_bin=0.01, has_max_e=1,
+                       max_e=0.0)
+    ctx.add_set(sp, np.arange(4, dtype=np.uint64), geom=2.5)
+    ctx.init()
+    with pytest.raises(S.SadmcError) as e:
+        ctx.run_batched(10)
+    assert e.value.code == -1
+
+
+def test_autotune_picks_a_measured_lane_count():
+    print("case autotune", flush=True)
+    sp = S.make_params(system=S.LJ, n=38, method=S.SAD, method_param=0.1, move_param=0.05, energy_bin=0.01,
+                       has_max_e=1, max_e=0.0)
+    best, rates = S.autotune_lanes(sp, np.arange(2048, dtype=np.uint64), 3.2, warm_moves=200, trial_moves=200)
+    assert best in rates and set(rates) == {4, 8, 16, 32, 64}  # 38 atoms over 2 lanes is over 16 per lane
+    assert rates[best] == max(rates.values()) > 0
