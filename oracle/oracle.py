@@ -26,6 +26,7 @@ class Params(C.Structure):
         ("radius", C.c_double), ("box", C.c_double), ("rc2", C.c_double),
         ("refresh", C.c_int32), ("order_g", C.c_int32), ("ising_init", C.c_int32),
         ("collect_pressure", C.c_int32), ("seed", C.c_uint64),
+        ("move_kind", C.c_int32), ("drop_hastings", C.c_int32), ("mlp", C.c_void_p),
     ]
 
 
@@ -106,6 +107,15 @@ def lib():
     L.or_wca_energy_ld.restype = C.c_double
     L.or_sad_gamma.argtypes = [C.c_double] * 4; L.or_sad_gamma.restype = C.c_double
     L.or_bench.argtypes = [C.POINTER(Params), C.c_int32, C.c_uint64, C.c_uint64]; L.or_bench.restype = C.c_double
+    vp = C.c_void_p
+    L.or_prop_features.argtypes = [C.c_int32, vp, C.c_int32, C.c_double, C.c_double, C.c_double, vp]
+    L.or_prop_mlp.argtypes = [vp, vp]; L.or_prop_mlp.restype = C.c_float
+    L.or_prop_gaussian.argtypes = [u64p, vp]
+    L.or_prop_lnh.argtypes = [C.c_float, C.c_float, C.c_float]; L.or_prop_lnh.restype = C.c_double
+    L.or_prop_loss_grad.argtypes = [vp, vp, C.c_int64, vp]; L.or_prop_loss_grad.restype = C.c_double
+    L.or_prop_loss_grad_f64.argtypes = [vp, vp, C.c_int64, vp]; L.or_prop_loss_grad_f64.restype = C.c_double
+    L.or_prop_probe.argtypes = [vp, C.c_uint64, C.c_int32, vp]
+    L.or_mc_plan_learned.argtypes = [vp, dp, dp, vp]; L.or_mc_plan_learned.restype = C.c_int
     _LIB = L
     return L
 
@@ -114,13 +124,68 @@ def _ptr(a):
     return a.ctypes.data_as(C.c_void_p) if a is not None else None
 
 
+PROP_NF, PROP_H, PROP_NP, PROP_NS, PROP_R, PROP_REC = 8, 32, 1377, 16, 2, 24
+PROP_SMIN, PROP_SMAX = -4.605170186, 0.0
+_KEEP = {}
+
+
 def make_params(**kw):
+    """Walker parameters; `mlp` (float32[1377]) selects the learned move."""
     p = Params()
     defaults = dict(energy_bin=0.0, move_param=0.05, rc2=2.0 ** (1.0 / 3.0))
     defaults.update(kw)
+    mlp = defaults.pop("mlp", None)
     for k, v in defaults.items():
         setattr(p, k, v)
+    if mlp is not None:
+        w = np.ascontiguousarray(mlp, dtype=np.float32)
+        assert w.shape == (PROP_NP,)
+        _KEEP[id(p)] = w
+        p.mlp = w.ctypes.data
+        p.move_kind = 1
     return p
+
+
+def bf16_round(a):
+    """Round float32 values to bfloat16 (nearest, ties to even), returned as float32."""
+    b = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    b = (b + 0x7FFF + ((b >> 16) & 1)) & 0xFFFF0000
+    return b.astype(np.uint32).view(np.float32)
+
+
+def prop_features(coords, skip, pos):
+    c = np.ascontiguousarray(coords, dtype=np.float64)
+    f = np.zeros(PROP_NF, np.float32)
+    lib().or_prop_features(len(c) // 3, _ptr(c), skip, pos[0], pos[1], pos[2], _ptr(f))
+    return f
+
+
+def prop_mlp(w, f):
+    w = np.ascontiguousarray(w, dtype=np.float32)
+    f = np.ascontiguousarray(f, dtype=np.float32)
+    return lib().or_prop_mlp(_ptr(w), _ptr(f))
+
+
+def prop_gaussian(rng):
+    xi = np.zeros(3, np.float32)
+    lib().or_prop_gaussian(rng.s, _ptr(xi))
+    return xi
+
+
+def prop_lnh(s, s_new, xi2):
+    return lib().or_prop_lnh(s, s_new, xi2)
+
+
+def prop_loss_grad(w, records, f64=False):
+    r = np.ascontiguousarray(records, dtype=np.float32).reshape(-1, PROP_REC)
+    g = np.zeros(PROP_NP)
+    if f64:
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        loss = lib().or_prop_loss_grad_f64(_ptr(w), _ptr(r), len(r), _ptr(g))
+    else:
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        loss = lib().or_prop_loss_grad(_ptr(w), _ptr(r), len(r), _ptr(g))
+    return loss, g
 
 
 class RNG:
@@ -218,6 +283,20 @@ class Walker:
 
     def confirm(self):
         lib().or_mc_confirm(self.h)
+
+    def plan_learned(self):
+        """(e_new, ln H, new position) of a learned move, or None if the wall rejects it."""
+        e, h = C.c_double(), C.c_double()
+        to = np.zeros(3)
+        if not lib().or_mc_plan_learned(self.h, C.byref(e), C.byref(h), _ptr(to)):
+            return None
+        return e.value, h.value, to
+
+    def probe(self, seed, count):
+        """`count` training records (features, J on the scale grid) of this walker."""
+        out = np.zeros((count, PROP_REC), np.float32)
+        lib().or_prop_probe(self.h, seed, count, _ptr(out))
+        return out
 
     def energy(self):
         return lib().or_mc_energy(self.h)

@@ -194,7 +194,7 @@ struct or_mc {
   /* pending change (possible_change) */
   int ch_kind; /* 0 none, 1 move */
   int64_t ch_which;
-  double ch_x, ch_y, ch_z, ch_e, ch_dabse;
+  double ch_x, ch_y, ch_z, ch_e, ch_dabse, ch_lnh;
   /* scripted energies */
   double *script;
   int64_t script_len, script_pos;
@@ -795,10 +795,43 @@ static int idx_ok(or_mc *m, int64_t i) {
   return 1;
 }
 
-/* energy.rs:440-512. */
-static int reject_move(or_mc *m, double e1, double e2) {
+/* Metropolis-Hastings test on the log acceptance ratio lr = (ln w1 - ln w2)
+ * + ln H of the learned move (the reference's tests when ln H = 0). */
+static int reject_lr(or_mc *m, double lr) { return lr < 0.0 && or_gen_f64(m->rng) > exp(lr); }
+
+/* energy.rs:440-512; lnh = ln H of a learned move, 0 for the cube move. */
+static int reject_move(or_mc *m, double e1, double e2, double lnh) {
   int64_t i1 = e_index(m, e1), i2 = e_index(m, e2);
   if (!idx_ok(m, i1) || !idx_ok(m, i2)) return 1;
+  if (m->p.move_kind == 1) {
+    int rejected;
+    switch (m->method) {
+      case OR_SAD: {
+        int64_t ilo = e_index(m, m->too_lo), ihi = e_index(m, m->too_hi);
+        if (!idx_ok(m, ilo) || !idx_ok(m, ihi)) return 1;
+        double lnw1 = e1 < m->too_lo ? m->lnw[ilo] + (e1 - m->too_lo) / m->min_T
+                      : e1 > m->too_hi ? m->lnw[ihi] : m->lnw[i1];
+        double lnw2 = e2 < m->too_lo ? m->lnw[ilo] + (e2 - m->too_lo) / m->min_T
+                      : e2 > m->too_hi ? m->lnw[ihi] : m->lnw[i2];
+        rejected = reject_lr(m, lnw1 - lnw2 + lnh);
+        if (!rejected && m->hist[i2] == 0 && e2 < m->too_hi && e2 > m->too_lo) {
+          m->lambda = (m->too_hi - m->too_lo) / m->min_T;
+          m->Ns += 1;
+          m->tL = m->moves;
+        }
+        return rejected;
+      }
+      case OR_SAMC:
+        return reject_lr(m, m->lnw[i1] - m->lnw[i2] + lnh);
+      case OR_WL:
+      case OR_INV_T_WL:
+        rejected = reject_lr(m, m->lnw[i1] - m->lnw[i2] + lnh);
+        if (!rejected && m->hist[i2] == 0 && m->wl_lowest > 0) m->wl_Ns += 1.0;
+        return rejected;
+      default:
+        return reject_lr(m, (e1 - e2) / m->T + lnh);
+    }
+  }
   switch (m->method) {
     case OR_SAD: {
       int64_t ilo = e_index(m, m->too_lo), ihi = e_index(m, m->too_hi);
@@ -967,6 +1000,259 @@ static void update_weights(or_mc *m, double E) {
 }
 
 /* energy.rs:904-974. */
+/* ---- learned proposal --------------------------------------------------
+ * No reference counterpart.  The definitions are in sadmc_oracle.h. */
+static const float kPropC0 = 0.9f, kPropDC = 1.6f / 7.0f, kPropA = 12.5f; /* 1/(2 w^2), w = 0.2 */
+static const float kPropSMin = -4.605170186f, kPropSMax = 0.0f;          /* ln 0.01, ln 1 */
+
+static void prop_features_xyz(int n, const double *x, const double *y, const double *z, int skip, double px,
+                              double py, double pz, float *f) {
+  for (int k = 0; k < OR_PROP_NF; k++) f[k] = 0.0f;
+  for (int j = 0; j < n; j++) {
+    if (j == skip) continue;
+    float a = (float)(x[j] - px), b = (float)(y[j] - py), c = (float)(z[j] - pz);
+    float r = sqrtf(a * a + b * b + c * c);
+    for (int k = 0; k < OR_PROP_NF; k++) {
+      float d = r - (kPropC0 + (float)k * kPropDC);
+      f[k] += expf(-kPropA * d * d);
+    }
+  }
+}
+
+void or_prop_features(int32_t n, const double *c, int32_t skip, double px, double py, double pz, float *f) {
+  prop_features_xyz(n, c, c + n, c + 2 * n, skip, px, py, pz, f);
+}
+
+/* Forward pass; h1, h2 (32 each) and the unclamped output may be returned. */
+static float prop_forward(const float *w, const float *f, float *h1, float *h2, float *raw) {
+  const float *W1 = w, *b1 = w + 256, *W2 = w + 288, *b2 = w + 1312, *w3 = w + 1344, *b3 = w + 1376;
+  float a1[OR_PROP_H], a2[OR_PROP_H];
+  for (int u = 0; u < OR_PROP_H; u++) {
+    float z = b1[u];
+    for (int k = 0; k < OR_PROP_NF; k++) z += W1[u * OR_PROP_NF + k] * f[k];
+    a1[u] = z > 0.0f ? z : 0.0f;
+  }
+  for (int u = 0; u < OR_PROP_H; u++) {
+    float z = b2[u];
+    for (int k = 0; k < OR_PROP_H; k++) z += W2[u * OR_PROP_H + k] * a1[k];
+    a2[u] = z > 0.0f ? z : 0.0f;
+  }
+  float s = b3[0];
+  for (int u = 0; u < OR_PROP_H; u++) s += w3[u] * a2[u];
+  if (h1) memcpy(h1, a1, sizeof a1);
+  if (h2) memcpy(h2, a2, sizeof a2);
+  if (raw) *raw = s;
+  return s < kPropSMin ? kPropSMin : (s > kPropSMax ? kPropSMax : s);
+}
+
+float or_prop_mlp(const float *w, const float *f) { return prop_forward(w, f, NULL, NULL, NULL); }
+
+void or_prop_gaussian(uint64_t rng[2], float *xi) {
+  float u1 = (float)or_gen_f64(rng), u2 = (float)or_gen_f64(rng);
+  float u3 = (float)or_gen_f64(rng), u4 = (float)or_gen_f64(rng);
+  const float two_pi = 6.283185307f;
+  float r1 = sqrtf(-2.0f * logf(1.0f - u1)), r2 = sqrtf(-2.0f * logf(1.0f - u3));
+  xi[0] = r1 * cosf(two_pi * u2);
+  xi[1] = r1 * sinf(two_pi * u2);
+  xi[2] = r2 * cosf(two_pi * u4);
+}
+
+double or_prop_lnh(float s, float s_new, float xi2) {
+  double ds = (double)s - (double)s_new;
+  return 3.0 * ds + 0.5 * (double)xi2 * (1.0 - exp(2.0 * ds));
+}
+
+/* Sum over the other atoms of the pair-term difference for atom w moving
+ * from f to t, in the walker's summation order. */
+static double lj_delta(or_mc *m, int64_t w, double tx, double ty, double tz, double fx, double fy, double fz) {
+  if (m->G <= 0) {
+    double e = 0.0;
+    for (int j = 0; j < m->n; j++)
+      if (j != w) e += lj_term(m, j, tx, ty, tz, fx, fy, fz);
+    return e;
+  }
+  double acc[64];
+  for (int l = 0; l < m->G; l++) {
+    acc[l] = 0.0;
+    for (int j = l; j < m->n; j += m->G)
+      if (j != w) acc[l] += lj_term(m, j, tx, ty, tz, fx, fy, fz);
+  }
+  return butterfly(acc, m->G);
+}
+
+static int lj_plan_learned(or_mc *m, double *e_out, double *lnh) {
+  int64_t w = (int64_t)or_uniform_index(m->rng, (uint64_t)m->n);
+  double fx = m->x[w], fy = m->y[w], fz = m->z[w];
+  float f[OR_PROP_NF], xi[3];
+  prop_features_xyz(m->n, m->x, m->y, m->z, (int)w, fx, fy, fz, f);
+  float s = prop_forward(m->p.mlp, f, NULL, NULL, NULL);
+  or_prop_gaussian(m->rng, xi);
+  float sig = expf(s);
+  double tx = fx + (double)(sig * xi[0]), ty = fy + (double)(sig * xi[1]), tz = fz + (double)(sig * xi[2]);
+  double r2to = norm2(tx, ty, tz), r2from = norm2(fx, fy, fz);
+  if (r2to > m->R2 && r2to > r2from) return 0;
+  prop_features_xyz(m->n, m->x, m->y, m->z, (int)w, tx, ty, tz, f);
+  float s2 = prop_forward(m->p.mlp, f, NULL, NULL, NULL);
+  double e = m->E + lj_delta(m, w, tx, ty, tz, fx, fy, fz);
+  *lnh = m->p.drop_hastings ? 0.0 : or_prop_lnh(s, s2, xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2]);
+  m->ch_kind = 1; m->ch_which = w; m->ch_x = tx; m->ch_y = ty; m->ch_z = tz; m->ch_e = e; m->ch_lnh = *lnh;
+  *e_out = e;
+  return 1;
+}
+
+int or_mc_plan_learned(or_mc *m, double *e_new, double *lnh, double *to) {
+  if (!lj_plan_learned(m, e_new, lnh)) return 0;
+  to[0] = m->ch_x; to[1] = m->ch_y; to[2] = m->ch_z;
+  return 1;
+}
+
+/* ln w(E) of a SAD walker with the clamped extension (energy.rs:445-463);
+ * returns 0 if E has no bin. */
+static int sad_lnw(const or_mc *m, double e, double *out) {
+  int64_t ilo = e_index(m, m->too_lo), ihi = e_index(m, m->too_hi), i = e_index(m, e);
+  if (ilo < 0 || ilo >= m->len || ihi < 0 || ihi >= m->len) return 0;
+  if (e < m->too_lo) { *out = m->lnw[ilo] + (e - m->too_lo) / m->min_T; return 1; }
+  if (e > m->too_hi) { *out = m->lnw[ihi]; return 1; }
+  if (i < 0 || i >= m->len) return 0;
+  *out = m->lnw[i];
+  return 1;
+}
+
+void or_prop_probe(const or_mc *mc, uint64_t seed, int32_t count, float *out) {
+  or_mc *m = (or_mc *)mc; /* lj_delta reads only */
+  uint64_t rng[2];
+  or_rng_seed(seed, rng);
+  double lnw1;
+  int have1 = sad_lnw(m, m->E, &lnw1);
+  for (int32_t c = 0; c < count; c++) {
+    float *rec = out + (int64_t)c * OR_PROP_REC;
+    int64_t w = (int64_t)or_uniform_index(rng, (uint64_t)m->n);
+    double fx = m->x[w], fy = m->y[w], fz = m->z[w];
+    prop_features_xyz(m->n, m->x, m->y, m->z, (int)w, fx, fy, fz, rec);
+    for (int k = 0; k < OR_PROP_NS; k++) {
+      float sig = expf(kPropSMin + (float)k * ((kPropSMax - kPropSMin) / (float)(OR_PROP_NS - 1)));
+      float jsum = 0.0f;
+      for (int r = 0; r < OR_PROP_R; r++) {
+        float xi[3];
+        or_prop_gaussian(rng, xi);
+        float dx = sig * xi[0], dy = sig * xi[1], dz = sig * xi[2];
+        double tx = fx + (double)dx, ty = fy + (double)dy, tz = fz + (double)dz;
+        double r2to = norm2(tx, ty, tz), r2from = norm2(fx, fy, fz);
+        if (r2to > m->R2 && r2to > r2from) continue;
+        double e2 = m->E + lj_delta(m, w, tx, ty, tz, fx, fy, fz);
+        if (m->p.has_max_e && e2 > m->p.max_e && e2 > m->E) continue;
+        double lnw2;
+        if (!have1 || !sad_lnw(m, e2, &lnw2)) continue;
+        double lr = lnw1 - lnw2;
+        float a = lr >= 0.0 ? 1.0f : (float)exp(lr);
+        jsum += a * (dx * dx + dy * dy + dz * dz);
+      }
+      rec[OR_PROP_NF + k] = jsum / (float)OR_PROP_R;
+    }
+  }
+}
+
+/* Jhat(s) by linear interpolation on the scale grid, its slope, and the
+ * normaliser max_m J_m (0 if every J_m is 0). */
+static void prop_target(const float *J, double s, double *val, double *slope, double *jmax) {
+  const double dS = ((double)kPropSMax - (double)kPropSMin) / (double)(OR_PROP_NS - 1);
+  double mx = 0.0;
+  for (int k = 0; k < OR_PROP_NS; k++) mx = J[k] > mx ? J[k] : mx;
+  *jmax = mx;
+  double q = (s - (double)kPropSMin) / dS;
+  int k = (int)floor(q);
+  if (k < 0) k = 0;
+  if (k > OR_PROP_NS - 2) k = OR_PROP_NS - 2;
+  double t = q - (double)k;
+  *val = (double)J[k] * (1.0 - t) + (double)J[k + 1] * t;
+  *slope = ((double)J[k + 1] - (double)J[k]) / dS;
+}
+
+double or_prop_loss_grad(const float *w, const float *records, int64_t count, double *grad) {
+  const float *W2 = w + 288, *w3 = w + 1344;
+  if (grad) memset(grad, 0, sizeof(double) * OR_PROP_NP);
+  double loss = 0.0;
+  for (int64_t n = 0; n < count; n++) {
+    const float *rec = records + n * OR_PROP_REC;
+    float h1[OR_PROP_H], h2[OR_PROP_H], raw;
+    prop_forward(w, rec, h1, h2, &raw);
+    double val, slope, mx;
+    double s = raw < kPropSMin ? kPropSMin : (raw > kPropSMax ? kPropSMax : raw);
+    prop_target(rec + OR_PROP_NF, s, &val, &slope, &mx);
+    if (mx <= 0.0) continue;
+    loss -= val / mx;
+    if (!grad) continue;
+    double d3 = (raw > kPropSMin && raw < kPropSMax) ? -slope / mx / (double)count : 0.0;
+    if (d3 == 0.0) continue;
+    double d2[OR_PROP_H];
+    grad[1376] += d3;
+    for (int u = 0; u < OR_PROP_H; u++) {
+      grad[1344 + u] += d3 * h2[u];
+      d2[u] = h2[u] > 0.0f ? d3 * w3[u] : 0.0;
+    }
+    for (int u = 0; u < OR_PROP_H; u++) {
+      grad[1312 + u] += d2[u];
+      for (int k = 0; k < OR_PROP_H; k++) grad[288 + u * OR_PROP_H + k] += d2[u] * h1[k];
+    }
+    for (int k = 0; k < OR_PROP_H; k++) {
+      if (!(h1[k] > 0.0f)) continue;
+      double d1 = 0.0;
+      for (int u = 0; u < OR_PROP_H; u++) d1 += d2[u] * W2[u * OR_PROP_H + k];
+      grad[256 + k] += d1;
+      for (int j = 0; j < OR_PROP_NF; j++) grad[k * OR_PROP_NF + j] += d1 * rec[j];
+    }
+  }
+  return loss / (double)count;
+}
+
+double or_prop_loss_grad_f64(const double *w, const float *records, int64_t count, double *grad) {
+  const double *W1 = w, *b1 = w + 256, *W2 = w + 288, *b2 = w + 1312, *w3 = w + 1344, *b3 = w + 1376;
+  if (grad) memset(grad, 0, sizeof(double) * OR_PROP_NP);
+  double loss = 0.0;
+  for (int64_t n = 0; n < count; n++) {
+    const float *rec = records + n * OR_PROP_REC;
+    double h1[OR_PROP_H], h2[OR_PROP_H];
+    for (int u = 0; u < OR_PROP_H; u++) {
+      double z = b1[u];
+      for (int k = 0; k < OR_PROP_NF; k++) z += W1[u * OR_PROP_NF + k] * (double)rec[k];
+      h1[u] = z > 0.0 ? z : 0.0;
+    }
+    for (int u = 0; u < OR_PROP_H; u++) {
+      double z = b2[u];
+      for (int k = 0; k < OR_PROP_H; k++) z += W2[u * OR_PROP_H + k] * h1[k];
+      h2[u] = z > 0.0 ? z : 0.0;
+    }
+    double raw = b3[0];
+    for (int u = 0; u < OR_PROP_H; u++) raw += w3[u] * h2[u];
+    double s = raw < kPropSMin ? kPropSMin : (raw > kPropSMax ? kPropSMax : raw);
+    double val, slope, mx;
+    prop_target(rec + OR_PROP_NF, s, &val, &slope, &mx);
+    if (mx <= 0.0) continue;
+    loss -= val / mx;
+    if (!grad) continue;
+    double d3 = (raw > kPropSMin && raw < kPropSMax) ? -slope / mx / (double)count : 0.0;
+    if (d3 == 0.0) continue;
+    double d2[OR_PROP_H];
+    grad[1376] += d3;
+    for (int u = 0; u < OR_PROP_H; u++) {
+      grad[1344 + u] += d3 * h2[u];
+      d2[u] = h2[u] > 0.0 ? d3 * w3[u] : 0.0;
+    }
+    for (int u = 0; u < OR_PROP_H; u++) {
+      grad[1312 + u] += d2[u];
+      for (int k = 0; k < OR_PROP_H; k++) grad[288 + u * OR_PROP_H + k] += d2[u] * h1[k];
+    }
+    for (int k = 0; k < OR_PROP_H; k++) {
+      if (!(h1[k] > 0.0)) continue;
+      double d1 = 0.0;
+      for (int u = 0; u < OR_PROP_H; u++) d1 += d2[u] * W2[u * OR_PROP_H + k];
+      grad[256 + k] += d1;
+      for (int j = 0; j < OR_PROP_NF; j++) grad[k * OR_PROP_NF + j] += d1 * (double)rec[j];
+    }
+  }
+  return loss / (double)count;
+}
+
 static void move_once(or_mc *m) {
   m->moves += 1;
   if (m->sys == OR_LJ || m->sys == OR_WCA) {
@@ -982,13 +1268,15 @@ static void move_once(or_mc *m) {
   m->acc_rate *= 1.0 - rs;
   double e2;
   int acc = 0;
-  if (sys_plan_move(m, m->tscale, &e2)) {
+  double lnh = 0.0;
+  int some = m->p.move_kind == 1 && m->sys == OR_LJ ? lj_plan_learned(m, &e2, &lnh) : sys_plan_move(m, m->tscale, &e2);
+  if (some) {
     int oob = 0;
     if (m->p.has_max_e) oob = e2 > m->p.max_e && e2 > e1;
     if (m->p.has_min_e) oob = oob || (e2 < m->p.min_e && e2 < e1);
     if (!oob) {
       prepare_for_state(m, e2);
-      if (!reject_move(m, e1, e2)) {
+      if (!reject_move(m, e1, e2, lnh)) {
         m->accepted += 1;
         m->acc_rate += rs;
         sys_confirm(m);

@@ -76,6 +76,9 @@ typedef struct {
   int32_t ising_init;       /* 0 reference seed 10137; 1 per-walker seed */
   int32_t collect_pressure; /* WCA pressure every N^2 moves */
   uint64_t seed;
+  int32_t move_kind;        /* 0 cube move, 1 learned Gaussian move (LJ) */
+  int32_t drop_hastings;    /* test hook: omit ln H (a planted mistake) */
+  const float *mlp;         /* learned move: OR_PROP_NP weights (BF16 values as float) */
 } or_params;
 
 typedef struct {
@@ -145,6 +148,40 @@ double or_wca_energy(int32_t n, const double *coords, double box, double rc2, in
 double or_wca_energy_ld(int32_t n, const double *coords, double box, double rc2);
 double or_lj_energy_ld(int32_t n, const double *coords);
 double or_sad_gamma(double lambda, double t, double tF, double num_states);
+
+/* ---- learned proposal (no reference counterpart) ----------------------
+ * Features of atom i at position p: f_k = sum_{j != i} exp(-(r_ij - c_k)^2 /
+ * (2 w^2)), c_k = 0.9 + k (1.6/7), w = 0.2, k = 0..7, in float with r_ij =
+ * sqrtf of the float r^2 of the double differences.  MLP 8 -> 32 -> 32 -> 1,
+ * ReLU, weights in the order W1[32][8], b1[32], W2[32][32], b2[32], w3[32],
+ * b3; the output s = ln(sigma) is clamped to [ln 0.01, 0].  Move: atom i
+ * uniform, xi ~ N(0, I3) by Box-Muller from four gen_f64 draws, p' = p +
+ * sigma(p) xi; ln H = 3 (s - s') + |xi|^2 (1 - exp(2 (s - s'))) / 2 (the
+ * two-sided Gaussian proposal ratio q(p'->p)/q(p->p')), added to the
+ * acceptance exponent.  Probe samples for training: features of a random
+ * atom and, for each of the 16 scales ln sigma_m = ln 0.01 + m (ln 100)/15,
+ * the mean over 2 Gaussian draws of alpha |d|^2 (alpha = the walker's
+ * acceptance probability of that displacement, Hastings term excluded). */
+#define OR_PROP_NF 8
+#define OR_PROP_H 32
+#define OR_PROP_NP 1377
+#define OR_PROP_NS 16
+#define OR_PROP_R 2
+#define OR_PROP_REC 24 /* floats per probe record: f[8], J[16] */
+void or_prop_features(int32_t n, const double *coords, int32_t skip, double px, double py, double pz, float *f);
+float or_prop_mlp(const float *w, const float *f);
+void or_prop_gaussian(uint64_t rng[2], float *xi);
+double or_prop_lnh(float s, float s_new, float xi2);
+/* Training loss -mean_n Jhat_n(s_n)/max_m J_n,m and its gradient (float
+ * forward, double accumulation).  grad may be NULL. */
+double or_prop_loss_grad(const float *w, const float *records, int64_t count, double *grad);
+/* The same in double throughout (for finite-difference checks). */
+double or_prop_loss_grad_f64(const double *w, const float *records, int64_t count, double *grad);
+/* `count` probe records of walker `mc` with the probe stream seeded by seed. */
+void or_prop_probe(const or_mc *mc, uint64_t seed, int32_t count, float *out);
+/* Learned move of walker `mc` (plan only): returns 0 if the wall rejects;
+ * e_new, ln H and the moved atom's new position are written. */
+int or_mc_plan_learned(or_mc *mc, double *e_new, double *lnh, double *to);
 
 /* CPU baseline: run `threads` independent walkers (one per thread) for
  * `moves` moves each; returns wall seconds (init excluded). */
