@@ -180,6 +180,31 @@ int sadmc_autotune_lanes(int device, const sadmc_set_params *p, int32_t n_walker
 int sadmc_plan_move(sadmc_ctx *ctx, int32_t set, int32_t walker, double scale, double *e_new, int32_t *some);
 int sadmc_confirm(sadmc_ctx *ctx, int32_t set, int32_t walker);
 
+/* ---- learned proposal (no reference counterpart) ----------------------
+ * LJ walker sets with move_kind = SADMC_MOVE_LEARNED propose Gaussian steps
+ * whose scale comes from a small network of the moved atom's radial
+ * features, with the two-sided Hastings term in the acceptance; the fixed
+ * cube move (lj.rs:365-374 with a uniform step) stays the default.  The
+ * network (1377 parameters, BF16 on the device, FP32 master copy) belongs to
+ * the context and is shared by its learned sets. */
+/* Single-walker learned plan: e_new, ln H and the moved atom's position. */
+int sadmc_plan_learned(sadmc_ctx *ctx, int32_t set, int32_t walker, double *e_new, double *lnh, double *to,
+                       int32_t *some);
+/* Default initial weights from a seed; resets the optimiser. */
+int sadmc_prop_init(sadmc_ctx *ctx, uint64_t seed);
+int sadmc_prop_set_weights(sadmc_ctx *ctx, const float *weights, int32_t count);
+int sadmc_prop_get_weights(sadmc_ctx *ctx, float *master, uint16_t *bf16);
+/* Append per_walker probe records (24 floats: 8 features, 16 acceptance-
+ * weighted squared jumps on the ln sigma grid) from every walker of an LJ
+ * SAD set; walker w draws from the stream seeded by seed + w. */
+int sadmc_prop_collect(sadmc_ctx *ctx, int32_t set, uint64_t seed, int32_t per_walker);
+int sadmc_prop_records(sadmc_ctx *ctx, float *out, int64_t max_records, int64_t *count);
+int sadmc_prop_load_records(sadmc_ctx *ctx, const float *records, int64_t count);
+/* Mean loss and gradient over records [first, first+count) (cyclic). */
+int sadmc_prop_gradient(sadmc_ctx *ctx, int64_t first, int32_t count, float *grad, float *loss);
+/* `steps` Adam steps on consecutive minibatches; losses[k] = step k's loss. */
+int sadmc_prop_train(sadmc_ctx *ctx, int32_t steps, int32_t batch, float lr, float *losses);
+
 /* ---- state: the serde view of EnergyMC (mc/mod.rs:111-120) ----------- */
 int sadmc_walker_count(const sadmc_ctx *ctx, int32_t set, int32_t *n_walkers, int64_t *bin_capacity);
 int sadmc_get_walker_state(sadmc_ctx *ctx, int32_t set, int32_t walker, sadmc_walker_state *out);

@@ -1168,6 +1168,15 @@ static void prop_target(const float *J, double s, double *val, double *slope, do
   *slope = ((double)J[k + 1] - (double)J[k]) / dS;
 }
 
+/* d(loss)/d(raw output) for one record before normalisation: -slope inside
+ * the clamp range; at a clamp, -slope only when it points back inside (a
+ * projected gradient, so a clamped output is not stuck). */
+static double prop_dloss(double raw, double slope) {
+  if (raw > kPropSMin && raw < kPropSMax) return -slope;
+  if ((raw >= kPropSMax && slope < 0.0) || (raw <= kPropSMin && slope > 0.0)) return -slope;
+  return 0.0;
+}
+
 double or_prop_loss_grad(const float *w, const float *records, int64_t count, double *grad) {
   const float *W2 = w + 288, *w3 = w + 1344;
   if (grad) memset(grad, 0, sizeof(double) * OR_PROP_NP);
@@ -1182,7 +1191,7 @@ double or_prop_loss_grad(const float *w, const float *records, int64_t count, do
     if (mx <= 0.0) continue;
     loss -= val / mx;
     if (!grad) continue;
-    double d3 = (raw > kPropSMin && raw < kPropSMax) ? -slope / mx / (double)count : 0.0;
+    double d3 = prop_dloss(raw, slope) / mx / (double)count;
     if (d3 == 0.0) continue;
     double d2[OR_PROP_H];
     grad[1376] += d3;
@@ -1230,7 +1239,7 @@ double or_prop_loss_grad_f64(const double *w, const float *records, int64_t coun
     if (mx <= 0.0) continue;
     loss -= val / mx;
     if (!grad) continue;
-    double d3 = (raw > kPropSMin && raw < kPropSMax) ? -slope / mx / (double)count : 0.0;
+    double d3 = prop_dloss(raw, slope) / mx / (double)count;
     if (d3 == 0.0) continue;
     double d2[OR_PROP_H];
     grad[1376] += d3;

@@ -173,7 +173,9 @@ float or_prop_mlp(const float *w, const float *f);
 void or_prop_gaussian(uint64_t rng[2], float *xi);
 double or_prop_lnh(float s, float s_new, float xi2);
 /* Training loss -mean_n Jhat_n(s_n)/max_m J_n,m and its gradient (float
- * forward, double accumulation).  grad may be NULL. */
+ * forward, double accumulation).  At a clamp of the output the gradient is
+ * projected: kept only when it points back into the range.  grad may be
+ * NULL. */
 double or_prop_loss_grad(const float *w, const float *records, int64_t count, double *grad);
 /* The same in double throughout (for finite-difference checks). */
 double or_prop_loss_grad_f64(const double *w, const float *records, int64_t count, double *grad);

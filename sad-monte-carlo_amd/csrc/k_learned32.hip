@@ -1,0 +1,9 @@
+// LJ kernels with the learned Gaussian move, FP32 pair terms.
+#define SADMC_KERNEL_TU
+#include "sadmc_kernels.h"
+namespace sadmc {
+hipError_t launch_learned32(int meth, int kmax, KernelKind kk, const SetArgs &a, const LaunchArgs &la,
+                            hipStream_t s) {
+  return launch_learned_km<1>(meth, kmax, kk, a, la, s);
+}
+}  // namespace sadmc

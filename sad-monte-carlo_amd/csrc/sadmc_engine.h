@@ -26,6 +26,7 @@
 #pragma once
 
 #include "sadmc_common.h"
+#include "sadmc_proposal.h"
 #include "../../include/sadmc.h"
 
 namespace sadmc {
@@ -72,9 +73,11 @@ struct SetArgs {
   int32_t method, move_plan, has_min_e, has_max_e, has_min_gamma;
   int32_t refresh, collect_pressure, precision, move_kind;
   double method_param, move_param, min_e, max_e, rc2, width;
+  const uint16_t *mlp;  // learned proposal: 1377 BF16 weights, or null
 };
 
-// Pending change of the single-walker plan/confirm entry points.
+// Pending change of the single-walker plan/confirm entry points (for a
+// learned LJ move, `dabse` carries ln H).
 struct Pending {
   int32_t some, which;
   double tx, ty, tz, e, dabse;
@@ -118,7 +121,8 @@ __device__ __forceinline__ uint64_t sample_zone(uint64_t &s0, uint64_t &s1, uint
 
 
 
-template <int SYS, int KMAX, int PREC, int METH>
+// MV = 1: the learned Gaussian move with its Hastings term (LJ only).
+template <int SYS, int KMAX, int PREC, int METH, int MV = 0>
 struct Engine {
   const SetArgs *Ap;  // kernel argument segment (uniform; read with scalar loads)
   SADMC_DF const SetArgs &args() const { return *Ap; }
@@ -169,6 +173,10 @@ struct Engine {
   // pending move
   int32_t pw;
   double ptx, pty, ptz, pe, pdabse;
+  // learned move: weights and this walker's exchange row in LDS, ln H
+  const float *PW;
+  float *XC;
+  double lnh;
 
   // ---------------------------------------------------------------- setup
   SADMC_DF void load_system(const SetArgs &a, int walker, int lane_in_group, double *lds_group) {
@@ -604,7 +612,142 @@ struct Engine {
     return true;
   }
 
+  // Features of the group's atoms around (px, py, pz), skipping atom w.
+  SADMC_DF void features(double px, double py, double pz, int w, float *f) {
+#pragma unroll
+    for (int c = 0; c < kPropNF; c++) f[c] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < KMAX; k++) {
+      if (k < K) {
+        int j = lane + G * k;
+        if (j < n && j != w) {
+          float a = (float)(x[k] - px), b = (float)(y[k] - py), c = (float)(z[k] - pz);
+          feature_add(a * a + b * b + c * c, f);
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < kPropNF; c++) f[c] = group_sum_f32(f[c], G);
+  }
+
+  // Sum over the group of the pair-term differences for atom w moving from
+  // f to t (lj.rs:86-105).
+  SADMC_DF double lj_delta(int w, double tx, double ty, double tz, double fx, double fy, double fz) {
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < KMAX; k++) {
+      if (k < K) {
+        int j = lane + G * k;
+        if (j < n && j != w) {
+          if (PREC == 1) {
+            float an = (float)(x[k] - tx), bn = (float)(y[k] - ty), cn = (float)(z[k] - tz);
+            float ao = (float)(x[k] - fx), bo = (float)(y[k] - fy), co = (float)(z[k] - fz);
+            float r2n = an * an + bn * bn + cn * cn;
+            float r2o = ao * ao + bo * bo + co * co;
+            acc += (double)lj_pair_f32(r2n) - (double)lj_pair_f32(r2o);
+          } else {
+            acc += lj_pair(norm2(x[k] - tx, y[k] - ty, z[k] - tz)) - lj_pair(norm2(x[k] - fx, y[k] - fy, z[k] - fz));
+          }
+        }
+      }
+    }
+    return group_sum(acc, G);
+  }
+
+  // The learned move: sigma from the network at the old position, a
+  // Gaussian step, sigma' at the new position for the Hastings term.
+  SADMC_DF bool learned_plan(double &e2) {
+    int w = (int)sample_zone(s0, s1, (uint64_t)n, args().zone_n);
+    const double fx = lx[w], fy = ly[w], fz = lz[w];
+    float f[kPropNF], xi[3];
+    features(fx, fy, fz, w, f);
+    const float s = mlp_forward(PW, XC, f, lane, G);
+    gaussian3(s0, s1, xi);
+    const float sig = expf(s);
+    const double tx = fx + (double)(sig * xi[0]), ty = fy + (double)(sig * xi[1]), tz = fz + (double)(sig * xi[2]);
+    const double r2to = norm2(tx, ty, tz), r2from = norm2(fx, fy, fz);
+    if (r2to > R2 && r2to > r2from) return false;
+    features(tx, ty, tz, w, f);
+    const float s2 = mlp_forward(PW, XC, f, lane, G);
+    const double e = E + lj_delta(w, tx, ty, tz, fx, fy, fz);
+    lnh = prop_lnh(s, s2, xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2]);
+    pw = w;
+    ptx = tx;
+    pty = ty;
+    ptz = tz;
+    pe = e;
+    pdabse = lnh;
+    e2 = e;
+    return true;
+  }
+
+  // SAD ln w(e) with the clamped extension (energy.rs:445-463); false if e
+  // has no bin.
+  SADMC_DF bool sad_lnw(double e, double &out) {
+    if (ilo < 0 || ilo >= len || ihi < 0 || ihi >= len) return false;
+    if (e < too_lo) {
+      out = lnw_lo + (e - too_lo) / min_T;
+      return true;
+    }
+    if (e > too_hi) {
+      out = lnw_hi;
+      return true;
+    }
+    const int32_t i = e_index(e);
+    if (i < 0 || i >= len) return false;
+    out = lnw_at(base + i);
+    return true;
+  }
+
+  // `count` training records: the features of a random atom and, per scale
+  // of the grid, the mean of alpha |d|^2 over kPropR Gaussian steps, from the
+  // probe stream seeded by `seed`.  Lane 0 writes out[count][24].
+  SADMC_DF void probe(uint64_t seed, int count, float *out) {
+    uint64_t q0, q1;
+    rng_seed(seed, q0, q1);
+    double lnw1 = 0.0;
+    const bool have1 = status == 0 && sad_lnw(E, lnw1);
+    for (int c = 0; c < count; c++) {
+      float rec[kPropRec];
+      const int w = (int)uniform_index(q0, q1, (uint64_t)n);
+      const double fx = lx[w], fy = ly[w], fz = lz[w];
+      features(fx, fy, fz, w, rec);
+#pragma unroll
+      for (int k = 0; k < kPropNS; k++) {
+        const float sig = expf(kPropSMin + (float)k * ((kPropSMax - kPropSMin) / (float)(kPropNS - 1)));
+        float jsum = 0.0f;
+        for (int r = 0; r < kPropR; r++) {
+          float xi[3];
+          gaussian3(q0, q1, xi);
+          const float dx = sig * xi[0], dy = sig * xi[1], dz = sig * xi[2];
+          const double tx = fx + (double)dx, ty = fy + (double)dy, tz = fz + (double)dz;
+          const double r2to = norm2(tx, ty, tz), r2from = norm2(fx, fy, fz);
+          if (r2to > R2 && r2to > r2from) continue;
+          const double e2 = E + lj_delta(w, tx, ty, tz, fx, fy, fz);
+          if (args().has_max_e && e2 > args().max_e && e2 > E) continue;
+          double lnw2;
+          if (!have1 || !sad_lnw(e2, lnw2)) continue;
+          const double lr = lnw1 - lnw2;
+          const float al = lr >= 0.0 ? 1.0f : (float)exp(lr);
+          jsum += al * (dx * dx + dy * dy + dz * dz);
+        }
+        rec[kPropNF + k] = jsum / (float)kPropR;
+      }
+      if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < kPropRec; i++) out[(int64_t)c * kPropRec + i] = rec[i];
+      }
+    }
+  }
+
   SADMC_DF bool plan_move(double d, double sc, double &e2) {
+    if (SYS == SYS_ISING) return ising_plan(e2);
+    if (MV == 1 && SYS == SYS_LJ) return learned_plan(e2);
+    return particle_plan(d, sc, e2);
+  }
+
+  // The initial descent of from_params always uses the cube move.
+  SADMC_DF bool cube_plan(double d, double sc, double &e2) {
     if (SYS == SYS_ISING) return ising_plan(e2);
     return particle_plan(d, sc, e2);
   }
@@ -682,8 +825,35 @@ struct Engine {
     return (lam + tm / tF_d) / (lam + tm / Ns_d * (tm / tF_d));
   }
 
-  // energy.rs:440-512.  `r2` is the record of the proposed bin.
+  // Metropolis-Hastings test on the log acceptance ratio.
+  SADMC_DF bool reject_lr(double lr) { return lr < 0.0 && gen_f64(s0, s1) > exp(lr); }
+
+  // energy.rs:440-512.  `r2` is the record of the proposed bin.  The
+  // learned move adds ln H to the exponent (the tests are the reference's
+  // when ln H = 0).
   SADMC_DF bool reject_move(double e1, double e2, double r2_lnw, uint64_t r2_hist) {
+    if (MV == 1) {
+      if (METH == M_SAD) {
+        double lnw1 = e1 < too_lo ? lnw_lo + (e1 - too_lo) / min_T : (e1 > too_hi ? lnw_hi : c_lnw);
+        double lnw2 = e2 < too_lo ? lnw_lo + (e2 - too_lo) / min_T : (e2 > too_hi ? lnw_hi : r2_lnw);
+        bool rejected = reject_lr(lnw1 - lnw2 + lnh);
+        if (!rejected && r2_hist == 0 && e2 < too_hi && e2 > too_lo) {
+          lam = (too_hi - too_lo) / min_T;
+          Ns += 1;
+          Ns_d = (double)Ns;
+          tL = moves;
+        }
+        return rejected;
+      } else if (METH == M_SAMC) {
+        return reject_lr(c_lnw - r2_lnw + lnh);
+      } else if (METH == M_WL) {
+        bool rejected = reject_lr(c_lnw - r2_lnw + lnh);
+        if (method != SADMC_SAMC && !rejected && r2_hist == 0 && wl_lowest > 0) wl_Ns += 1.0;
+        return rejected;
+      } else {
+        return reject_lr((e1 - e2) / args().method_param + lnh);
+      }
+    }
     if (METH == M_SAD) {
       double lnw1 = e1 < too_lo ? lnw_lo + (e1 - too_lo) / min_T : (e1 > too_hi ? lnw_hi : c_lnw);
       double lnw2 = e2 < too_lo ? lnw_lo + (e2 - too_lo) / min_T : (e2 > too_hi ? lnw_hi : r2_lnw);
@@ -1036,7 +1206,7 @@ struct Engine {
       const double sc = SYS == SYS_ISING ? 0.0 : uniform_scale(-0.05, 0.05);
       for (uint64_t k = 0; k < 100000000ULL; k++) {
         double newe;
-        if (plan_move(0.05, sc, newe)) {
+        if (cube_plan(0.05, sc, newe)) {
           if (newe < E) confirm();
           if (E < args().max_e) break;
         }

@@ -28,6 +28,14 @@ static hipError_t launch(int sys, int prec, int method, int kmax, KernelKind kk,
                          const LaunchArgs &la, hipStream_t s) {
   const int m = method_family(method);
   if (sys == SYS_ISING) return launch_ising(m, kk, a, la, s);
+  if (sys == SYS_LJ && a.move_kind == SADMC_MOVE_LEARNED && (kk == KK_RUN || kk == KK_PLAN || kk == KK_PROBE)) {
+    if (prec == SADMC_FP32_PAIRS) return launch_learned32(m, kmax, kk, a, la, s);
+    return launch_learned64(m, kmax, kk, a, la, s);
+  }
+  if (sys == SYS_LJ && kk == KK_PROBE) {
+    if (prec == SADMC_FP32_PAIRS) return launch_learned32(m, kmax, kk, a, la, s);
+    return launch_learned64(m, kmax, kk, a, la, s);
+  }
   if (sys == SYS_LJ) {
     if (prec == SADMC_FP32_PAIRS) return launch_lj32(m, kmax, kk, a, la, s);
     return launch_lj64(m, kmax, kk, a, la, s);
@@ -78,6 +86,18 @@ struct sadmc_ctx {
   size_t batch_lds = 0;
   std::map<uint64_t, hipGraphExec_t> graphs;  // captured batched launch per move count
   std::vector<hipGraph_t> graph_defs;
+  // learned proposal: FP32 master weights, Adam moments, BF16 copy the
+  // kernels read; probe records; training scratch
+  float *d_pw = nullptr, *d_pm = nullptr, *d_pv = nullptr;
+  uint16_t *d_pbf = nullptr;
+  int prop_step = 0;
+  int64_t prop_cursor = 0;
+  float *d_rec = nullptr;
+  int64_t nrec = 0, rec_cap = 0;
+  float *d_partial = nullptr;
+  int partial_blocks = 0;
+  float *d_grad = nullptr, *d_loss = nullptr;
+  int loss_cap = 0;
 };
 
 static int fail(sadmc_ctx *c, int code, const std::string &msg) {
@@ -300,6 +320,9 @@ void sadmc_destroy(sadmc_ctx *c) {
   for (auto &g : c->graph_defs) (void)hipGraphDestroy(g);
   (void)hipFree(c->d_sets);
   (void)hipFree(c->d_tiles);
+  for (void *p : {(void *)c->d_pw, (void *)c->d_pm, (void *)c->d_pv, (void *)c->d_pbf, (void *)c->d_rec,
+                  (void *)c->d_partial, (void *)c->d_grad, (void *)c->d_loss})
+    (void)hipFree(p);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -348,7 +371,12 @@ int sadmc_add_set(sadmc_ctx *c, const sadmc_set_params *p, int32_t W, const uint
   if (p->method == SADMC_CANONICAL && !(p->method_param > 0)) return fail(c, SADMC_E_ARG, "canonical T must be > 0");
   if (p->move_param <= 0 && p->move_plan == SADMC_TRANSLATION_SCALE && p->system != SADMC_ISING)
     return fail(c, SADMC_E_ARG, "translation scale must be > 0");
-  if (p->move_kind != SADMC_MOVE_CUBE) return fail(c, SADMC_E_ARG, "learned moves use the learned-proposal API");
+  if (p->move_kind == SADMC_MOVE_LEARNED) {
+    if (p->system != SADMC_LJ || (p->method != SADMC_SAD && p->method != SADMC_CANONICAL))
+      return fail(c, SADMC_E_ARG, "the learned move is implemented for LJ walkers with SAD or canonical sampling");
+  } else if (p->move_kind != SADMC_MOVE_CUBE) {
+    return fail(c, SADMC_E_ARG, "unknown move kind");
+  }
   s.width = p->energy_bin > 0 ? p->energy_bin : (p->system == SADMC_ISING ? 4.0 : 1.0);
   sadmc_bin_window(p, geom ? geom[0] : 0.0, &s.e_lo, &s.e_hi, &s.cap);
   s.seeds.assign(seeds, seeds + W);
@@ -409,6 +437,7 @@ static void fill_args(SetHost &s) {
   a.rc2 = s.p.rc2;
   a.width = s.width;
   a.flag = s.d_flag;
+  a.mlp = nullptr;
   const uint64_t nn = (uint64_t)s.p.n;
   a.zone_n = ~0ULL - ((~0ULL - nn + 1) % nn);
   a.zone_site = (nn << __builtin_clzll(nn)) - 1;
@@ -436,6 +465,54 @@ static int check_flags(sadmc_ctx *c) {
     }
   }
   return SADMC_OK;
+}
+
+// Default network: W1 ~ N(0, 0.25^2), W2 ~ N(0, 0.25^2), w3 ~ N(0, 0.05^2),
+// hidden biases 0.1, output bias ln 0.1 (sigma near 0.1 everywhere).
+static void default_weights(uint64_t seed, float *w) {
+  uint64_t s0, s1;
+  rng_seed(seed ^ 0x5EEDC0DE5EEDC0DEULL, s0, s1);
+  auto normal = [&]() {
+    double u1 = gen_f64(s0, s1), u2 = gen_f64(s0, s1);
+    return std::sqrt(-2.0 * std::log(1.0 - u1)) * std::cos(6.283185307179586 * u2);
+  };
+  for (int i = 0; i < kPropNP; i++) w[i] = 0.0f;
+  for (int i = kW1; i < kB1; i++) w[i] = (float)(0.25 * normal());
+  for (int i = kB1; i < kW2; i++) w[i] = 0.1f;
+  for (int i = kW2; i < kB2; i++) w[i] = (float)(0.25 * normal());
+  for (int i = kB2; i < kW3; i++) w[i] = 0.1f;
+  for (int i = kW3; i < kB3; i++) w[i] = (float)(0.05 * normal());
+  w[kB3] = (float)std::log(0.1);
+}
+
+static int upload_weights(sadmc_ctx *c, const float *w) {
+  std::vector<float> m(kPropLds, 0.0f), z(kPropLds, 0.0f);
+  std::vector<uint16_t> b(kPropLds, 0);
+  for (int i = 0; i < kPropNP; i++) {
+    if (!std::isfinite(w[i])) return fail(c, SADMC_E_ARG, "non-finite weight");
+    m[i] = w[i];
+    b[i] = f32_to_bf16(w[i]);
+  }
+  HIPCHECK(c, hipMemcpyAsync(c->d_pw, m.data(), sizeof(float) * kPropLds, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(c->d_pm, z.data(), sizeof(float) * kPropLds, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(c->d_pv, z.data(), sizeof(float) * kPropLds, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(c->d_pbf, b.data(), sizeof(uint16_t) * kPropLds, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  c->prop_step = 0;
+  return SADMC_OK;
+}
+
+static int ensure_prop(sadmc_ctx *c) {
+  if (c->d_pw) return SADMC_OK;
+  HIPCHECK(c, hipSetDevice(c->device));
+  HIPCHECK(c, hipMalloc(&c->d_pw, sizeof(float) * kPropLds));
+  HIPCHECK(c, hipMalloc(&c->d_pm, sizeof(float) * kPropLds));
+  HIPCHECK(c, hipMalloc(&c->d_pv, sizeof(float) * kPropLds));
+  HIPCHECK(c, hipMalloc(&c->d_pbf, sizeof(uint16_t) * kPropLds));
+  HIPCHECK(c, hipMalloc(&c->d_grad, sizeof(float) * kPropLds));
+  std::vector<float> w(kPropNP);
+  default_weights(0, w.data());
+  return upload_weights(c, w.data());
 }
 
 int sadmc_init(sadmc_ctx *c) {
@@ -486,6 +563,11 @@ int sadmc_init(sadmc_ctx *c) {
       }
     }
     fill_args(s);
+    if (s.p.move_kind == SADMC_MOVE_LEARNED) {
+      int rc = ensure_prop(c);
+      if (rc) return rc;
+      s.args.mlp = c->d_pbf;
+    }
     LaunchArgs la{};
     la.e_lo = s.e_lo;
     HIPCHECK(c, launch(s.p.system, s.p.precision, s.p.method, s.KMAX, KK_INIT, s.args, la, c->stream));
@@ -536,7 +618,8 @@ static int build_batch(sadmc_ctx *c) {
   size_t lds = 0;
   for (size_t k = 0; k < c->sets.size(); k++) {
     const SetHost &s = c->sets[k];
-    if (s.p.method != SADMC_SAD || s.p.precision != SADMC_FP64 || s.p.system == SADMC_ISING)
+    if (s.p.method != SADMC_SAD || s.p.precision != SADMC_FP64 || s.p.system == SADMC_ISING ||
+        s.p.move_kind != SADMC_MOVE_CUBE)
       return fail(c, SADMC_E_ARG, "the batched launch runs SAD walkers of LJ/WCA sets with FP64 pair terms");
     const int per_block = 64 / s.G;
     lds = std::max(lds, (size_t)per_block * 3 * s.npad * sizeof(double));
@@ -679,6 +762,171 @@ int sadmc_confirm(sadmc_ctx *c, int32_t set, int32_t walker) {
   HIPCHECK(c, launch(s->p.system, s->p.precision, s->p.method, s->KMAX, KK_CONFIRM, s->args, la, c->stream));
   Pending none{};
   HIPCHECK(c, hipMemcpyAsync(s->d_pending, &none, sizeof none, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  return SADMC_OK;
+}
+
+int sadmc_plan_learned(sadmc_ctx *c, int32_t set, int32_t walker, double *e_new, double *lnh, double *to,
+                       int32_t *some) {
+  SetHost *s = get_set(c, set, walker);
+  if (!s || !c->inited) return fail(c, SADMC_E_ARG, "bad set/walker or not initialised");
+  if (s->p.move_kind != SADMC_MOVE_LEARNED) return fail(c, SADMC_E_ARG, "the set does not use the learned move");
+  LaunchArgs la{};
+  la.walker = walker;
+  la.pending = s->d_pending;
+  HIPCHECK(c, launch(s->p.system, s->p.precision, s->p.method, s->KMAX, KK_PLAN, s->args, la, c->stream));
+  Pending p;
+  HIPCHECK(c, hipMemcpyAsync(&p, s->d_pending, sizeof p, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  if (e_new) *e_new = p.e;
+  if (lnh) *lnh = p.dabse;
+  if (to) {
+    to[0] = p.tx;
+    to[1] = p.ty;
+    to[2] = p.tz;
+  }
+  if (some) *some = p.some;
+  return SADMC_OK;
+}
+
+// ---------------------------------------------------------- learned proposal
+int sadmc_prop_init(sadmc_ctx *c, uint64_t seed) {
+  if (!c) return SADMC_E_ARG;
+  int rc = ensure_prop(c);
+  if (rc) return rc;
+  std::vector<float> w(kPropNP);
+  default_weights(seed, w.data());
+  return upload_weights(c, w.data());
+}
+
+int sadmc_prop_set_weights(sadmc_ctx *c, const float *w, int32_t count) {
+  if (!c || !w || count != kPropNP) return fail(c, SADMC_E_ARG, "expected 1377 weights");
+  int rc = ensure_prop(c);
+  if (rc) return rc;
+  return upload_weights(c, w);
+}
+
+int sadmc_prop_get_weights(sadmc_ctx *c, float *master, uint16_t *bf16) {
+  if (!c) return SADMC_E_ARG;
+  int rc = ensure_prop(c);
+  if (rc) return rc;
+  if (master) HIPCHECK(c, hipMemcpyAsync(master, c->d_pw, sizeof(float) * kPropNP, hipMemcpyDeviceToHost, c->stream));
+  if (bf16) HIPCHECK(c, hipMemcpyAsync(bf16, c->d_pbf, sizeof(uint16_t) * kPropNP, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  return SADMC_OK;
+}
+
+static int reserve_records(sadmc_ctx *c, int64_t need) {
+  if (need <= c->rec_cap) return SADMC_OK;
+  int64_t cap = std::max<int64_t>(need, c->rec_cap * 2);
+  float *p = nullptr;
+  HIPCHECK(c, hipMalloc(&p, sizeof(float) * kPropRec * cap));
+  if (c->nrec)
+    HIPCHECK(c, hipMemcpyAsync(p, c->d_rec, sizeof(float) * kPropRec * c->nrec, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  (void)hipFree(c->d_rec);
+  c->d_rec = p;
+  c->rec_cap = cap;
+  return SADMC_OK;
+}
+
+int sadmc_prop_collect(sadmc_ctx *c, int32_t set, uint64_t seed, int32_t per_walker) {
+  SetHost *s = get_set(c, set, 0);
+  if (!s || !c->inited || per_walker <= 0) return fail(c, SADMC_E_ARG, "bad set or count, or not initialised");
+  if (s->p.system != SADMC_LJ || s->p.method != SADMC_SAD)
+    return fail(c, SADMC_E_ARG, "probe records come from LJ SAD walkers");
+  int rc = reserve_records(c, c->nrec + (int64_t)s->W * per_walker);
+  if (rc) return rc;
+  LaunchArgs la{};
+  la.seed = seed;
+  la.count = per_walker;
+  la.out = c->d_rec + c->nrec * kPropRec;
+  HIPCHECK(c, launch(s->p.system, s->p.precision, s->p.method, s->KMAX, KK_PROBE, s->args, la, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  c->nrec += (int64_t)s->W * per_walker;
+  return SADMC_OK;
+}
+
+int sadmc_prop_records(sadmc_ctx *c, float *out, int64_t max_records, int64_t *count) {
+  if (!c) return SADMC_E_ARG;
+  if (count) *count = c->nrec;
+  const int64_t k = std::min(max_records, c->nrec);
+  if (out && k > 0) {
+    HIPCHECK(c, hipMemcpyAsync(out, c->d_rec, sizeof(float) * kPropRec * k, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+  }
+  return SADMC_OK;
+}
+
+int sadmc_prop_load_records(sadmc_ctx *c, const float *rec, int64_t count) {
+  if (!c || count < 0 || (count && !rec)) return fail(c, SADMC_E_ARG, "bad records");
+  HIPCHECK(c, hipSetDevice(c->device));
+  c->nrec = 0;
+  c->prop_cursor = 0;
+  int rc = reserve_records(c, count);
+  if (rc) return rc;
+  if (count) {
+    HIPCHECK(c, hipMemcpyAsync(c->d_rec, rec, sizeof(float) * kPropRec * count, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+  }
+  c->nrec = count;
+  return SADMC_OK;
+}
+
+static int reserve_partial(sadmc_ctx *c, int blocks) {
+  if (blocks <= c->partial_blocks) return SADMC_OK;
+  (void)hipFree(c->d_partial);
+  c->d_partial = nullptr;
+  HIPCHECK(c, hipMalloc(&c->d_partial, sizeof(float) * (kPropNP + 1) * blocks));
+  c->partial_blocks = blocks;
+  return SADMC_OK;
+}
+
+int sadmc_prop_gradient(sadmc_ctx *c, int64_t first, int32_t count, float *grad, float *loss) {
+  if (!c || count <= 0 || first < 0) return fail(c, SADMC_E_ARG, "bad range");
+  int rc = ensure_prop(c);
+  if (rc) return rc;
+  if (c->nrec == 0) return fail(c, SADMC_E_STATE, "no probe records");
+  const int blocks = (count + 63) / 64;
+  rc = reserve_partial(c, blocks);
+  if (rc) return rc;
+  if (c->loss_cap < 1) {
+    HIPCHECK(c, hipMalloc(&c->d_loss, sizeof(float) * 1024));
+    c->loss_cap = 1024;
+  }
+  HIPCHECK(c, launch_prop_grad(c->d_rec, c->nrec, first, count, c->d_pbf, c->d_partial, c->stream));
+  HIPCHECK(c, launch_prop_update(c->d_partial, blocks, count, c->d_pw, c->d_pm, c->d_pv, c->d_pbf, 0.0f, 1,
+                                 c->d_grad, c->d_loss, c->stream));
+  if (grad) HIPCHECK(c, hipMemcpyAsync(grad, c->d_grad, sizeof(float) * kPropNP, hipMemcpyDeviceToHost, c->stream));
+  if (loss) HIPCHECK(c, hipMemcpyAsync(loss, c->d_loss, sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  return SADMC_OK;
+}
+
+int sadmc_prop_train(sadmc_ctx *c, int32_t steps, int32_t batch, float lr, float *losses) {
+  if (!c || steps <= 0 || batch <= 0 || !(lr > 0)) return fail(c, SADMC_E_ARG, "bad training arguments");
+  int rc = ensure_prop(c);
+  if (rc) return rc;
+  if (c->nrec == 0) return fail(c, SADMC_E_STATE, "no probe records");
+  const int blocks = (batch + 63) / 64;
+  rc = reserve_partial(c, blocks);
+  if (rc) return rc;
+  if (c->loss_cap < steps) {
+    (void)hipFree(c->d_loss);
+    c->d_loss = nullptr;
+    HIPCHECK(c, hipMalloc(&c->d_loss, sizeof(float) * steps));
+    c->loss_cap = steps;
+  }
+  for (int k = 0; k < steps; k++) {
+    const int64_t first = c->prop_cursor % c->nrec;
+    c->prop_cursor = (c->prop_cursor + batch) % c->nrec;
+    c->prop_step += 1;
+    HIPCHECK(c, launch_prop_grad(c->d_rec, c->nrec, first, batch, c->d_pbf, c->d_partial, c->stream));
+    HIPCHECK(c, launch_prop_update(c->d_partial, blocks, batch, c->d_pw, c->d_pm, c->d_pv, c->d_pbf, lr,
+                                   c->prop_step, nullptr, c->d_loss + k, c->stream));
+  }
+  if (losses)
+    HIPCHECK(c, hipMemcpyAsync(losses, c->d_loss, sizeof(float) * steps, hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   return SADMC_OK;
 }

@@ -11,7 +11,7 @@
 
 namespace sadmc {
 
-enum KernelKind { KK_RUN = 0, KK_INIT = 1, KK_PLAN = 2, KK_CONFIRM = 3 };
+enum KernelKind { KK_RUN = 0, KK_INIT = 1, KK_PLAN = 2, KK_CONFIRM = 3, KK_PROBE = 4 };
 
 struct LaunchArgs {
   uint64_t moves;
@@ -19,6 +19,9 @@ struct LaunchArgs {
   int walker;
   double scale;
   Pending *pending;
+  uint64_t seed;  // probe: walker w uses the stream seeded by seed + w
+  int count;      // probe: records per walker
+  float *out;     // probe: [W][count][24]
 };
 
 // One workgroup of the batched launch: walkers [first_walker, first_walker +
@@ -33,6 +36,16 @@ hipError_t launch_batched(const SetArgs *d_sets, const BatchTile *d_tiles, int n
 hipError_t launch_ising(int meth, KernelKind kk, const SetArgs &a, const LaunchArgs &la, hipStream_t s);
 hipError_t launch_lj64(int meth, int kmax, KernelKind kk, const SetArgs &a, const LaunchArgs &la, hipStream_t s);
 hipError_t launch_lj32(int meth, int kmax, KernelKind kk, const SetArgs &a, const LaunchArgs &la, hipStream_t s);
+// Learned-move LJ kernels (run, plan, probe), FP64 or FP32 pair terms.
+hipError_t launch_learned64(int meth, int kmax, KernelKind kk, const SetArgs &a, const LaunchArgs &la,
+                            hipStream_t s);
+hipError_t launch_learned32(int meth, int kmax, KernelKind kk, const SetArgs &a, const LaunchArgs &la,
+                            hipStream_t s);
+// Training of the proposal network (k_train.hip).
+hipError_t launch_prop_grad(const float *rec, int64_t nrec, int64_t first, int count, const uint16_t *wbf,
+                            float *partial, hipStream_t s);
+hipError_t launch_prop_update(const float *partial, int nblk, int count, float *master, float *m, float *v,
+                              uint16_t *wbf, float lr, int step, float *grad_out, float *loss_out, hipStream_t s);
 hipError_t launch_wca(int meth, int kmax, KernelKind kk, const SetArgs &a, const LaunchArgs &la, hipStream_t s);
 
 #ifdef SADMC_KERNEL_TU
@@ -128,6 +141,8 @@ static hipError_t launch_t(KernelKind kk, const SetArgs &a, const LaunchArgs &la
     case KK_CONFIRM:
       hipLaunchKernelGGL((k_confirm<SYS, KMAX, PREC, METH>), dim3(1), block, lds1, s, a, la.walker, la.pending);
       break;
+    default:
+      return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
@@ -151,6 +166,111 @@ static hipError_t launch_km(int meth, int kmax, KernelKind kk, const SetArgs &a,
     case 4: return launch_m<SYS, 4, PREC>(meth, kk, a, la, s);
     case 8: return launch_m<SYS, 8, PREC>(meth, kk, a, la, s);
     default: return launch_m<SYS, 16, PREC>(meth, kk, a, la, s);
+  }
+}
+
+// ---------------------------------------------------------- learned move
+// LDS: the positions of the 64/G walkers, then the network weights as float,
+// then 32 floats per walker for the hidden-layer exchange.
+template <int KMAX, int PREC, int METH>
+__global__ __launch_bounds__(64) void k_run_learned(SetArgs a, uint64_t moves) {
+  extern __shared__ double lds[];
+  const int G = a.G, per = 64 / G;
+  float *lw = (float *)(lds + per * 3 * a.npad);
+  load_weights(a.mlp, lw);
+  const int group = threadIdx.x / G;
+  const int lane = threadIdx.x & (G - 1);
+  const int walker = blockIdx.x * per + group;
+  if (walker >= a.W) return;
+  Engine<SYS_LJ, KMAX, PREC, METH, 1> e;
+  e.PW = lw;
+  e.XC = lw + kPropLds + group * kPropH;
+  e.load(a, walker, lane, lds + group * 3 * a.npad);
+  e.run(moves);
+  e.store(walker);
+}
+
+template <int KMAX, int PREC, int METH>
+__global__ __launch_bounds__(64) void k_plan_learned(SetArgs a, int walker, Pending *out) {
+  extern __shared__ double lds[];
+  float *lw = (float *)(lds + 3 * a.npad);
+  load_weights(a.mlp, lw);
+  if ((int)threadIdx.x >= a.G) return;
+  Engine<SYS_LJ, KMAX, PREC, METH, 1> e;
+  e.PW = lw;
+  e.XC = lw + kPropLds;
+  e.load(a, walker, threadIdx.x, lds);
+  double e2 = 0.0;
+  bool some = e.plan_move(0.0, 0.0, e2);
+  if (threadIdx.x == 0) {
+    Pending p;
+    p.some = some ? 1 : 0;
+    p.which = e.pw;
+    p.tx = e.ptx;
+    p.ty = e.pty;
+    p.tz = e.ptz;
+    p.e = e2;
+    p.dabse = some ? e.lnh : 0.0;
+    *out = p;
+  }
+  e.store(walker);
+}
+
+// Training records of every walker (reads the walkers, changes nothing).
+template <int KMAX, int PREC>
+__global__ __launch_bounds__(64) void k_probe(SetArgs a, uint64_t seed, int count, float *out) {
+  extern __shared__ double lds[];
+  const int G = a.G;
+  const int group = threadIdx.x / G;
+  const int lane = threadIdx.x & (G - 1);
+  const int walker = blockIdx.x * (64 / G) + group;
+  if (walker >= a.W) return;
+  Engine<SYS_LJ, KMAX, PREC, M_SAD, 1> e;
+  e.load(a, walker, lane, lds + group * 3 * a.npad);
+  e.probe(seed + (uint64_t)walker, count, out + (int64_t)walker * count * kPropRec);
+}
+
+template <int KMAX, int PREC, int METH>
+static hipError_t launch_learned_t(KernelKind kk, const SetArgs &a, const LaunchArgs &la, hipStream_t s) {
+  const int G = a.G;
+  const int per_block = 64 / G;
+  const size_t wbytes = sizeof(float) * (kPropLds + (size_t)per_block * kPropH);
+  const size_t lds = (size_t)per_block * 3 * a.npad * sizeof(double) + wbytes;
+  const size_t lds1 = (size_t)3 * a.npad * sizeof(double) + sizeof(float) * (kPropLds + kPropH);
+  const dim3 grid((a.W + per_block - 1) / per_block), block(64);
+  switch (kk) {
+    case KK_RUN:
+      hipLaunchKernelGGL((k_run_learned<KMAX, PREC, METH>), grid, block, lds, s, a, la.moves);
+      break;
+    case KK_PLAN:
+      hipLaunchKernelGGL((k_plan_learned<KMAX, PREC, METH>), dim3(1), block, lds1, s, a, la.walker, la.pending);
+      break;
+    case KK_PROBE:
+      hipLaunchKernelGGL((k_probe<KMAX, PREC>), grid, block, (size_t)per_block * 3 * a.npad * sizeof(double), s, a,
+                         la.seed, la.count, la.out);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int KMAX, int PREC>
+static hipError_t launch_learned_m(int meth, KernelKind kk, const SetArgs &a, const LaunchArgs &la, hipStream_t s) {
+  if (meth == M_SAD) return launch_learned_t<KMAX, PREC, M_SAD>(kk, a, la, s);
+  if (meth == M_CAN) return launch_learned_t<KMAX, PREC, M_CAN>(kk, a, la, s);
+  return hipErrorInvalidValue;
+}
+
+template <int PREC>
+static hipError_t launch_learned_km(int meth, int kmax, KernelKind kk, const SetArgs &a, const LaunchArgs &la,
+                                    hipStream_t s) {
+  switch (kmax) {
+    case 1: return launch_learned_m<1, PREC>(meth, kk, a, la, s);
+    case 2: return launch_learned_m<2, PREC>(meth, kk, a, la, s);
+    case 4: return launch_learned_m<4, PREC>(meth, kk, a, la, s);
+    case 8: return launch_learned_m<8, PREC>(meth, kk, a, la, s);
+    default: return launch_learned_m<16, PREC>(meth, kk, a, la, s);
   }
 }
 

@@ -79,7 +79,13 @@ EXPORTS = [
     "sadmc_get_walker_state", "sadmc_walker_bins", "sadmc_walker_coords", "sadmc_walker_spins",
     "sadmc_download_raw", "sadmc_upload_raw", "sadmc_init_coords", "sadmc_init_spins", "sadmc_bin_window",
     "sadmc_run_batched", "sadmc_graph_run", "sadmc_autotune_lanes",
+    "sadmc_plan_learned", "sadmc_prop_init", "sadmc_prop_set_weights", "sadmc_prop_get_weights",
+    "sadmc_prop_collect", "sadmc_prop_records", "sadmc_prop_load_records", "sadmc_prop_gradient",
+    "sadmc_prop_train",
 ]
+
+PROP_NP, PROP_REC, PROP_NS = 1377, 24, 16
+PROP_SMIN, PROP_SMAX = -4.605170186, 0.0
 
 
 def lib():
@@ -116,6 +122,15 @@ def lib():
     L.sadmc_graph_run.argtypes = [vp, u64]
     L.sadmc_autotune_lanes.argtypes = [C.c_int, C.POINTER(SetParams), i32, vp, vp, u64, u64, C.POINTER(i32),
                                        C.POINTER(d)]
+    L.sadmc_plan_learned.argtypes = [vp, i32, i32, C.POINTER(d), C.POINTER(d), vp, C.POINTER(i32)]
+    L.sadmc_prop_init.argtypes = [vp, u64]
+    L.sadmc_prop_set_weights.argtypes = [vp, vp, i32]
+    L.sadmc_prop_get_weights.argtypes = [vp, vp, vp]
+    L.sadmc_prop_collect.argtypes = [vp, i32, u64, i32]
+    L.sadmc_prop_records.argtypes = [vp, vp, i64, C.POINTER(i64)]
+    L.sadmc_prop_load_records.argtypes = [vp, vp, i64]
+    L.sadmc_prop_gradient.argtypes = [vp, i64, i32, vp, C.POINTER(C.c_float)]
+    L.sadmc_prop_train.argtypes = [vp, i32, i32, C.c_float, vp]
     for name in EXPORTS:
         if name not in ("sadmc_last_error", "sadmc_destroy", "sadmc_abi_version"):
             getattr(L, name).restype = C.c_int
@@ -219,6 +234,54 @@ class Context:
         self._check(fn(self.h, moves), "graph_run" if graph else "run_batched")
         if sync:
             self.sync()
+
+    # ---- learned proposal
+    def plan_learned(self, s, w):
+        """(e_new, ln H, new position) of walker w's learned move, or None if the wall rejects it."""
+        e, h, some = C.c_double(), C.c_double(), C.c_int32()
+        to = np.zeros(3)
+        self._check(lib().sadmc_plan_learned(self.h, s, w, C.byref(e), C.byref(h), _p(to), C.byref(some)),
+                    "plan_learned")
+        return (e.value, h.value, to) if some.value else None
+
+    def prop_init(self, seed=0):
+        self._check(lib().sadmc_prop_init(self.h, seed), "prop_init")
+
+    def prop_set_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        self._check(lib().sadmc_prop_set_weights(self.h, _p(w), len(w)), "prop_set_weights")
+
+    def prop_get_weights(self):
+        """(FP32 master weights, the BF16 copy as float32)."""
+        m = np.zeros(PROP_NP, np.float32)
+        b = np.zeros(PROP_NP, np.uint16)
+        self._check(lib().sadmc_prop_get_weights(self.h, _p(m), _p(b)), "prop_get_weights")
+        return m, (b.astype(np.uint32) << 16).view(np.float32)
+
+    def prop_collect(self, s, seed, per_walker):
+        self._check(lib().sadmc_prop_collect(self.h, s, seed, per_walker), "prop_collect")
+
+    def prop_records(self):
+        n = C.c_int64()
+        self._check(lib().sadmc_prop_records(self.h, None, 0, C.byref(n)), "prop_records")
+        out = np.zeros((n.value, PROP_REC), np.float32)
+        self._check(lib().sadmc_prop_records(self.h, _p(out), n.value, C.byref(n)), "prop_records")
+        return out
+
+    def prop_load_records(self, rec):
+        rec = np.ascontiguousarray(rec, dtype=np.float32).reshape(-1, PROP_REC)
+        self._check(lib().sadmc_prop_load_records(self.h, _p(rec), len(rec)), "prop_load_records")
+
+    def prop_gradient(self, first, count):
+        g = np.zeros(PROP_NP, np.float32)
+        loss = C.c_float()
+        self._check(lib().sadmc_prop_gradient(self.h, first, count, _p(g), C.byref(loss)), "prop_gradient")
+        return loss.value, g
+
+    def prop_train(self, steps, batch, lr):
+        losses = np.zeros(steps, np.float32)
+        self._check(lib().sadmc_prop_train(self.h, steps, batch, lr, _p(losses)), "prop_train")
+        return losses
 
     def sync(self):
         self._check(lib().sadmc_sync(self.h), "sync")
