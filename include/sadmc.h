@@ -205,6 +205,16 @@ int sadmc_prop_gradient(sadmc_ctx *ctx, int64_t first, int32_t count, float *gra
 /* `steps` Adam steps on consecutive minibatches; losses[k] = step k's loss. */
 int sadmc_prop_train(sadmc_ctx *ctx, int32_t steps, int32_t batch, float lr, float *losses);
 
+/* ---- mean entropy on the device (plotting/parse-binning.py:150-171) ---
+ * S(E) of every SAD walker of a set (ln w inside [too_lo, too_hi], extended
+ * below with slope 1/min_T and outside with ln(H/<H>)), each walker shifted
+ * to mean 0 over the energy window [e_a, e_b] (e_a >= e_b: the intersection
+ * of the walkers' [too_lo, too_hi]), averaged over the walkers covering each
+ * bin.  Writes bin k's centre as e_first + k * width, S[k] and count[k];
+ * returns SADMC_E_CAPACITY (with *nbins set) if max_bins is too small. */
+int sadmc_mean_entropy(sadmc_ctx *ctx, int32_t set, double e_a, double e_b, int64_t max_bins, double *e_first,
+                       int64_t *nbins, double *S, int64_t *count);
+
 /* ---- state: the serde view of EnergyMC (mc/mod.rs:111-120) ----------- */
 int sadmc_walker_count(const sadmc_ctx *ctx, int32_t set, int32_t *n_walkers, int64_t *bin_capacity);
 int sadmc_get_walker_state(sadmc_ctx *ctx, int32_t set, int32_t walker, sadmc_walker_state *out);

@@ -11,10 +11,13 @@ Restates the reference's offline analysis:
   histogram is added where it is positive.
 - `heat_capacity` follows `plotting/final_heat_capacity.py:81-89`:
   Cv(T) = <(E - U)^2> / T^2 under P(E) proportional to exp(S(E) - E/T).
-- `mean_entropy` combines many walkers (the reference analyses one walker
-  per file): every walker's S is placed on a common energy grid by bin
-  index, its free constant fixed on the bins all walkers cover, and the
-  estimates are averaged over the walkers that cover each bin.
+- `mean_entropy` combines many SAD walkers (the reference analyses one
+  walker per file): every walker's S is placed on a common energy grid by
+  bin index, its free constant fixed by its mean over a common energy window,
+  and the estimates are averaged over the walkers that cover each bin; bins
+  outside a walker's [too_lo, too_hi] with an empty histogram carry no
+  estimate (the reference's script writes 0 there).  This is the host
+  statement of the device reduction sadmc_mean_entropy.
 
 All arithmetic is float64 numpy on data already downloaded from the device.
 """
@@ -70,39 +73,57 @@ def heat_capacity(T, E, S):
     return C
 
 
-def mean_entropy(walkers, width=None, e_min=None):
-    """Average S(E) over walkers on a common grid.
+def walker_entropy_masked(state, bins):
+    """(E, S, valid) of one SAD walker as the walker-set reduction uses it:
+    the parse-binning.py:150-171 extension, with bins outside [too_lo,
+    too_hi] whose histogram is 0 marked invalid (no estimate) instead of 0."""
+    E = bin_centres(state)
+    lnw = np.array(bins["lnw"], dtype=np.float64, copy=True)
+    hist = np.asarray(bins["histogram"], dtype=np.float64)
+    too_lo, too_hi, mT = float(state["too_lo"]), float(state["too_hi"]), float(state["min_T"])
+    i_lo = int(np.abs(E - too_lo).argmin())
+    i_hi = int(np.abs(E - too_hi).argmin())
+    mean_hist = hist[i_lo:i_hi + 1].mean()
+    lo, hi = E < too_lo, E > too_hi
+    valid = ~((lo | hi) & (hist == 0))
+    with np.errstate(divide="ignore"):
+        lnw[lo] = lnw[i_lo] + (E[lo] - too_lo) / mT + np.log(hist[lo] / mean_hist)
+        lnw[hi] = lnw[i_hi] + np.log(hist[hi] / mean_hist)
+    return E, lnw, valid
+
+
+def mean_entropy(walkers, window=None):
+    """Average S(E) over SAD walkers on a common grid.
 
     `walkers` is an iterable of (state, bins) pairs.  Each walker's S is
-    defined up to a constant; the constants are fixed by making every walker
-    average to zero over the bins that all walkers cover (or, with no such
-    bin, by normalising sum exp(S) = 1 per walker).  Returns (E, S, count):
-    bin centres, mean entropy (maximum 0), and how many walkers covered each
-    bin.  Bins no walker covers are dropped.
+    defined up to a constant, fixed by making its mean over the energy window
+    (default: the intersection of the walkers' [too_lo, too_hi]) zero;
+    walkers without an estimate in every bin of the window are left out.  Returns (E, S,
+    count): bin centres, mean entropy, and how many walkers gave an estimate
+    in each bin.  Bins without an estimate are dropped.
     """
-    per = []
-    for st, b in walkers:
-        _, S = walker_entropy(st, b)
-        per.append((float(st["bins_min"]), float(st["bins_width"]), S))
-    if not per:
+    walkers = [(st, b) for st, b in walkers if int(st["status"]) == 0 and int(st["len"]) > 0]
+    if not walkers:
         raise ValueError("no walkers")
-    w = per[0][1] if width is None else width
-    if any(p[1] != w for p in per):
-        raise ValueError("walkers with different bin widths")
-    lo = min(p[0] for p in per) if e_min is None else e_min
-    offs = [int(round((p[0] - lo) / w)) for p in per]
-    n = max(o + len(p[2]) for o, p in zip(offs, per))
-    count = np.zeros(n, dtype=np.int64)
-    for o, p in zip(offs, per):
-        count[o:o + len(p[2])] += 1
-    common = count == len(per)
+    w = float(walkers[0][0]["bins_width"])
+    gmin = min(float(st["bins_min"]) for st, _ in walkers)
+    if window is None:
+        window = (max(float(st["too_lo"]) for st, _ in walkers), min(float(st["too_hi"]) for st, _ in walkers))
+    ga = int(np.ceil((window[0] - gmin) / w - 0.5))
+    gb = int(np.floor((window[1] - gmin) / w - 0.5)) + 1
+    offs = [int(round((float(st["bins_min"]) - gmin) / w)) for st, _ in walkers]
+    n = max(o + int(st["len"]) for o, (st, _) in zip(offs, walkers))
     total = np.zeros(n)
-    for o, p in zip(offs, per):
-        S = p[2]
-        mine = common[o:o + len(S)]
-        S = S - (S[mine].mean() if mine.any() else np.log(np.exp(S).sum()))
-        total[o:o + len(S)] += S
+    count = np.zeros(n, dtype=np.int64)
+    for o, (st, b) in zip(offs, walkers):
+        _, S, valid = walker_entropy_masked(st, b)
+        g = o + np.arange(len(S))
+        inwin = (g >= ga) & (g < gb)
+        if inwin.sum() != gb - ga or not valid[inwin].all():
+            continue
+        S = S - S[inwin].mean()
+        total[g[valid]] += S[valid]
+        count[g[valid]] += 1
     keep = count > 0
-    E = lo + (np.arange(n) + 0.5) * w
-    S = total[keep] / count[keep]
-    return E[keep], S - S.max(), count[keep]
+    E = gmin + (np.arange(n) + 0.5) * w
+    return E[keep], total[keep] / count[keep], count[keep]

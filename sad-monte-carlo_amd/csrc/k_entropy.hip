@@ -1,0 +1,152 @@
+// Mean entropy S(E) of a walker set, reduced on the device.
+//
+// Per walker (plotting/parse-binning.py:150-171): inside [too_lo, too_hi]
+// S = ln w; below too_lo S = ln w(i_lo) + (E - too_lo)/min_T + ln(H/<H>),
+// above too_hi S = ln w(i_hi) + ln(H/<H>), <H> the mean histogram over
+// [i_lo, i_hi] and i_lo, i_hi the bins whose centres are nearest too_lo and
+// too_hi.  Bins outside the range with H = 0 carry no estimate and are
+// skipped (the reference's script sets them to 0 for its one walker).  Each
+// walker's free constant is removed by subtracting its mean S over a common
+// energy window [ga, gb) of global bins (a walker without an estimate in
+// every window bin is left out), then S is averaged over the walkers
+// covering each bin.
+//
+// k_entropy_walker: one wavefront per walker computes i_lo, i_hi, <H> and the
+// window mean.  k_entropy_bins: one lane per global bin loops over the
+// walkers; consecutive lanes read consecutive bins of the same walker, so
+// every load of the 16-byte hot records is coalesced.
+#include "sadmc_kernels.h"
+
+namespace sadmc {
+
+struct EntropyWalker {
+  double too_lo, too_hi, min_T, lnw_lo, lnw_hi, mean_hist, c, bmin;
+  int64_t off, base, len;
+  int32_t ok, pad;
+};
+
+__global__ __launch_bounds__(64) void k_entropy_walker(const sadmc_walker_state *ws, const HotBin *hot, int64_t cap,
+                                                       double width, double gmin, int64_t ga, int64_t gb,
+                                                       EntropyWalker *out) {
+  const int w = blockIdx.x;
+  const sadmc_walker_state &s = ws[w];
+  const int64_t len = s.len, base = s.base;
+  const double bmin = s.bins_min;
+  const HotBin *H = hot + (int64_t)w * cap + base;
+  const int64_t off = (int64_t)llround((bmin - gmin) / width);
+  if (len <= 0 || s.status != 0 || s.method != SADMC_SAD) {
+    if (threadIdx.x == 0) {
+      EntropyWalker e = {};
+      e.ok = 0;
+      out[w] = e;
+    }
+    return;
+  }
+  // bin nearest a given energy: first index of the smallest |E_i - e|
+  auto nearest = [&](double e) {
+    int64_t i = (int64_t)floor((e - bmin) / width);
+    i = i < 0 ? 0 : (i > len - 1 ? len - 1 : i);
+    int64_t best = i;
+    double bd = fabs(bmin + ((double)i + 0.5) * width - e);
+    for (int64_t j = i - 1; j <= i + 1; j++) {
+      if (j < 0 || j >= len) continue;
+      double d = fabs(bmin + ((double)j + 0.5) * width - e);
+      if (d < bd || (d == bd && j < best)) {
+        bd = d;
+        best = j;
+      }
+    }
+    return best;
+  };
+  const int64_t ilo = nearest(s.too_lo), ihi = nearest(s.too_hi);
+  double hsum = 0.0;
+  for (int64_t i = ilo + threadIdx.x; i <= ihi; i += 64) hsum += (double)H[i].hist;
+  for (int d = 32; d >= 1; d >>= 1) hsum += __shfl_xor(hsum, d, 64);
+  const double mean_hist = hsum / (double)(ihi - ilo + 1);
+  const double lnw_lo = H[ilo].lnw, lnw_hi = H[ihi].lnw;
+  // the window mean of the extended S; every window bin must carry an estimate
+  double csum = 0.0;
+  int64_t ccount = 0;
+  for (int64_t g = ga + threadIdx.x; g < gb; g += 64) {
+    const int64_t i = g - off;
+    if (i < 0 || i >= len) continue;
+    const HotBin b = H[i];
+    const double E = bmin + ((double)i + 0.5) * width;
+    if (E < s.too_lo) {
+      if (b.hist == 0) continue;
+      csum += lnw_lo + (E - s.too_lo) / s.min_T + log((double)b.hist / mean_hist);
+    } else if (E > s.too_hi) {
+      if (b.hist == 0) continue;
+      csum += lnw_hi + log((double)b.hist / mean_hist);
+    } else {
+      csum += b.lnw;
+    }
+    ccount += 1;
+  }
+  for (int d = 32; d >= 1; d >>= 1) {
+    csum += __shfl_xor(csum, d, 64);
+    ccount += __shfl_xor(ccount, d, 64);
+  }
+  if (threadIdx.x == 0) {
+    EntropyWalker e;
+    e.too_lo = s.too_lo;
+    e.too_hi = s.too_hi;
+    e.min_T = s.min_T;
+    e.lnw_lo = lnw_lo;
+    e.lnw_hi = lnw_hi;
+    e.mean_hist = mean_hist;
+    e.c = ccount > 0 ? csum / (double)ccount : 0.0;
+    e.bmin = bmin;
+    e.off = off;
+    e.base = base;
+    e.len = len;
+    e.ok = (s.status == 0 && s.method == SADMC_SAD && ccount == gb - ga && len > 0) ? 1 : 0;
+    e.pad = 0;
+    out[w] = e;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_entropy_bins(const EntropyWalker *ew, int W, const HotBin *hot, int64_t cap,
+                                                      double width, int64_t ng, double *S, int64_t *count) {
+  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (g >= ng) return;
+  double sum = 0.0;
+  int64_t n = 0;
+  for (int w = 0; w < W; w++) {
+    const EntropyWalker e = ew[w];
+    const int64_t i = g - e.off;
+    if (!e.ok || i < 0 || i >= e.len) continue;
+    const HotBin b = hot[(int64_t)w * cap + e.base + i];
+    const double E = e.bmin + ((double)i + 0.5) * width;
+    double v;
+    if (E < e.too_lo) {
+      if (b.hist == 0) continue;
+      v = e.lnw_lo + (E - e.too_lo) / e.min_T + log((double)b.hist / e.mean_hist);
+    } else if (E > e.too_hi) {
+      if (b.hist == 0) continue;
+      v = e.lnw_hi + log((double)b.hist / e.mean_hist);
+    } else {
+      v = b.lnw;
+    }
+    sum += v - e.c;
+    n += 1;
+  }
+  S[g] = n ? sum / (double)n : 0.0;
+  count[g] = n;
+}
+
+hipError_t launch_entropy(const sadmc_walker_state *ws, const HotBin *hot, int W, int64_t cap, double width,
+                          double gmin, int64_t ga, int64_t gb, int64_t ng, void *ew, double *S, int64_t *count,
+                          hipStream_t s) {
+  hipLaunchKernelGGL(k_entropy_walker, dim3(W), dim3(64), 0, s, ws, hot, cap, width, gmin, ga, gb,
+                     (EntropyWalker *)ew);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_entropy_bins, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, s, (const EntropyWalker *)ew,
+                     W, hot, cap, width, ng, S, count);
+  return hipGetLastError();
+}
+
+size_t entropy_walker_bytes() { return sizeof(EntropyWalker); }
+
+}  // namespace sadmc

@@ -931,6 +931,59 @@ int sadmc_prop_train(sadmc_ctx *c, int32_t steps, int32_t batch, float lr, float
   return SADMC_OK;
 }
 
+int sadmc_mean_entropy(sadmc_ctx *c, int32_t set, double e_a, double e_b, int64_t max_bins, double *e_first,
+                       int64_t *nbins, double *S, int64_t *count) {
+  SetHost *s = get_set(c, set, 0);
+  if (!s || !c->inited) return fail(c, SADMC_E_ARG, "bad set or not initialised");
+  if (s->p.method != SADMC_SAD) return fail(c, SADMC_E_ARG, "the entropy reduction reads SAD walkers");
+  HIPCHECK(c, hipSetDevice(c->device));
+  std::vector<sadmc_walker_state> ws(s->W);
+  HIPCHECK(c, hipMemcpyAsync(ws.data(), s->d_ws, sizeof(sadmc_walker_state) * s->W, hipMemcpyDeviceToHost,
+                             c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  const double w = s->width;
+  double gmin = 0.0, lo = -HUGE_VAL, hi = HUGE_VAL;
+  bool any = false;
+  for (const auto &x : ws) {
+    if (x.status || x.len <= 0) continue;
+    gmin = any ? std::min(gmin, x.bins_min) : x.bins_min;
+    lo = std::max(lo, x.too_lo);
+    hi = std::min(hi, x.too_hi);
+    any = true;
+  }
+  if (!any) return fail(c, SADMC_E_STATE, "no running walkers");
+  int64_t ng = 0;
+  for (const auto &x : ws)
+    if (!x.status && x.len > 0) ng = std::max<int64_t>(ng, (int64_t)llround((x.bins_min - gmin) / w) + x.len);
+  if (!(e_a < e_b)) {
+    e_a = lo;
+    e_b = hi;
+  }
+  const int64_t ga = (int64_t)std::ceil((e_a - gmin) / w - 0.5), gb = (int64_t)std::floor((e_b - gmin) / w - 0.5) + 1;
+  if (!(gb > ga)) return fail(c, SADMC_E_STATE, "no common energy window for the walkers");
+  if (e_first) *e_first = gmin + 0.5 * w;
+  if (nbins) *nbins = ng;
+  if (ng > max_bins) return fail(c, SADMC_E_CAPACITY, "output holds fewer bins than the set covers");
+  void *ew = nullptr;
+  double *dS = nullptr;
+  int64_t *dC = nullptr;
+  int rc = SADMC_OK;
+  if (hipMalloc(&ew, entropy_walker_bytes() * s->W) != hipSuccess || hipMalloc(&dS, sizeof(double) * ng) != hipSuccess ||
+      hipMalloc(&dC, sizeof(int64_t) * ng) != hipSuccess) {
+    rc = fail(c, SADMC_E_HIP, "allocation failed");
+  } else if (launch_entropy(s->d_ws, s->d_hot, s->W, s->cap, w, gmin, ga, gb, ng, ew, dS, dC, c->stream) !=
+                 hipSuccess ||
+             (S && hipMemcpyAsync(S, dS, sizeof(double) * ng, hipMemcpyDeviceToHost, c->stream) != hipSuccess) ||
+             (count && hipMemcpyAsync(count, dC, sizeof(int64_t) * ng, hipMemcpyDeviceToHost, c->stream) != hipSuccess) ||
+             hipStreamSynchronize(c->stream) != hipSuccess) {
+    rc = fail(c, SADMC_E_HIP, "entropy reduction failed");
+  }
+  (void)hipFree(ew);
+  (void)hipFree(dS);
+  (void)hipFree(dC);
+  return rc;
+}
+
 int sadmc_walker_count(const sadmc_ctx *c, int32_t set, int32_t *n, int64_t *cap) {
   if (!c || set < 0 || set >= (int32_t)c->sets.size()) return SADMC_E_ARG;
   if (n) *n = c->sets[set].W;

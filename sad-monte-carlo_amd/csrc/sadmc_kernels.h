@@ -44,6 +44,12 @@ hipError_t launch_learned32(int meth, int kmax, KernelKind kk, const SetArgs &a,
 // Training of the proposal network (k_train.hip).
 hipError_t launch_prop_grad(const float *rec, int64_t nrec, int64_t first, int count, const uint16_t *wbf,
                             float *partial, hipStream_t s);
+// On-device mean entropy of a walker set (k_entropy.hip); `ew` holds
+// entropy_walker_bytes() per walker.
+hipError_t launch_entropy(const sadmc_walker_state *ws, const HotBin *hot, int W, int64_t cap, double width,
+                          double gmin, int64_t ga, int64_t gb, int64_t ng, void *ew, double *S, int64_t *count,
+                          hipStream_t s);
+size_t entropy_walker_bytes();
 hipError_t launch_prop_update(const float *partial, int nblk, int count, float *master, float *m, float *v,
                               uint16_t *wbf, float lr, int step, float *grad_out, float *loss_out, hipStream_t s);
 hipError_t launch_wca(int meth, int kmax, KernelKind kk, const SetArgs &a, const LaunchArgs &la, hipStream_t s);

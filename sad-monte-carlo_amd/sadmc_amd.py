@@ -81,7 +81,7 @@ EXPORTS = [
     "sadmc_run_batched", "sadmc_graph_run", "sadmc_autotune_lanes",
     "sadmc_plan_learned", "sadmc_prop_init", "sadmc_prop_set_weights", "sadmc_prop_get_weights",
     "sadmc_prop_collect", "sadmc_prop_records", "sadmc_prop_load_records", "sadmc_prop_gradient",
-    "sadmc_prop_train",
+    "sadmc_prop_train", "sadmc_mean_entropy",
 ]
 
 PROP_NP, PROP_REC, PROP_NS = 1377, 24, 16
@@ -131,6 +131,7 @@ def lib():
     L.sadmc_prop_load_records.argtypes = [vp, vp, i64]
     L.sadmc_prop_gradient.argtypes = [vp, i64, i32, vp, C.POINTER(C.c_float)]
     L.sadmc_prop_train.argtypes = [vp, i32, i32, C.c_float, vp]
+    L.sadmc_mean_entropy.argtypes = [vp, i32, d, d, i64, C.POINTER(d), C.POINTER(i64), vp, vp]
     for name in EXPORTS:
         if name not in ("sadmc_last_error", "sadmc_destroy", "sadmc_abi_version"):
             getattr(L, name).restype = C.c_int
@@ -234,6 +235,23 @@ class Context:
         self._check(fn(self.h, moves), "graph_run" if graph else "run_batched")
         if sync:
             self.sync()
+
+    def mean_entropy(self, s, window=None):
+        """(E, S, count) of a SAD set reduced on the device (bins no walker covers dropped)."""
+        ea, eb = window if window is not None else (0.0, 0.0)
+        e0, nb = C.c_double(), C.c_int64()
+        rc = lib().sadmc_mean_entropy(self.h, s, ea, eb, 0, C.byref(e0), C.byref(nb), None, None)
+        if rc not in (0, -3):
+            self._check(rc, "mean_entropy")
+        S = np.zeros(nb.value)
+        cnt = np.zeros(nb.value, np.int64)
+        self._check(lib().sadmc_mean_entropy(self.h, s, ea, eb, nb.value, C.byref(e0), C.byref(nb), _p(S), _p(cnt)),
+                    "mean_entropy")
+        w = self.sets[s][0].energy_bin if self.sets[s][0].energy_bin > 0 else (4.0 if self.sets[s][0].system == ISING
+                                                                               else 1.0)
+        E = e0.value + np.arange(nb.value) * w
+        keep = cnt > 0
+        return E[keep], S[keep], cnt[keep]
 
     # ---- learned proposal
     def plan_learned(self, s, w):
