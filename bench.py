@@ -7,17 +7,19 @@ moves uniform in a cube of half-width 0.1 sigma; SAD with min_T = 0.05
 epsilon, energy bins of 0.01 epsilon, --max-allowed-energy 0; FP32 pair terms
 accumulated in FP64, FP64 full-energy refresh every 1024 moves.
 
-A step is one launch that advances every walker by --moves-per-step moves.
-The walkers first run --burn-in moves untimed (SAD's start-up phase, when the
-energy range is still being discovered), then --warmup untimed steps, then
-exactly --steps timed steps bracketed by stream synchronisation.  `value` is
-total moves / wall seconds of the timed steps; the kernel time per launch
-comes from HIP events on the library's stream.
+A step is one launch that advances every walker by --moves-per-step moves;
+the default 20 steps of 500000 moves time each walker's 1e7-move run (the
+configuration's timing run, from SAD's start-up phase, when the energy range
+is still being discovered, to its steady state).  --warmup untimed launches
+of 2000 moves come first.  `value` is total moves / wall seconds of the timed
+steps, bracketed by stream synchronisation; the kernel time per launch comes
+from HIP events on the library's stream.
 
 The same line carries the FP64 variant's throughput, the relative energy
 error of a sample of walkers against an FP64 recompute, the roofline figures
-of the run kernel, and a CPU baseline: the C oracle (the same algorithm and
-configuration, one walker per thread) timed on this host.
+of the run kernel, and CPU baselines: the C oracle (the same algorithm and
+configuration, one walker per thread on every host core, each walker running
+the same 1e7-move trajectory) timed on this host, for both precisions.
 """
 import argparse
 import json
@@ -38,6 +40,7 @@ RADIUS = 3.0
 FLOP_PER_MOVE = 30 * (N_ATOMS - 1)  # 2(N-1) pair terms x 15 flop (division counted as one)
 FP32_VECTOR_PEAK = 157.3            # TFLOP/s, MI355X
 FP64_VECTOR_PEAK = 78.6
+WARMUP_MOVES = 2000                 # moves per warmup launch
 
 
 def log(msg):
@@ -61,26 +64,23 @@ def lj_energy_fp64(c):
     return float(np.sum(4.0 * (s3 * s3 - s3)))
 
 
-def run_gpu(precision, walkers, lanes, burn_in, warmup, steps, mps, label):
+def run_gpu(precision, walkers, lanes, warmup, steps, mps, label):
+    """Time `steps` launches of `mps` moves of every walker, after `warmup`
+    short launches (WARMUP_MOVES each) of the same walkers."""
     ctx = S.Context(0)
     sid = ctx.add_set(lj_params(precision, lanes), np.arange(walkers, dtype=np.uint64), geom=RADIUS)
     ctx.init()
-    log(f"[{label}] init done; burn-in {burn_in} moves")
-    left = burn_in
-    while left > 0:
-        m = min(left, 10000)
-        ctx.run(m)
-        left -= m
     for _ in range(warmup):
-        ctx.run(mps)
+        ctx.run(WARMUP_MOVES)
     log(f"[{label}] timing {steps} steps of {mps} moves x {walkers} walkers")
     ctx.sync()
     kernel_ms = []
     t0 = time.perf_counter()
-    for _ in range(steps):
+    for k in range(steps):
         ctx.run(mps, sync=False)
         ctx.sync()
         kernel_ms.append(ctx.last_run_ms())
+        log(f"[{label}] step {k + 1}/{steps}: {kernel_ms[-1]:.1f} ms")
     wall = time.perf_counter() - t0
     moves = steps * mps * walkers
     # relative energy error of a sample of walkers against an FP64 recompute
@@ -95,9 +95,11 @@ def run_gpu(precision, walkers, lanes, burn_in, warmup, steps, mps, label):
                 max_rel_energy_error=float(np.max(rel)), mean_rel_energy_error=float(np.mean(rel)))
 
 
-def cpu_baseline(precision, seconds_target=10.0):
-    """The C oracle (same algorithm and configuration), one walker per thread,
-    built here with -O3 -march=native."""
+def cpu_baseline(precision, moves, rounds):
+    """The C oracle (same algorithm and configuration, the reference's
+    summation order), one walker per thread over every core, each walker
+    running the same `moves`-move trajectory as a GPU walker; built here with
+    -O3 -march=native."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import ctypes as C
     import oracle as O
@@ -112,28 +114,28 @@ def cpu_baseline(precision, seconds_target=10.0):
                       energy_bin=0.01, has_max_e=1, max_e=0.0, precision=precision,
                       refresh=1024 if precision == 1 else 0, order_g=0)
     threads = len(os.sched_getaffinity(0))
-    # calibrate on one core, then size the sample to ~seconds_target of wall time
-    t1 = lib.or_bench(C.byref(p), 1, 400000, 1000)
-    per_core = 400000 / t1
-    moves = int(per_core * seconds_target)
-    wall = lib.or_bench(C.byref(p), threads, moves, 0)
+    wall = 0.0
+    for r in range(rounds):
+        wall += lib.or_bench(C.byref(p), threads, moves, r * threads)
+        log(f"[cpu {'fp32' if precision else 'fp64'}] round {r + 1}/{rounds}: {wall:.1f} s")
     os.remove(so)
     cpu = subprocess.run(["bash", "-c", "grep -m1 'model name' /proc/cpuinfo | cut -d: -f2"], capture_output=True,
                          text=True).stdout.strip()
-    return dict(value=threads * moves / wall, unit="moves/s", cores=threads, kind="port",
-                sample=f"{threads} walkers x {moves} moves (C oracle, reference order, -O3 -march=native) "
-                       f"in {wall:.1f} s on {cpu}; single-core calibration {per_core / 1e6:.2f} M moves/s")
+    n = rounds * threads
+    return dict(value=n * moves / wall, unit="moves/s", cores=threads, kind="port",
+                sample=f"{n} walkers (seeds 0..{n - 1}) x {moves} moves each, {threads} threads, C oracle in the "
+                       f"reference's summation order, -O3 -march=native, {wall:.1f} s on {cpu}")
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--moves-per-step", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--moves-per-step", type=int, default=500000)
     ap.add_argument("--walkers", type=int, default=16384)
     ap.add_argument("--lanes", type=int, default=8)
-    ap.add_argument("--burn-in", type=int, default=30000)
+    ap.add_argument("--cpu-rounds", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     a = ap.parse_args()
@@ -141,16 +143,15 @@ def main():
         print(f"bench.py measures one GPU only (walkers are independent; see DESIGN.md for sharding); "
               f"--gpus {a.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
-    main_run = run_gpu(S.FP32_PAIRS, a.walkers, a.lanes, a.burn_in, a.warmup, a.steps, a.moves_per_step,
-                       "lj31 fp32 pairs")
+    main_run = run_gpu(S.FP32_PAIRS, a.walkers, a.lanes, a.warmup, a.steps, a.moves_per_step, "lj31 fp32 pairs")
     moves_per_launch = a.walkers * a.moves_per_step
     achieved = FLOP_PER_MOVE * moves_per_launch / (main_run["kernel_ms"] * 1e-3) / 1e12
+    run_moves = a.steps * a.moves_per_step
     variants = {}
     if not a.no_fp64:
-        r = run_gpu(S.FP64, a.walkers, a.lanes, a.burn_in, a.warmup, max(5, a.steps // 2), a.moves_per_step,
-                    "lj31 fp64")
-        variants["fp64"] = dict(value=r["value"], ms_per_step=r["wall"] * 1e3 / max(5, a.steps // 2),
-                                kernel_ms=r["kernel_ms"], max_rel_energy_error=r["max_rel_energy_error"],
+        r = run_gpu(S.FP64, a.walkers, a.lanes, a.warmup, a.steps, a.moves_per_step, "lj31 fp64")
+        variants["fp64"] = dict(value=r["value"], ms_per_step=r["wall"] * 1e3 / a.steps, kernel_ms=r["kernel_ms"],
+                                max_rel_energy_error=r["max_rel_energy_error"],
                                 roofline_frac=FLOP_PER_MOVE * moves_per_launch / (r["kernel_ms"] * 1e-3) / 1e12
                                 / FP64_VECTOR_PEAK)
     out = {
@@ -167,10 +168,11 @@ def main():
         "dtype": "f32 pair terms, f64 sums",
         "data": "synthetic",
         "config": {
-            "workload": "LJ31 SAD precision configuration: 16384 walkers, N=31, sphere R=3.0, cube moves 0.1, "
-                        "min_T=0.05, bins 0.01, max_allowed_energy 0, FP32 pairs + FP64 sums, FP64 refresh/1024",
+            "workload": "LJ31 SAD precision configuration: 16384 walkers x 1e7 moves, N=31, sphere R=3.0, cube "
+                        "moves 0.1, min_T=0.05, bins 0.01, max_allowed_energy 0, FP32 pairs + FP64 sums, FP64 "
+                        "refresh every 1024 moves",
             "walkers": a.walkers, "lanes_per_walker": a.lanes, "moves_per_step": a.moves_per_step,
-            "burn_in_moves": a.burn_in,
+            "timed_moves_per_walker": run_moves, "warmup_moves_per_step": WARMUP_MOVES,
         },
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_VECTOR_PEAK, "unit": "TFLOP/s",
                      "frac": achieved / FP32_VECTOR_PEAK, "traffic": None,
@@ -180,8 +182,12 @@ def main():
     }
     if not a.no_cpu:
         log("cpu baseline: C oracle, one walker per thread")
-        out["cpu_baseline"] = cpu_baseline(1)
+        out["cpu_baseline"] = cpu_baseline(1, run_moves, a.cpu_rounds)
         out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+        if "fp64" in variants:
+            c64 = cpu_baseline(0, run_moves, a.cpu_rounds)
+            variants["fp64"]["cpu_baseline"] = c64
+            variants["fp64"]["gpu_over_cpu"] = variants["fp64"]["value"] / c64["value"]
     print(json.dumps(out))
 
 

@@ -72,7 +72,7 @@ struct SetArgs {
   int32_t W, n, npad, G, K;
   int32_t method, move_plan, has_min_e, has_max_e, has_min_gamma;
   int32_t refresh, collect_pressure, precision, move_kind;
-  double method_param, move_param, min_e, max_e, rc2, width;
+  double method_param, move_param, min_e, max_e, rc2, width, inv_width;
   const uint16_t *mlp;  // learned proposal: 1377 BF16 weights, or null
 };
 
@@ -177,6 +177,12 @@ struct Engine {
   const float *PW;
   float *XC;
   double lnh;
+  // this move's SAD gamma, computed while the proposed bin's record loads
+  // (valid unless the move changed lambda or N_s), and the next uniform with
+  // the generator state after drawing it, computed before it is needed
+  double gpre, spec_u;
+  bool gdirty;
+  uint64_t spec0, spec1;
 
   // ---------------------------------------------------------------- setup
   SADMC_DF void load_system(const SetArgs &a, int walker, int lane_in_group, double *lds_group) {
@@ -339,8 +345,14 @@ struct Engine {
 
   // --------------------------------------------------------------- bins
   // Bins::state_to_index (energy.rs:371-373); `as usize` saturates.
+  // The quotient by the width is first formed with the reciprocal; only
+  // when that lands within 1e-9 of an integer (where the two roundings could
+  // truncate differently) is the division itself computed.
   SADMC_DF int32_t e_index(double e) const {
-    double v = (e - bmin) / args().width;
+    const double d = e - bmin;
+    double v = d * args().inv_width;
+    const double f = v - trunc(v);
+    if (!(f > 1e-9 && f < 1.0 - 1e-9 && v < 1.0e6)) v = d / args().width;
     if (!(v > 0.0)) return 0;
     if (v >= 2.0e9) return 0x7fffffff;
     return (int32_t)v;
@@ -825,8 +837,28 @@ struct Engine {
     return (lam + tm / tF_d) / (lam + tm / Ns_d * (tm / tF_d));
   }
 
+  // Draw the next uniform ahead of the test that may use it.
+  SADMC_DF void spec_draw() {
+    spec0 = s0;
+    spec1 = s1;
+    spec_u = gen_f64(spec0, spec1);
+  }
+  // `need && u > exp(x)`, consuming the uniform only when `need`.
+  SADMC_DF bool mh_test(bool need, double x) {
+    if (!need) return false;
+    s0 = spec0;
+    s1 = spec1;
+    return spec_u > exp(x);
+  }
   // Metropolis-Hastings test on the log acceptance ratio.
-  SADMC_DF bool reject_lr(double lr) { return lr < 0.0 && gen_f64(s0, s1) > exp(lr); }
+  SADMC_DF bool reject_lr(double lr) { return mh_test(lr < 0.0, lr); }
+  // ln w(e) with the SAD clamped extension (energy.rs:445-463); `inside` is
+  // the bin's ln w.
+  SADMC_DF double sad_lnw_ext(double e, double inside) const {
+    if (e < too_lo) return lnw_lo + (e - too_lo) / min_T;
+    if (e > too_hi) return lnw_hi;
+    return inside;
+  }
 
   // energy.rs:440-512.  `r2` is the record of the proposed bin.  The
   // learned move adds ln H to the exponent (the tests are the reference's
@@ -834,14 +866,14 @@ struct Engine {
   SADMC_DF bool reject_move(double e1, double e2, double r2_lnw, uint64_t r2_hist) {
     if (MV == 1) {
       if (METH == M_SAD) {
-        double lnw1 = e1 < too_lo ? lnw_lo + (e1 - too_lo) / min_T : (e1 > too_hi ? lnw_hi : c_lnw);
-        double lnw2 = e2 < too_lo ? lnw_lo + (e2 - too_lo) / min_T : (e2 > too_hi ? lnw_hi : r2_lnw);
+        const double lnw1 = sad_lnw_ext(e1, c_lnw), lnw2 = sad_lnw_ext(e2, r2_lnw);
         bool rejected = reject_lr(lnw1 - lnw2 + lnh);
         if (!rejected && r2_hist == 0 && e2 < too_hi && e2 > too_lo) {
           lam = (too_hi - too_lo) / min_T;
           Ns += 1;
           Ns_d = (double)Ns;
           tL = moves;
+          gdirty = true;
         }
         return rejected;
       } else if (METH == M_SAMC) {
@@ -855,25 +887,24 @@ struct Engine {
       }
     }
     if (METH == M_SAD) {
-      double lnw1 = e1 < too_lo ? lnw_lo + (e1 - too_lo) / min_T : (e1 > too_hi ? lnw_hi : c_lnw);
-      double lnw2 = e2 < too_lo ? lnw_lo + (e2 - too_lo) / min_T : (e2 > too_hi ? lnw_hi : r2_lnw);
-      bool rejected = lnw2 > lnw1 && gen_f64(s0, s1) > exp(lnw1 - lnw2);
+      const double lnw1 = sad_lnw_ext(e1, c_lnw), lnw2 = sad_lnw_ext(e2, r2_lnw);
+      bool rejected = mh_test(lnw2 > lnw1, lnw1 - lnw2);
       if (!rejected && r2_hist == 0 && e2 < too_hi && e2 > too_lo) {
         lam = (too_hi - too_lo) / min_T;
         Ns += 1;
         Ns_d = (double)Ns;
         tL = moves;
+        gdirty = true;
       }
       return rejected;
     } else if (METH == M_SAMC) {
-      return r2_lnw > c_lnw && gen_f64(s0, s1) > exp(c_lnw - r2_lnw);
+      return mh_test(r2_lnw > c_lnw, c_lnw - r2_lnw);
     } else if (METH == M_WL) {
-      bool rejected = r2_lnw > c_lnw && gen_f64(s0, s1) > exp(c_lnw - r2_lnw);
+      bool rejected = mh_test(r2_lnw > c_lnw, c_lnw - r2_lnw);
       if (method != SADMC_SAMC && !rejected && r2_hist == 0 && wl_lowest > 0) wl_Ns += 1.0;
       return rejected;
     } else {
-      if (e1 >= e2) return false;
-      return gen_f64(s0, s1) > exp((e1 - e2) / args().method_param);
+      return mh_test(e1 < e2, (e1 - e2) / args().method_param);
     }
   }
 
@@ -894,7 +925,7 @@ struct Engine {
   SADMC_DF void update_sad(double E_, int32_t i) {
     const double E = E_;
     if (!(too_lo > too_hi || E < too_lo || E > too_hi)) {
-      c_lnw += sad_gamma();
+      c_lnw += gdirty ? sad_gamma() : gpre;
       if (cur_p == base + ilo) lnw_lo = c_lnw;
       if (cur_p == base + ihi) lnw_hi = c_lnw;
     }
@@ -1068,6 +1099,7 @@ struct Engine {
   SADMC_DF void move_once() {
     moves += 1;
     tm += 1.0;
+    gdirty = true;
     if (SYS != SYS_ISING && moves == next_check) {
       if (args().refresh > 0) {
         E = full_energy();
@@ -1107,6 +1139,12 @@ struct Engine {
           w2 = hb.lnw;
           h2 = hb.hist;
         }
+        // independent of the record being loaded
+        if (METH == M_SAD) {
+          gpre = sad_gamma();
+          gdirty = false;
+        }
+        spec_draw();
         if (!reject_move(e1, e2, w2, h2)) {
           accepted += 1;
           acc_rate += rs;

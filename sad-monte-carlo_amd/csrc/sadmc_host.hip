@@ -436,6 +436,7 @@ static void fill_args(SetHost &s) {
   a.max_e = s.p.max_e;
   a.rc2 = s.p.rc2;
   a.width = s.width;
+  a.inv_width = 1.0 / s.width;
   a.flag = s.d_flag;
   a.mlp = nullptr;
   const uint64_t nn = (uint64_t)s.p.n;
