@@ -127,6 +127,18 @@ def cpu_baseline(precision, moves, rounds):
                        f"reference's summation order, -O3 -march=native, {wall:.1f} s on {cpu}")
 
 
+def copy_probe(nbytes):
+    """A plain device-to-device copy of known size, for scaling the HBM byte
+    counters (FETCH_SIZE, WRITE_SIZE) of a profiler run."""
+    import torch
+    a = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"copied {nbytes} bytes three times")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,11 +150,15 @@ def main():
     ap.add_argument("--cpu-rounds", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, help="only run a device copy of this many bytes")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (walkers are independent; see DESIGN.md for sharding); "
               f"--gpus {a.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
+    if a.copy_probe:
+        copy_probe(a.copy_probe)
+        return
     main_run = run_gpu(S.FP32_PAIRS, a.walkers, a.lanes, a.warmup, a.steps, a.moves_per_step, "lj31 fp32 pairs")
     moves_per_launch = a.walkers * a.moves_per_step
     achieved = FLOP_PER_MOVE * moves_per_launch / (main_run["kernel_ms"] * 1e-3) / 1e12

@@ -133,7 +133,10 @@ struct Engine {
   PExtra *X;
   // system
   double *lx, *ly, *lz;
-  double x[KMAX], y[KMAX], z[KMAX];
+  double x[KMAX], y[KMAX], z[KMAX];  // this lane's atoms (unused by the learned move, which reads LDS)
+  SADMC_DF double xk(int k) const { return MV ? lx[lane + G * k] : x[k]; }
+  SADMC_DF double yk(int k) const { return MV ? ly[lane + G * k] : y[k]; }
+  SADMC_DF double zk(int k) const { return MV ? lz[lane + G * k] : z[k]; }
   uint64_t spins;
   double R2, L, halfL;
   // EnergyMC scalars
@@ -208,12 +211,15 @@ struct Engine {
       for (int k = 0; k < KMAX; k++) {
         if (k < K) {
           int j = lane + G * k;
-          x[k] = P[j];
-          y[k] = P[a.npad + j];
-          z[k] = P[2 * a.npad + j];
-          lx[j] = x[k];
-          ly[j] = y[k];
-          lz[j] = z[k];
+          const double vx = P[j], vy = P[a.npad + j], vz = P[2 * a.npad + j];
+          if (MV == 0) {
+            x[k] = vx;
+            y[k] = vy;
+            z[k] = vz;
+          }
+          lx[j] = vx;
+          ly[j] = vy;
+          lz[j] = vz;
         }
       }
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -335,9 +341,9 @@ struct Engine {
       for (int k = 0; k < KMAX; k++) {
         if (k < K) {
           int j = lane + G * k;
-          P[j] = x[k];
-          P[args().npad + j] = y[k];
-          P[2 * args().npad + j] = z[k];
+          P[j] = xk(k);
+          P[args().npad + j] = yk(k);
+          P[2 * args().npad + j] = zk(k);
         }
       }
     }
@@ -512,10 +518,10 @@ struct Engine {
         if (i < n) {
           for (int j = 0; j < i; j++) {
             if (SYS == SYS_LJ) {
-              acc += lj_pair(norm2(x[k] - lx[j], y[k] - ly[j], z[k] - lz[j]));
+              acc += lj_pair(norm2(xk(k) - lx[j], yk(k) - ly[j], zk(k) - lz[j]));
             } else {
-              acc += wca_pair(norm2(min_image(x[k] - lx[j], L, halfL), min_image(y[k] - ly[j], L, halfL),
-                                    min_image(z[k] - lz[j], L, halfL)),
+              acc += wca_pair(norm2(min_image(xk(k) - lx[j], L, halfL), min_image(yk(k) - ly[j], L, halfL),
+                                    min_image(zk(k) - lz[j], L, halfL)),
                               args().rc2);
             }
           }
@@ -535,8 +541,8 @@ struct Engine {
         if (i < n) {
           for (int j = 0; j < n; j++) {
             if (j == i) continue;
-            acc += wca_pair_pressure(norm2(min_image(x[k] - lx[j], L, halfL), min_image(y[k] - ly[j], L, halfL),
-                                           min_image(z[k] - lz[j], L, halfL)),
+            acc += wca_pair_pressure(norm2(min_image(xk(k) - lx[j], L, halfL), min_image(yk(k) - ly[j], L, halfL),
+                                           min_image(zk(k) - lz[j], L, halfL)),
                                      args().rc2);
           }
         }
@@ -592,20 +598,20 @@ struct Engine {
         if (j < n && j != w) {
           if (SYS == SYS_LJ) {
             if (PREC == 1) {
-              float an = (float)(x[k] - tx), bn = (float)(y[k] - ty), cn = (float)(z[k] - tz);
-              float ao = (float)(x[k] - fx), bo = (float)(y[k] - fy), co = (float)(z[k] - fz);
+              float an = (float)(xk(k) - tx), bn = (float)(yk(k) - ty), cn = (float)(zk(k) - tz);
+              float ao = (float)(xk(k) - fx), bo = (float)(yk(k) - fy), co = (float)(zk(k) - fz);
               float r2n = an * an + bn * bn + cn * cn;
               float r2o = ao * ao + bo * bo + co * co;
               acc += (double)lj_pair_f32(r2n) - (double)lj_pair_f32(r2o);
             } else {
-              acc += lj_pair(norm2(x[k] - tx, y[k] - ty, z[k] - tz)) - lj_pair(norm2(x[k] - fx, y[k] - fy, z[k] - fz));
+              acc += lj_pair(norm2(xk(k) - tx, yk(k) - ty, zk(k) - tz)) - lj_pair(norm2(xk(k) - fx, yk(k) - fy, zk(k) - fz));
             }
           } else {
-            double vn = wca_pair(norm2(min_image(x[k] - tx, L, halfL), min_image(y[k] - ty, L, halfL),
-                                       min_image(z[k] - tz, L, halfL)),
+            double vn = wca_pair(norm2(min_image(xk(k) - tx, L, halfL), min_image(yk(k) - ty, L, halfL),
+                                       min_image(zk(k) - tz, L, halfL)),
                                  args().rc2);
-            double vo = wca_pair(norm2(min_image(x[k] - fx, L, halfL), min_image(y[k] - fy, L, halfL),
-                                       min_image(z[k] - fz, L, halfL)),
+            double vo = wca_pair(norm2(min_image(xk(k) - fx, L, halfL), min_image(yk(k) - fy, L, halfL),
+                                       min_image(zk(k) - fz, L, halfL)),
                                  args().rc2);
             acc += vn - vo;
             accabs += vn + vo;
@@ -624,18 +630,17 @@ struct Engine {
     return true;
   }
 
-  // Features of the group's atoms around (px, py, pz), skipping atom w.
+  // Features of the group's atoms around (px, py, pz), skipping atom w;
+  // positions are read from the LDS mirror to keep registers free.
   SADMC_DF void features(double px, double py, double pz, int w, float *f) {
 #pragma unroll
     for (int c = 0; c < kPropNF; c++) f[c] = 0.0f;
-#pragma unroll
-    for (int k = 0; k < KMAX; k++) {
-      if (k < K) {
-        int j = lane + G * k;
-        if (j < n && j != w) {
-          float a = (float)(x[k] - px), b = (float)(y[k] - py), c = (float)(z[k] - pz);
-          feature_add(a * a + b * b + c * c, f);
-        }
+#pragma unroll 1
+    for (int k = 0; k < K; k++) {
+      int j = lane + G * k;
+      if (j < n && j != w) {
+        float a = (float)(lx[j] - px), b = (float)(ly[j] - py), c = (float)(lz[j] - pz);
+        feature_add(a * a + b * b + c * c, f);
       }
     }
 #pragma unroll
@@ -652,13 +657,13 @@ struct Engine {
         int j = lane + G * k;
         if (j < n && j != w) {
           if (PREC == 1) {
-            float an = (float)(x[k] - tx), bn = (float)(y[k] - ty), cn = (float)(z[k] - tz);
-            float ao = (float)(x[k] - fx), bo = (float)(y[k] - fy), co = (float)(z[k] - fz);
+            float an = (float)(xk(k) - tx), bn = (float)(yk(k) - ty), cn = (float)(zk(k) - tz);
+            float ao = (float)(xk(k) - fx), bo = (float)(yk(k) - fy), co = (float)(zk(k) - fz);
             float r2n = an * an + bn * bn + cn * cn;
             float r2o = ao * ao + bo * bo + co * co;
             acc += (double)lj_pair_f32(r2n) - (double)lj_pair_f32(r2o);
           } else {
-            acc += lj_pair(norm2(x[k] - tx, y[k] - ty, z[k] - tz)) - lj_pair(norm2(x[k] - fx, y[k] - fy, z[k] - fz));
+            acc += lj_pair(norm2(xk(k) - tx, yk(k) - ty, zk(k) - tz)) - lj_pair(norm2(xk(k) - fx, yk(k) - fy, zk(k) - fz));
           }
         }
       }
@@ -801,7 +806,7 @@ struct Engine {
     int slot = pw / G;
 #pragma unroll
     for (int k = 0; k < KMAX; k++) {
-      if (lane == owner && k == slot) {
+      if (MV == 0 && lane == owner && k == slot) {
         x[k] = ptx;
         y[k] = pty;
         z[k] = ptz;

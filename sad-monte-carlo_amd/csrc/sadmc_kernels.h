@@ -178,8 +178,10 @@ static hipError_t launch_km(int meth, int kmax, KernelKind kk, const SetArgs &a,
 // ---------------------------------------------------------- learned move
 // LDS: the positions of the 64/G walkers, then the network weights as float,
 // then 32 floats per walker for the hidden-layer exchange.
+// Two waves per SIMD: the register budget the 16384-walker sets need.
 template <int KMAX, int PREC, int METH>
-__global__ __launch_bounds__(64) void k_run_learned(SetArgs a, uint64_t moves) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_run_learned(SetArgs a,
+                                                                                             uint64_t moves) {
   extern __shared__ double lds[];
   const int G = a.G, per = 64 / G;
   float *lw = (float *)(lds + per * 3 * a.npad);

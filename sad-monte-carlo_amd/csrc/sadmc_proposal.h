@@ -52,9 +52,11 @@ __device__ __forceinline__ float group_sum_f32(float v, int G) {
   return v;
 }
 
-// Contribution of one neighbour at float distance^2 r2 to the 8 features.
+// Contribution of one neighbour at float distance^2 r2 to the 8 features
+// (hardware square root and exponential: the proposal only has to be the
+// same function at both ends of a move, not correctly rounded).
 __device__ __forceinline__ void feature_add(float r2, float *f) {
-  const float r = sqrtf(r2);
+  const float r = __builtin_amdgcn_sqrtf(r2);
 #pragma unroll
   for (int k = 0; k < kPropNF; k++) {
     const float d = r - (kPropC0 + (float)k * kPropDC);
@@ -67,17 +69,33 @@ __device__ __forceinline__ void feature_add(float r2, float *f) {
 // through `xc` (32 floats of LDS per walker).  W: the weights in LDS.
 __device__ __forceinline__ float mlp_forward(const float *W, float *xc, const float *f, int lane, int G) {
   for (int u = lane; u < kPropH; u += G) {
+    const float4 *w1 = (const float4 *)(W + kW1 + u * kPropNF);
+    const float4 a = w1[0], b = w1[1];
     float z = W[kB1 + u];
-#pragma unroll
-    for (int k = 0; k < kPropNF; k++) z += W[kW1 + u * kPropNF + k] * f[k];
+    z += a.x * f[0];
+    z += a.y * f[1];
+    z += a.z * f[2];
+    z += a.w * f[3];
+    z += b.x * f[4];
+    z += b.y * f[5];
+    z += b.z * f[6];
+    z += b.w * f[7];
     xc[u] = z > 0.0f ? z : 0.0f;
   }
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  const float4 *x4 = (const float4 *)xc;
   float part = 0.0f;
   for (int u = lane; u < kPropH; u += G) {
+    const float4 *w2 = (const float4 *)(W + kW2 + u * kPropH);
     float z = W[kB2 + u];
 #pragma unroll
-    for (int k = 0; k < kPropH; k++) z += W[kW2 + u * kPropH + k] * xc[k];
+    for (int k = 0; k < kPropH / 4; k++) {
+      const float4 w = w2[k], h = x4[k];
+      z += w.x * h.x;
+      z += w.y * h.y;
+      z += w.z * h.z;
+      z += w.w * h.w;
+    }
     const float h = z > 0.0f ? z : 0.0f;
     part += W[kW3 + u] * h;
   }
@@ -86,15 +104,19 @@ __device__ __forceinline__ float mlp_forward(const float *W, float *xc, const fl
   return s < kPropSMin ? kPropSMin : (s > kPropSMax ? kPropSMax : s);
 }
 
-// Three standard normal draws by Box-Muller from four uniforms.
+// Three standard normal draws by Box-Muller from four uniforms, with the
+// hardware log2, square root and sin/cos (whose argument is in revolutions,
+// so cos(2 pi u) is one instruction).  The Hastings term is computed from the
+// draws actually used.
 __device__ __forceinline__ void gaussian3(uint64_t &s0, uint64_t &s1, float *xi) {
   const float u1 = (float)gen_f64(s0, s1), u2 = (float)gen_f64(s0, s1);
   const float u3 = (float)gen_f64(s0, s1), u4 = (float)gen_f64(s0, s1);
-  const float two_pi = 6.283185307f;
-  const float r1 = sqrtf(-2.0f * logf(1.0f - u1)), r2 = sqrtf(-2.0f * logf(1.0f - u3));
-  xi[0] = r1 * cosf(two_pi * u2);
-  xi[1] = r1 * sinf(two_pi * u2);
-  xi[2] = r2 * cosf(two_pi * u4);
+  const float m2ln2 = -1.3862943611f;  // -2 ln 2
+  const float r1 = __builtin_amdgcn_sqrtf(m2ln2 * __builtin_amdgcn_logf(1.0f - u1));
+  const float r2 = __builtin_amdgcn_sqrtf(m2ln2 * __builtin_amdgcn_logf(1.0f - u3));
+  xi[0] = r1 * __builtin_amdgcn_cosf(u2);
+  xi[1] = r1 * __builtin_amdgcn_sinf(u2);
+  xi[2] = r2 * __builtin_amdgcn_cosf(u4);
 }
 
 SADMC_DF double prop_lnh(float s, float s_new, float xi2) {
