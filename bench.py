@@ -127,6 +127,154 @@ def cpu_baseline(precision, moves, rounds):
                        f"reference's summation order, -O3 -march=native, {wall:.1f} s on {cpu}")
 
 
+def learned_cv(budget, walkers, lanes, chunk, out_path):
+    """Wall time to Cv(T) within 2% for LJ38 (sphere 3.2 sigma, SAD min_T =
+    0.03, 16384 walkers): plain cube-move walkers against learned-proposal
+    walkers.  The reference curve is the plain walkers' Cv after `budget`
+    seconds of GPU time.  The learned walkers are the same walkers: they run
+    the cube move while plain-walker records are collected (64 per walker,
+    1e6 in all), the network is trained, and then they continue with the
+    learned move; their clock includes the collection and the training.  Cv
+    comes from the device entropy reduction at checkpoints every `chunk`
+    moves; the time to 2% is the first checkpoint after which every later
+    checkpoint stays within 2% of the reference on the temperature window."""
+    import analysis as A
+    T = np.linspace(0.05, 0.40, 71)
+    windows = {"T0.05-0.40": T >= 0.05, "T0.10-0.40": T >= 0.10, "T0.15-0.40": T >= 0.15}
+
+    def params(kind):
+        return S.make_params(system=S.LJ, n=38, lanes=lanes, method=S.SAD, method_param=0.03, move_param=0.1,
+                             energy_bin=0.01, has_max_e=1, max_e=0.0, move_kind=kind)
+
+    def checkpoint(ctx, sid, t, tag, curves):
+        try:
+            E, Sg, _ = ctx.mean_entropy(sid)
+        except S.SadmcError as e:  # no energy window common to all walkers yet
+            log(f"[{tag}] t={t:7.1f} s  no estimate: {e}")
+            return
+        cv = A.heat_capacity(T, E, Sg)
+        curves.append((t, cv))
+        log(f"[{tag}] t={t:7.1f} s  Cv(0.1)={cv[10]:8.2f} Cv(0.2)={cv[30]:8.2f} Cv(0.3)={cv[50]:8.2f}")
+
+    seeds = np.arange(walkers, dtype=np.uint64)
+    # plain walkers
+    ctx = S.Context(0)
+    sid = ctx.add_set(params(S.MOVE_CUBE), seeds, geom=3.2)
+    ctx.init()
+    plain, t = [], 0.0
+    while t < budget:
+        ctx.run(chunk)
+        t += ctx.last_run_ms() / 1e3
+        checkpoint(ctx, sid, t, "plain", plain)
+    ctx.close()
+    ref = plain[-1][1]
+    # learned walkers: cube moves while records are collected, training, learned moves
+    ctx = S.Context(0)
+    sid = ctx.add_set(params(S.MOVE_CUBE), seeds, geom=3.2)
+    ctx.init()
+    learned, t = [], 0.0
+    t0 = time.perf_counter()
+    for k in range(16):
+        ctx.run(chunk // 16)
+        t += ctx.last_run_ms() / 1e3
+        tc = time.perf_counter()
+        ctx.prop_collect(sid, 1000 + k, 4)
+        t += time.perf_counter() - tc
+    tc = time.perf_counter()
+    ctx.prop_init(0)
+    losses = ctx.prop_train(3000, 4096, 3e-3)
+    w, _ = ctx.prop_get_weights()
+    st, bins = ctx.download_raw(sid)
+    coords = np.concatenate([ctx.coords(sid, k) for k in range(walkers)])
+    ctx.close()
+    ctx = S.Context(0)
+    ctx.prop_set_weights(w)
+    sid = ctx.add_set(params(S.MOVE_LEARNED), seeds, geom=3.2)
+    ctx.init()
+    ctx.upload_raw(sid, st, bins, coords=coords)
+    t += time.perf_counter() - tc
+    log(f"[learned] collected 1e6 records and trained in {t:.1f} s (loss {losses[:20].mean():.3f} -> "
+        f"{losses[-100:].mean():.3f})")
+    checkpoint(ctx, sid, t, "learned", learned)
+    while t < budget:
+        ctx.run(chunk // 2)
+        t += ctx.last_run_ms() / 1e3
+        checkpoint(ctx, sid, t, "learned", learned)
+    ctx.close()
+
+    def time_to(curves, mask):
+        dev = [float(np.max(np.abs(cv[mask] / ref[mask] - 1.0))) for _, cv in curves]
+        for i in range(len(curves)):
+            if all(d < 0.02 for d in dev[i:]):
+                return curves[i][0]
+        return None
+
+    res = {}
+    for name, mask in windows.items():
+        tp, tl = time_to(plain[:-1], mask), time_to(learned, mask)
+        res[name] = {"plain_s": tp, "learned_s": tl,
+                     "learned_over_plain": (tl / tp) if (tp and tl) else None}
+    with open(out_path, "w") as f:
+        json.dump({"T": T.tolist(), "reference": ref.tolist(),
+                   "plain": [(t, cv.tolist()) for t, cv in plain],
+                   "learned": [(t, cv.tolist()) for t, cv in learned]}, f)
+    return res
+
+
+def batched(moves, per_size, launches):
+    """The mixed batch: LJ clusters N = 13, 19, 31, 38 in spheres of radius
+    2.5, 2.7, 3.0, 3.2 sigma and periodic WCA fluids N = 32, 64, 128, 256
+    with the packing fraction drawn per walker from U[0.3, 0.5]; `per_size`
+    walkers each, SAD min_T = 0.1, cube moves of half-width 0.05 sigma, FP64
+    energy refresh every 1024 moves.  Lanes per walker come from the
+    autotuner (timed on as many walkers as the batch holds); the whole batch
+    runs as one persistent launch captured in a HIP graph, timed against the
+    per-set launches of the same walkers."""
+    rng = np.random.default_rng(7)
+    specs = []
+    for n, R in ((13, 2.5), (19, 2.7), (31, 3.0), (38, 3.2)):
+        specs.append((dict(system=S.LJ, n=n, energy_bin=0.01, has_max_e=1, max_e=0.0), np.full(per_size, R)))
+    for n in (32, 64, 128, 256):
+        eta = rng.uniform(0.3, 0.5, per_size)
+        specs.append((dict(system=S.WCA, n=n, energy_bin=0.05, has_max_e=1, max_e=2.5 * n),
+                      np.cbrt(n / (6.0 * eta / np.pi))))
+    sets = []
+    for k, (kw, geom) in enumerate(specs):
+        sp = S.make_params(method=S.SAD, method_param=0.1, move_param=0.05, refresh=1024, **kw)
+        seeds = np.arange(per_size, dtype=np.uint64) + 100000 * k
+        # trial on as many walkers as the whole batch holds, past the start-up phase
+        trial = np.arange(len(specs) * per_size, dtype=np.uint64)
+        best, rates = S.autotune_lanes(sp, trial, np.resize(geom, len(trial)), warm_moves=20000, trial_moves=2000)
+        sp.lanes = best
+        log(f"[batched] {'LJ' if kw['system'] == S.LJ else 'WCA'}{kw['n']}: lanes {best} "
+            + " ".join(f"{g}:{r / 1e9:.2f}" for g, r in sorted(rates.items())))
+        sets.append((sp, seeds, geom))
+    out = {}
+    for mode in ("per_set", "graph"):
+        ctx = S.Context(0)
+        for sp, seeds, geom in sets:
+            ctx.add_set(sp, seeds, geom=geom)
+        ctx.init()
+        # one untimed launch (captures the graph)
+        if mode == "graph":
+            ctx.run_batched(moves, graph=True)
+        else:
+            ctx.run(moves)
+        ms = []
+        for _ in range(launches):
+            if mode == "graph":
+                ctx.run_batched(moves, graph=True)
+            else:
+                ctx.run(moves)
+            ms.append(ctx.last_run_ms())
+        ctx.close()
+        total = len(sets) * per_size * moves
+        out[mode] = {"moves_per_s": total / (np.mean(ms) / 1e3), "ms_per_launch": float(np.mean(ms))}
+        log(f"[batched] {mode}: {out[mode]['moves_per_s'] / 1e9:.3f} G moves/s")
+    out["lanes"] = {f"{'LJ' if sp.system == S.LJ else 'WCA'}{sp.n}": sp.lanes for sp, _, _ in sets}
+    return out
+
+
 def copy_probe(nbytes):
     """A plain device-to-device copy of known size, for scaling the HBM byte
     counters (FETCH_SIZE, WRITE_SIZE) of a profiler run."""
@@ -151,6 +299,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, help="only run a device copy of this many bytes")
+    ap.add_argument("--learned-cv", type=float, default=0.0,
+                    help="only run the LJ38 wall-time-to-Cv comparison with this GPU-time budget (s) per method")
+    ap.add_argument("--learned-cv-out", default="learned_cv.json")
+    ap.add_argument("--batched", type=int, default=0,
+                    help="only run the mixed LJ/WCA batch (this many moves per launch)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (walkers are independent; see DESIGN.md for sharding); "
@@ -158,6 +311,18 @@ def main():
         sys.exit(2)
     if a.copy_probe:
         copy_probe(a.copy_probe)
+        return
+    if a.batched > 0:
+        res = batched(a.batched, 4096, a.steps)
+        print(json.dumps({"metric": "mixed LJ/WCA SAD batch moves/s on 1 GPU", "unit": "moves/s",
+                          "higher_is_better": True, "walkers_per_size": 4096, "moves_per_launch": a.batched,
+                          "launches": a.steps, **res}))
+        return
+    if a.learned_cv > 0:
+        res = learned_cv(a.learned_cv, a.walkers, a.lanes, 1000000, a.learned_cv_out)
+        print(json.dumps({"metric": "LJ38 wall time to Cv(T) within 2% (plain vs learned)", "unit": "s",
+                          "higher_is_better": False, "budget_s": a.learned_cv, "walkers": a.walkers,
+                          "windows": res}))
         return
     main_run = run_gpu(S.FP32_PAIRS, a.walkers, a.lanes, a.warmup, a.steps, a.moves_per_step, "lj31 fp32 pairs")
     moves_per_launch = a.walkers * a.moves_per_step

@@ -21,7 +21,10 @@ __device__ __forceinline__ void batched_walker(const SetArgs &a, int walker, int
   e.store(walker);
 }
 
-__global__ __launch_bounds__(64) void k_batched(const SetArgs *sets, const BatchTile *tiles, uint64_t moves) {
+// Register budget of two waves per SIMD: the kernel holds every variant, and
+// without the bound the widest one would cap the whole batch at one wave.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_batched(
+    const SetArgs *sets, const BatchTile *tiles, uint64_t moves) {
   extern __shared__ double lds[];
   const BatchTile t = tiles[blockIdx.x];
   const SetArgs &a = sets[t.set];

@@ -133,10 +133,14 @@ struct Engine {
   PExtra *X;
   // system
   double *lx, *ly, *lz;
-  double x[KMAX], y[KMAX], z[KMAX];  // this lane's atoms (unused by the learned move, which reads LDS)
-  SADMC_DF double xk(int k) const { return MV ? lx[lane + G * k] : x[k]; }
-  SADMC_DF double yk(int k) const { return MV ? ly[lane + G * k] : y[k]; }
-  SADMC_DF double zk(int k) const { return MV ? lz[lane + G * k] : z[k]; }
+  // This lane's atoms in registers, or (kLdsPos) read from the LDS mirror:
+  // the learned move, WCA and groups with many atoms per lane keep their
+  // registers for the rest of the state.
+  static constexpr bool kLdsPos = MV == 1 || SYS == SYS_WCA || KMAX >= 8;
+  double x[KMAX], y[KMAX], z[KMAX];
+  SADMC_DF double xk(int k) const { return kLdsPos ? lx[lane + G * k] : x[k]; }
+  SADMC_DF double yk(int k) const { return kLdsPos ? ly[lane + G * k] : y[k]; }
+  SADMC_DF double zk(int k) const { return kLdsPos ? lz[lane + G * k] : z[k]; }
   uint64_t spins;
   double R2, L, halfL;
   // EnergyMC scalars
@@ -212,7 +216,7 @@ struct Engine {
         if (k < K) {
           int j = lane + G * k;
           const double vx = P[j], vy = P[a.npad + j], vz = P[2 * a.npad + j];
-          if (MV == 0) {
+          if (!kLdsPos) {
             x[k] = vx;
             y[k] = vy;
             z[k] = vz;
@@ -806,7 +810,7 @@ struct Engine {
     int slot = pw / G;
 #pragma unroll
     for (int k = 0; k < KMAX; k++) {
-      if (MV == 0 && lane == owner && k == slot) {
+      if (!kLdsPos && lane == owner && k == slot) {
         x[k] = ptx;
         y[k] = pty;
         z[k] = ptz;
