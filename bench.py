@@ -40,6 +40,9 @@ RADIUS = 3.0
 FLOP_PER_MOVE = 30 * (N_ATOMS - 1)  # 2(N-1) pair terms x 15 flop (division counted as one)
 FP32_VECTOR_PEAK = 157.3            # TFLOP/s, MI355X
 FP64_VECTOR_PEAK = 78.6
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_MOVE = 80                 # lnw at two bins, read-modify-write of hist, lnw, e_tot, e_sq
+PROFILE = os.path.join(ROOT, "profiles", "k_run_lj31_fp32_sad_16384w.md")
 WARMUP_MOVES = 2000                 # moves per warmup launch
 
 
@@ -62,6 +65,19 @@ def lj_energy_fp64(c):
     s = 1.0 / r2[iu]
     s3 = s * s * s
     return float(np.sum(4.0 * (s3 * s3 - s3)))
+
+
+def measured_traffic(a):
+    """HBM bytes per launch of the run kernel for this workload, from the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary kept in profiles/ (scaled by
+    the copy-kernel calibration there); None if it is not there."""
+    try:
+        for line in open(PROFILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def run_gpu(precision, walkers, lanes, warmup, steps, mps, label):
@@ -306,7 +322,7 @@ def main():
                     help="only run the mixed LJ/WCA batch (this many moves per launch)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py measures one GPU only (walkers are independent; see DESIGN.md for sharding); "
+        print(f"bench.py measures one GPU only (walkers are independent chains and shard by walker); "
               f"--gpus {a.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
     if a.copy_probe:
@@ -327,6 +343,8 @@ def main():
     main_run = run_gpu(S.FP32_PAIRS, a.walkers, a.lanes, a.warmup, a.steps, a.moves_per_step, "lj31 fp32 pairs")
     moves_per_launch = a.walkers * a.moves_per_step
     achieved = FLOP_PER_MOVE * moves_per_launch / (main_run["kernel_ms"] * 1e-3) / 1e12
+    achieved_gbs = BYTES_PER_MOVE * moves_per_launch / (main_run["kernel_ms"] * 1e-3) / 1e9
+    traffic = measured_traffic(a) if (a.walkers, a.moves_per_step) == (16384, 500000) else None
     run_moves = a.steps * a.moves_per_step
     variants = {}
     if not a.no_fp64:
@@ -355,9 +373,14 @@ def main():
             "walkers": a.walkers, "lanes_per_walker": a.lanes, "moves_per_step": a.moves_per_step,
             "timed_moves_per_walker": run_moves, "warmup_moves_per_step": WARMUP_MOVES,
         },
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_VECTOR_PEAK, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_VECTOR_PEAK, "traffic": None,
-                     "flop_per_move": FLOP_PER_MOVE, "kernel_ms_per_launch": main_run["kernel_ms"]},
+        # 900 flop / 80 B = 11.25 flop/B, under the FP32 ridge of 157.3 / 8 = 19.7 flop/B: the roofline that
+        # bounds this kernel is HBM; the VALU fraction is given beside it
+        "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic,
+                     "bytes_per_move": BYTES_PER_MOVE, "flop_per_move": FLOP_PER_MOVE,
+                     "kernel_ms_per_launch": main_run["kernel_ms"],
+                     "valu": {"achieved": achieved, "peak": FP32_VECTOR_PEAK, "unit": "TFLOP/s",
+                              "frac": achieved / FP32_VECTOR_PEAK}},
         "energy_error": {"max_rel": main_run["max_rel_energy_error"], "mean_rel": main_run["mean_rel_energy_error"]},
         "variants": variants,
     }

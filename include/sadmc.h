@@ -3,7 +3,8 @@
  *
  * C ABI.  Plain pointers, sizes and status codes; no C++ or torch types.
  * The reference's FFI-less Rust types this boundary stands in for are cited
- * on each entry point; INTEGRATION.md shows the Rust binding.
+ * on each entry point; a Rust caller declares these functions in an
+ * extern "C" block and links libsadmc_amd.so.
  *
  * Model.  A context owns one HIP stream and any number of walker *sets*.  A
  * set is a batch of independent walkers of one system kind and size (one

@@ -230,7 +230,7 @@ struct or_mc {
 
 /* ising.rs:45-49: spins are the low bit of successive draws, in site order.
  * rule 0 uses the reference's fixed seed 10137; rule 1 a per-walker stream
- * (BASELINE "i.i.d. from the seed"). */
+ * (spins i.i.d. from each walker's own seed). */
 void or_ising_init_spins(int32_t n, int32_t rule, uint64_t seed, int8_t *S) {
   uint64_t s[2];
   or_rng_seed(rule == 0 ? 10137ULL : (seed ^ OR_INIT_TAG), s);
@@ -253,8 +253,8 @@ double or_ising_energy(int32_t N, const int8_t *S) {
 
 /* ---- LJ (lj.rs) ------------------------------------------------------- */
 
-/* Initial positions (BASELINE rule, no reference counterpart; lj.rs:126-205
- * is the reference's own rule): atoms added one at a time, each drawn
+/* Initial positions (the workloads' rule, no reference counterpart;
+ * lj.rs:126-205 is the reference's own rule): atoms added one at a time, each drawn
  * uniformly in the sphere by rejection from the cube [-R,R)^3 and redrawn
  * until every earlier atom is at least 0.8 sigma away. */
 int or_lj_init(int32_t n, double radius, uint64_t seed, double *c) {
