@@ -918,11 +918,29 @@ struct Engine {
   }
 
   // Running max of t_found over logical bins [a, b].
-  SADMC_DF void tmax_scan(int32_t a, int32_t b) {
-    for (int32_t j = a; j <= b; j++) {
-      uint64_t t = Cb[base + j].tfound;
-      if (t > tmax) tmax = t;
+  // (The scans over bins that SAD's range changes need are split over the
+  // walker's G lanes: each lane takes every G-th bin, so the loads of one
+  // scan are in flight together; integer sums and maxima are then combined
+  // across the group, which leaves every result as the sequential one.)
+  SADMC_DF uint64_t group_max_u64(uint64_t v) const {
+    for (int d = 1; d < G; d <<= 1) {
+      const uint64_t o = __shfl_xor(v, d, 64);
+      v = o > v ? o : v;
     }
+    return v;
+  }
+  SADMC_DF uint32_t group_sum_u32(uint32_t v) const {
+    for (int d = 1; d < G; d <<= 1) v += __shfl_xor(v, d, 64);
+    return v;
+  }
+  SADMC_DF void tmax_scan(int32_t a, int32_t b) {
+    uint64_t m = 0;
+    for (int32_t j = a + lane; j <= b; j += G) {
+      const uint64_t t = Cb[base + j].tfound;
+      m = t > m ? t : m;
+    }
+    m = group_max_u64(m);
+    if (m > tmax) tmax = m;
   }
 
   SADMC_DF void set_lnw(int32_t p, double v) {
@@ -947,18 +965,26 @@ struct Engine {
         }
         const double vhi = lnw_hi;
         int32_t jlo = ihi > 0 ? ihi - 1 : 0, jhi = i + 1 < len ? i + 1 : len - 1;
-        for (int32_t j = jlo; j <= jhi; j++) {
-          double ej = i_energy(j);
-          if (ej > too_hi && ej <= E) {
-            int32_t pj = base + j;
-            if (hist_at(pj) != 0) {
-              set_lnw(pj, vhi);
-              Ns += 1;
-            } else {
-              set_lnw(pj, 0.0);
-            }
+        uint32_t cnt = 0;
+        for (int32_t j = jlo + lane; j <= jhi; j += G) {
+          const double ej = i_energy(j);
+          const int32_t pj = base + j;
+          if (ej > too_hi && ej <= E && pj != cur_p) {
+            const bool nz = H[pj].hist != 0;
+            H[pj].lnw = nz ? vhi : 0.0;
+            cnt += nz ? 1u : 0u;
           }
         }
+        cnt = group_sum_u32(cnt);
+        if (i >= jlo && i <= jhi) {  // the current bin, whose record every lane holds
+          const double ec = i_energy(i);
+          if (ec > too_hi && ec <= E) {
+            c_lnw = c_hist != 0 ? vhi : 0.0;
+            cnt += c_hist != 0 ? 1u : 0u;
+          }
+        }
+        Ns += cnt;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         lam = (E - too_lo) / min_T;
         tL = moves;
         double nh = i_energy(i);
@@ -975,24 +1001,41 @@ struct Engine {
           status = SADMC_WS_INDEX;
           return;
         }
-        const int32_t plo = base + ilo;
+        // every bin below too_lo extends from the old ln w(too_lo): the bin
+        // of too_lo itself comes last in energy order, and ln w(too_lo) is
+        // reloaded from the new range below
         int32_t jlo = i > 0 ? i - 1 : 0, jhi = ilo + 1 < len ? ilo + 1 : len - 1;
-        for (int32_t j = jlo; j <= jhi; j++) {
-          double ej = i_energy(j);
-          if (ej < too_lo && ej >= E) {
-            int32_t pj = base + j;
-            double v;
-            if (hist_at(pj) != 0) {
-              v = lnw_lo + (ej - too_lo) / min_T;
+        const double vlo = lnw_lo;
+        uint32_t cnt = 0;
+        for (int32_t j = jlo + lane; j <= jhi; j += G) {
+          const double ej = i_energy(j);
+          const int32_t pj = base + j;
+          if (ej < too_lo && ej >= E && pj != cur_p) {
+            const bool nz = H[pj].hist != 0;
+            double v = 0.0;
+            if (nz) {
+              v = vlo + (ej - too_lo) / min_T;
               if (v < 0.0) v = 0.0;
-              Ns += 1;
-            } else {
-              v = 0.0;
             }
-            set_lnw(pj, v);
-            if (pj == plo) lnw_lo = v;
+            H[pj].lnw = v;
+            cnt += nz ? 1u : 0u;
           }
         }
+        cnt = group_sum_u32(cnt);
+        if (i >= jlo && i <= jhi) {  // the current bin, whose record every lane holds
+          const double ec = i_energy(i);
+          if (ec < too_lo && ec >= E) {
+            double v = 0.0;
+            if (c_hist != 0) {
+              v = vlo + (ec - too_lo) / min_T;
+              if (v < 0.0) v = 0.0;
+            }
+            c_lnw = v;
+            cnt += c_hist != 0 ? 1u : 0u;
+          }
+        }
+        Ns += cnt;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         lam = (too_hi - E) / min_T;
         tL = moves;
         double nl = i_energy(i);
