@@ -143,10 +143,10 @@ def cpu_baseline(precision, moves, rounds):
                        f"reference's summation order, -O3 -march=native, {wall:.1f} s on {cpu}")
 
 
-def learned_cv(budget, walkers, lanes, chunk, out_path):
+def learned_cv(budget, walkers, lanes, chunk, out_path, system="lj38"):
     """Wall time to Cv(T) within 2% for LJ38 (sphere 3.2 sigma, SAD min_T =
-    0.03, 16384 walkers): plain cube-move walkers against learned-proposal
-    walkers.  The reference curve is the plain walkers' Cv after `budget`
+    0.03, 16384 walkers; or LJ13, sphere 2.5, min_T = 0.05): plain cube-move
+    walkers against learned-proposal walkers.  The reference curve is the plain walkers' Cv after `budget`
     seconds of GPU time.  The learned walkers are the same walkers: they run
     the cube move while plain-walker records are collected (64 per walker,
     1e6 in all), the network is trained, and then they continue with the
@@ -158,8 +158,10 @@ def learned_cv(budget, walkers, lanes, chunk, out_path):
     T = np.linspace(0.05, 0.40, 71)
     windows = {"T0.05-0.40": T >= 0.05, "T0.10-0.40": T >= 0.10, "T0.15-0.40": T >= 0.15}
 
+    n, radius, min_t = {"lj38": (38, 3.2, 0.03), "lj13": (13, 2.5, 0.05)}[system]
+
     def params(kind):
-        return S.make_params(system=S.LJ, n=38, lanes=lanes, method=S.SAD, method_param=0.03, move_param=0.1,
+        return S.make_params(system=S.LJ, n=n, lanes=lanes, method=S.SAD, method_param=min_t, move_param=0.1,
                              energy_bin=0.01, has_max_e=1, max_e=0.0, move_kind=kind)
 
     def checkpoint(ctx, sid, t, tag, curves):
@@ -175,7 +177,7 @@ def learned_cv(budget, walkers, lanes, chunk, out_path):
     seeds = np.arange(walkers, dtype=np.uint64)
     # plain walkers
     ctx = S.Context(0)
-    sid = ctx.add_set(params(S.MOVE_CUBE), seeds, geom=3.2)
+    sid = ctx.add_set(params(S.MOVE_CUBE), seeds, geom=radius)
     ctx.init()
     plain, t = [], 0.0
     while t < budget:
@@ -186,7 +188,7 @@ def learned_cv(budget, walkers, lanes, chunk, out_path):
     ref = plain[-1][1]
     # learned walkers: cube moves while records are collected, training, learned moves
     ctx = S.Context(0)
-    sid = ctx.add_set(params(S.MOVE_CUBE), seeds, geom=3.2)
+    sid = ctx.add_set(params(S.MOVE_CUBE), seeds, geom=radius)
     ctx.init()
     learned, t = [], 0.0
     t0 = time.perf_counter()
@@ -205,7 +207,7 @@ def learned_cv(budget, walkers, lanes, chunk, out_path):
     ctx.close()
     ctx = S.Context(0)
     ctx.prop_set_weights(w)
-    sid = ctx.add_set(params(S.MOVE_LEARNED), seeds, geom=3.2)
+    sid = ctx.add_set(params(S.MOVE_LEARNED), seeds, geom=radius)
     ctx.init()
     ctx.upload_raw(sid, st, bins, coords=coords)
     t += time.perf_counter() - tc
@@ -318,6 +320,7 @@ def main():
     ap.add_argument("--learned-cv", type=float, default=0.0,
                     help="only run the LJ38 wall-time-to-Cv comparison with this GPU-time budget (s) per method")
     ap.add_argument("--learned-cv-out", default="learned_cv.json")
+    ap.add_argument("--learned-cv-system", default="lj38", choices=["lj38", "lj13"])
     ap.add_argument("--batched", type=int, default=0,
                     help="only run the mixed LJ/WCA batch (this many moves per launch)")
     a = ap.parse_args()
@@ -335,8 +338,10 @@ def main():
                           "launches": a.steps, **res}))
         return
     if a.learned_cv > 0:
-        res = learned_cv(a.learned_cv, a.walkers, a.lanes, 1000000, a.learned_cv_out)
-        print(json.dumps({"metric": "LJ38 wall time to Cv(T) within 2% (plain vs learned)", "unit": "s",
+        lanes = a.lanes if a.learned_cv_system == "lj38" else 4
+        res = learned_cv(a.learned_cv, a.walkers, lanes, 1000000, a.learned_cv_out, a.learned_cv_system)
+        print(json.dumps({"metric": f"{a.learned_cv_system.upper()} wall time to Cv(T) within 2% (plain vs learned)",
+                          "unit": "s",
                           "higher_is_better": False, "budget_s": a.learned_cv, "walkers": a.walkers,
                           "windows": res}))
         return
