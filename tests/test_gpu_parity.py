@@ -189,3 +189,29 @@ def test_resume_is_bit_identical(gpu_ctx_factory):
     st_b, bins_b = b.download_raw(sb)
     assert st_a.tobytes() == st_b.tobytes()
     assert bins_a.tobytes() == bins_b.tobytes()
+
+
+CONT_METHODS = [
+    ("lj13_samc", S.LJ, 13, 2.5, 4, dict(method=S.SAMC, method_param=500.0, energy_bin=0.01, has_max_e=1,
+                                         max_e=0.0)),
+    ("lj13_wl", S.LJ, 13, 2.5, 4, dict(method=S.WL, energy_bin=0.05, has_min_e=1, min_e=-44.0, has_max_e=1,
+                                       max_e=0.0)),
+    ("lj13_inv_t_wl", S.LJ, 13, 2.5, 4, dict(method=S.INV_T_WL, energy_bin=0.05, has_min_e=1, min_e=-44.0,
+                                             has_max_e=1, max_e=0.0)),
+    ("lj19_canonical", S.LJ, 19, 2.7, 8, dict(method=S.CANONICAL, method_param=0.3, energy_bin=0.01,
+                                              has_max_e=1, max_e=0.0)),
+    ("lj13_fp32_canonical", S.LJ, 13, 2.5, 4, dict(method=S.CANONICAL, method_param=0.2, energy_bin=0.01,
+                                                   has_max_e=1, max_e=0.0, precision=S.FP32_PAIRS, refresh=1024)),
+    ("wca32_canonical", S.WCA, 32, 3.6, 8, dict(method=S.CANONICAL, method_param=1.0, energy_bin=0.05,
+                                                has_max_e=1, max_e=80.0)),
+    ("wca64_samc", S.WCA, 64, 4.3, 16, dict(method=S.SAMC, method_param=500.0, energy_bin=0.05, has_max_e=1,
+                                            max_e=160.0)),
+]
+
+
+@pytest.mark.parametrize("name,system,n,geom,lanes,kw", CONT_METHODS, ids=[c[0] for c in CONT_METHODS])
+def test_continuous_methods_bit_exact(gpu_ctx_factory, name, system, n, geom, lanes, kw):
+    """SAMC, WL, 1/t-WL and canonical sampling of the LJ and WCA systems."""
+    print(f"case {name}", flush=True)
+    sp = S.make_params(system=system, n=n, lanes=lanes, move_param=0.1 if system == S.LJ else 0.05, **kw)
+    _run_and_compare(gpu_ctx_factory, sp, np.arange(4, dtype=np.uint64) + 300, geom, 6000, chunks=2, label=name)
