@@ -30,6 +30,13 @@ class Params(C.Structure):
     ]
 
 
+class Replica(C.Structure):
+    _fields_ = [("E", C.c_double), ("error", C.c_double), ("T", C.c_double), ("total_energy", C.c_double),
+                ("total_energy_squared", C.c_double), ("accepted", C.c_uint64), ("rejected", C.c_uint64),
+                ("ignored", C.c_uint64), ("accepted_swaps", C.c_uint64), ("rejected_swaps", C.c_uint64),
+                ("rng", C.c_uint64 * 2)]
+
+
 class Scalars(C.Structure):
     _fields_ = [
         ("E", C.c_double), ("error", C.c_double), ("moves", C.c_uint64), ("accepted", C.c_uint64),
@@ -108,6 +115,14 @@ def lib():
     L.or_sad_gamma.argtypes = [C.c_double] * 4; L.or_sad_gamma.restype = C.c_double
     L.or_bench.argtypes = [C.POINTER(Params), C.c_int32, C.c_uint64, C.c_uint64]; L.or_bench.restype = C.c_double
     vp = C.c_void_p
+    L.or_rng_jump.argtypes = [u64p]
+    L.or_pt_new.argtypes = [C.POINTER(Params), C.c_int32, vp, C.c_uint64, vp, vp, C.c_int32]
+    L.or_pt_new.restype = vp
+    L.or_pt_run.argtypes = [vp, C.c_uint64]
+    L.or_pt_replica.argtypes = [vp, C.c_int32, C.POINTER(Replica)]
+    L.or_pt_coords.argtypes = [vp, C.c_int32, vp]
+    L.or_pt_rng.argtypes = [vp, u64p]
+    L.or_pt_free.argtypes = [vp]
     L.or_prop_features.argtypes = [C.c_int32, vp, C.c_int32, C.c_double, C.c_double, C.c_double, vp]
     L.or_prop_mlp.argtypes = [vp, vp]; L.or_prop_mlp.restype = C.c_float
     L.or_prop_gaussian.argtypes = [u64p, vp]
@@ -363,3 +378,47 @@ def sad_gamma(lam, t, tF, ns):
 
 def bench(params, threads, moves, seed0=0):
     return lib().or_bench(C.byref(params), threads, moves, seed0)
+
+
+class Tempering:
+    """A parallel-tempering ladder (src/mc/tempering.rs)."""
+
+    def __init__(self, params, temperatures, seed, coords=None, spins=None, canonical_steps=1):
+        T = np.ascontiguousarray(temperatures, dtype=np.float64)
+        c = None if coords is None else np.ascontiguousarray(coords, dtype=np.float64)
+        sp = None if spins is None else np.ascontiguousarray(spins, dtype=np.int8)
+        self.R = len(T)
+        self.n = params.n
+        self.system = params.system
+        self.h = lib().or_pt_new(C.byref(params), self.R, _ptr(T), seed, _ptr(c), _ptr(sp), canonical_steps)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_pt_free(self.h)
+            self.h = None
+
+    def run(self, rounds):
+        lib().or_pt_run(self.h, rounds)
+
+    def replica(self, r):
+        out = Replica()
+        lib().or_pt_replica(self.h, r, C.byref(out))
+        d = {name: getattr(out, name) for name, _ in Replica._fields_}
+        d["rng"] = list(out.rng)
+        return d
+
+    def coords(self, r):
+        c = np.zeros(3 * self.n)
+        lib().or_pt_coords(self.h, r, _ptr(c))
+        return c
+
+    def rng(self):
+        s = (C.c_uint64 * 2)()
+        lib().or_pt_rng(self.h, s)
+        return list(s)
+
+
+def rng_jump(state):
+    s = (C.c_uint64 * 2)(*state)
+    lib().or_rng_jump(s)
+    return list(s)

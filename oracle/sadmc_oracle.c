@@ -1523,6 +1523,144 @@ static void *bench_thread(void *v) {
   return NULL;
 }
 
+/* ---- parallel tempering (tempering.rs) ------------------------------- */
+
+/* The xoroshiro128+ jump of 2^64 steps (the published jump polynomial). */
+void or_rng_jump(uint64_t s[2]) {
+  static const uint64_t J[2] = {0xDF900294D8F554A5ULL, 0x170865DF4B3201FCULL};
+  uint64_t a = 0, b = 0;
+  for (int w = 0; w < 2; w++)
+    for (int k = 0; k < 64; k++) {
+      if ((J[w] >> k) & 1ULL) {
+        a ^= s[0];
+        b ^= s[1];
+      }
+      or_rng_next(s);
+    }
+  s[0] = a;
+  s[1] = b;
+}
+
+struct or_pt {
+  int R, steps;
+  or_mc **rep;
+  or_replica *st;
+  uint64_t rng[2];
+  double scale;
+};
+
+static uint64_t sys_moves_to_randomize(const or_mc *m) {
+  return m->sys == OR_ISING ? (uint64_t)m->n * (uint64_t)m->n : (uint64_t)m->n;
+}
+
+or_pt *or_pt_new(const or_params *p, int32_t R, const double *T, uint64_t seed, const double *coords,
+                 const int8_t *spins, int32_t canonical_steps) {
+  or_pt *pt = (or_pt *)calloc(1, sizeof(or_pt));
+  pt->R = R;
+  pt->rep = (or_mc **)calloc((size_t)R, sizeof(or_mc *));
+  pt->st = (or_replica *)calloc((size_t)R, sizeof(or_replica));
+  or_params q = *p;
+  q.seed = seed;
+  q.has_max_e = 0;
+  q.has_min_e = 0;
+  q.method = OR_CANONICAL;
+  q.method_param = 1.0;
+  for (int r = 0; r < R; r++) {
+    pt->rep[r] = or_mc_new(&q, coords, spins);  /* the generator is seed_from_u64(seed) */
+    pt->st[r].T = T[r];
+  }
+  pt->steps = (int)(sys_moves_to_randomize(pt->rep[0]) * (uint64_t)canonical_steps);
+  pt->scale = p->move_param;
+  or_rng_seed(seed, pt->rng);
+  or_rng_jump(pt->rng);
+  return pt;
+}
+
+static void pt_collect(or_replica *s, double e) {
+  s->total_energy += e;
+  s->total_energy_squared += e * e;
+  if (e >= 0.0) s->ignored += 1;
+}
+
+static void pt_swap_systems(or_mc *a, or_mc *b) {
+  double *t;
+  int8_t *u;
+  double E = a->E, err = a->error;
+  a->E = b->E; a->error = b->error; b->E = E; b->error = err;
+  t = a->x; a->x = b->x; b->x = t;
+  t = a->y; a->y = b->y; b->y = t;
+  t = a->z; a->z = b->z; b->z = t;
+  u = a->S; a->S = b->S; b->S = u;
+}
+
+void or_pt_run(or_pt *pt, uint64_t rounds) {
+  for (uint64_t k = 0; k < rounds; k++) {
+    for (int r = 0; r < pt->R; r++) {  /* tempering.rs:96-112 */
+      or_mc *m = pt->rep[r];
+      or_replica *s = &pt->st[r];
+      for (int i = 0; i < pt->steps; i++) {
+        double e;
+        if (!sys_plan_move(m, pt->scale, &e)) continue;
+        double bde = (e - m->E) / s->T;
+        if (bde < 0.0 || or_gen_f64(m->rng) < exp(-bde)) {
+          sys_confirm(m);
+          s->accepted += 1;
+        } else {
+          s->rejected += 1;
+        }
+        pt_collect(s, m->E);
+      }
+    }
+    /* tempering.rs:288-318: rand 0.7's bool is the top bit of next_u32, and
+     * rand_xoshiro's next_u32 the high half of next_u64 */
+    int first = (or_rng_next(pt->rng) >> 63) ? 0 : 1;
+    for (int i = first; i + 1 < pt->R; i += 2) {
+      or_mc *m0 = pt->rep[i], *m1 = pt->rep[i + 1];
+      or_replica *s0 = &pt->st[i], *s1 = &pt->st[i + 1];
+      double de_db = (m0->E - m1->E) * (1.0 / s0->T - 1.0 / s1->T);
+      if (de_db >= 0.0 || or_gen_f64(m1->rng) < exp(de_db)) {
+        s0->accepted_swaps += 1;
+        s1->accepted_swaps += 1;
+        pt_swap_systems(m0, m1);
+      } else {
+        s0->rejected_swaps += 1;
+        s1->rejected_swaps += 1;
+      }
+      pt_collect(s0, m0->E);
+      pt_collect(s1, m1->E);
+    }
+  }
+}
+
+void or_pt_replica(const or_pt *pt, int32_t r, or_replica *out) {
+  *out = pt->st[r];
+  out->E = pt->rep[r]->E;
+  out->error = pt->rep[r]->error;
+  out->rng[0] = pt->rep[r]->rng[0];
+  out->rng[1] = pt->rep[r]->rng[1];
+}
+
+void or_pt_coords(const or_pt *pt, int32_t r, double *c) {
+  const or_mc *m = pt->rep[r];
+  if (m->sys == OR_ISING) return;
+  memcpy(c, m->x, sizeof(double) * m->n);
+  memcpy(c + m->n, m->y, sizeof(double) * m->n);
+  memcpy(c + 2 * m->n, m->z, sizeof(double) * m->n);
+}
+
+void or_pt_rng(const or_pt *pt, uint64_t out[2]) {
+  out[0] = pt->rng[0];
+  out[1] = pt->rng[1];
+}
+
+void or_pt_free(or_pt *pt) {
+  if (!pt) return;
+  for (int r = 0; r < pt->R; r++) or_mc_free(pt->rep[r]);
+  free(pt->rep);
+  free(pt->st);
+  free(pt);
+}
+
 double or_bench(const or_params *p, int32_t threads, uint64_t moves, uint64_t seed0) {
   pthread_t *t = (pthread_t *)malloc(sizeof(pthread_t) * threads);
   bench_arg *a = (bench_arg *)malloc(sizeof(bench_arg) * threads);

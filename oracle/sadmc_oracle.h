@@ -185,6 +185,31 @@ void or_prop_probe(const or_mc *mc, uint64_t seed, int32_t count, float *out);
  * e_new, ln H and the moved atom's new position are written. */
 int or_mc_plan_learned(or_mc *mc, double *e_new, double *lnh, double *to);
 
+/* ---- parallel tempering (src/mc/tempering.rs) --------------------------
+ * One ladder of R replicas at temperatures T[0..R), all starting from the
+ * same configuration with clones of one generator seeded by `seed`
+ * (tempering.rs:147-158); the ladder's own generator is that one after a
+ * 2^64-step jump.  A round (run_once, tempering.rs:272-341) makes
+ * canonical_steps * min_moves_to_randomize moves per replica (accept when
+ * beta dE < 0 or u < exp(-beta dE), tempering.rs:96-112), then, with pairs
+ * (0,1),(2,3),... or (1,2),(3,4),... chosen by the ladder generator's
+ * bool, swaps the systems of a pair when d = (E0 - E1)(1/T0 - 1/T1) >= 0
+ * or u < exp(d) with u from the second replica's generator. */
+typedef struct {
+  double E, error, T, total_energy, total_energy_squared;
+  uint64_t accepted, rejected, ignored, accepted_swaps, rejected_swaps;
+  uint64_t rng[2];
+} or_replica;
+typedef struct or_pt or_pt;
+void or_rng_jump(uint64_t s[2]);
+or_pt *or_pt_new(const or_params *p, int32_t replicas, const double *T, uint64_t seed, const double *coords,
+                 const int8_t *spins, int32_t canonical_steps);
+void or_pt_run(or_pt *pt, uint64_t rounds);
+void or_pt_replica(const or_pt *pt, int32_t r, or_replica *out);
+void or_pt_coords(const or_pt *pt, int32_t r, double *coords);
+void or_pt_rng(const or_pt *pt, uint64_t out[2]);
+void or_pt_free(or_pt *pt);
+
 /* CPU baseline: run `threads` independent walkers (one per thread) for
  * `moves` moves each; returns wall seconds (init excluded). */
 double or_bench(const or_params *p, int32_t threads, uint64_t moves, uint64_t seed0);
