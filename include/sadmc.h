@@ -129,6 +129,16 @@ typedef struct {
   uint64_t histogram, t_found, round_trips, visit_epoch, wl_hist;
 } sadmc_bin;
 
+/* One replica of a parallel-tempering ladder (tempering.rs:44-71). */
+typedef struct {
+  uint64_t rng[2];              /* the replica's generator */
+  double E, error, T;           /* its system's energy and error estimate; its temperature */
+  double total_energy, total_energy_squared;
+  uint64_t accepted, rejected, ignored, accepted_swaps, rejected_swaps;
+  double geom;                  /* sphere radius or box edge of the ladder */
+  int32_t status, reserved0;
+} sadmc_replica;
+
 typedef struct sadmc_ctx sadmc_ctx;
 
 /* ---- context --------------------------------------------------------- */
@@ -215,6 +225,27 @@ int sadmc_prop_train(sadmc_ctx *ctx, int32_t steps, int32_t batch, float lr, flo
  * returns SADMC_E_CAPACITY (with *nbins set) if max_bins is too small. */
 int sadmc_mean_entropy(sadmc_ctx *ctx, int32_t set, double e_a, double e_b, int64_t max_bins, double *e_first,
                        int64_t *nbins, double *S, int64_t *count);
+
+/* ---- parallel tempering (src/mc/tempering.rs) ---------------------------
+ * `ladders` independent ladders of `replicas` canonical replicas each, at
+ * temperatures temps[0..replicas).  Ladder l starts every replica from one
+ * configuration (coords/spins of ladder l, or the init rule with seeds[l])
+ * and every replica's generator from seed_from_u64(seeds[l]); the ladder's
+ * own generator is that one jumped by 2^64 (tempering.rs:147-158).  A round
+ * (MC::run_once, tempering.rs:272-341) makes canonical_steps *
+ * min_moves_to_randomize moves per replica (Replica::run_once, :96-112),
+ * cube moves of half-width p->move_param, then tries the odd or even
+ * neighbour swaps.  One workgroup per ladder: replicas * lanes <= 1024. */
+int sadmc_pt_add(sadmc_ctx *ctx, const sadmc_set_params *p, int32_t ladders, int32_t replicas, const double *temps,
+                 const uint64_t *seeds, const double *geom, const double *coords, const int8_t *spins,
+                 int32_t canonical_steps, int32_t *pt_id);
+int sadmc_pt_run(sadmc_ctx *ctx, int32_t pt, uint64_t rounds);
+/* ladders * replicas records, ladder-major. */
+int sadmc_pt_replicas(sadmc_ctx *ctx, int32_t pt, sadmc_replica *out);
+int sadmc_pt_coords(sadmc_ctx *ctx, int32_t pt, int32_t ladder, int32_t replica, double *coords);
+int sadmc_pt_spins(sadmc_ctx *ctx, int32_t pt, int32_t ladder, int32_t replica, int8_t *spins);
+/* The ladders' own generators: 2 words per ladder. */
+int sadmc_pt_ladder_rng(sadmc_ctx *ctx, int32_t pt, uint64_t *out);
 
 /* ---- state: the serde view of EnergyMC (mc/mod.rs:111-120) ----------- */
 int sadmc_walker_count(const sadmc_ctx *ctx, int32_t set, int32_t *n_walkers, int64_t *bin_capacity);

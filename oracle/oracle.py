@@ -122,6 +122,7 @@ def lib():
     L.or_pt_replica.argtypes = [vp, C.c_int32, C.POINTER(Replica)]
     L.or_pt_coords.argtypes = [vp, C.c_int32, vp]
     L.or_pt_rng.argtypes = [vp, u64p]
+    L.or_pt_spins.argtypes = [vp, C.c_int32, vp]
     L.or_pt_free.argtypes = [vp]
     L.or_prop_features.argtypes = [C.c_int32, vp, C.c_int32, C.c_double, C.c_double, C.c_double, vp]
     L.or_prop_mlp.argtypes = [vp, vp]; L.or_prop_mlp.restype = C.c_float
@@ -411,6 +412,11 @@ class Tempering:
         c = np.zeros(3 * self.n)
         lib().or_pt_coords(self.h, r, _ptr(c))
         return c
+
+    def spins(self, r):
+        s = np.zeros(self.n * self.n, np.int8)
+        lib().or_pt_spins(self.h, r, _ptr(s))
+        return s
 
     def rng(self):
         s = (C.c_uint64 * 2)()

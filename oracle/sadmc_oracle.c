@@ -1648,6 +1648,8 @@ void or_pt_coords(const or_pt *pt, int32_t r, double *c) {
   memcpy(c + 2 * m->n, m->z, sizeof(double) * m->n);
 }
 
+void or_pt_spins(const or_pt *pt, int32_t r, int8_t *s) { or_mc_spins(pt->rep[r], s); }
+
 void or_pt_rng(const or_pt *pt, uint64_t out[2]) {
   out[0] = pt->rng[0];
   out[1] = pt->rng[1];

@@ -207,6 +207,7 @@ or_pt *or_pt_new(const or_params *p, int32_t replicas, const double *T, uint64_t
 void or_pt_run(or_pt *pt, uint64_t rounds);
 void or_pt_replica(const or_pt *pt, int32_t r, or_replica *out);
 void or_pt_coords(const or_pt *pt, int32_t r, double *coords);
+void or_pt_spins(const or_pt *pt, int32_t r, int8_t *spins);
 void or_pt_rng(const or_pt *pt, uint64_t out[2]);
 void or_pt_free(or_pt *pt);
 

@@ -48,6 +48,23 @@ SADMC_HD uint64_t rng_next(uint64_t &s0, uint64_t &s1) {
   return r;
 }
 
+// The 2^64-step jump of xoroshiro128+ (the published jump polynomial), as
+// rand_xoshiro's jump() (tempering.rs:158).
+SADMC_HD void rng_jump(uint64_t &s0, uint64_t &s1) {
+  const uint64_t J[2] = {0xDF900294D8F554A5ULL, 0x170865DF4B3201FCULL};
+  uint64_t a = 0, b = 0;
+  for (int w = 0; w < 2; w++)
+    for (int k = 0; k < 64; k++) {
+      if ((J[w] >> k) & 1ULL) {
+        a ^= s0;
+        b ^= s1;
+      }
+      rng_next(s0, s1);
+    }
+  s0 = a;
+  s1 = b;
+}
+
 // rng.gen_range(low, high) (src/system/ising.rs:105-106): rand 0.7
 // UniformInt::sample_single, conservative zone, widening multiply.
 SADMC_HD uint64_t gen_range(uint64_t &s0, uint64_t &s1, uint64_t low, uint64_t high) {

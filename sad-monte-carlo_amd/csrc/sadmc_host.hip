@@ -69,6 +69,17 @@ struct SetHost {
   SetArgs args;
 };
 
+// A parallel-tempering ladder set on the device.
+struct PtHost {
+  int L = 0, R = 0, G = 1, K = 1, KMAX = 1, npad = 0, threads = 64, sys = 0;
+  SetArgs args;
+  PtArgs pa;
+  sadmc_replica *d_rep = nullptr;
+  double *d_slot_e = nullptr, *d_pos = nullptr, *d_geom = nullptr;
+  int32_t *d_slot_of = nullptr;
+  uint64_t *d_spins = nullptr, *d_lrng = nullptr;
+};
+
 }  // namespace
 
 struct sadmc_ctx {
@@ -98,6 +109,7 @@ struct sadmc_ctx {
   int partial_blocks = 0;
   float *d_grad = nullptr, *d_loss = nullptr;
   int loss_cap = 0;
+  std::vector<PtHost> pts;
 };
 
 static int fail(sadmc_ctx *c, int code, const std::string &msg) {
@@ -323,6 +335,10 @@ void sadmc_destroy(sadmc_ctx *c) {
   for (void *p : {(void *)c->d_pw, (void *)c->d_pm, (void *)c->d_pv, (void *)c->d_pbf, (void *)c->d_rec,
                   (void *)c->d_partial, (void *)c->d_grad, (void *)c->d_loss})
     (void)hipFree(p);
+  for (auto &t : c->pts)
+    for (void *p : {(void *)t.d_rep, (void *)t.d_slot_e, (void *)t.d_pos, (void *)t.d_geom, (void *)t.d_slot_of,
+                    (void *)t.d_spins, (void *)t.d_lrng})
+      (void)hipFree(p);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1118,6 +1134,216 @@ int sadmc_upload_raw(sadmc_ctx *c, int32_t set, const sadmc_walker_state *states
   }
   int32_t zero = 0;
   HIPCHECK(c, hipMemcpy(s.d_flag, &zero, sizeof zero, hipMemcpyHostToDevice));
+  return SADMC_OK;
+}
+
+// ------------------------------------------------------ parallel tempering
+
+int sadmc_pt_add(sadmc_ctx *c, const sadmc_set_params *p, int32_t L, int32_t R, const double *temps,
+                 const uint64_t *seeds, const double *geom, const double *coords, const int8_t *spins,
+                 int32_t canonical_steps, int32_t *pt_id) {
+  if (!c || !p || !temps || !seeds || L <= 0 || R <= 0) return fail(c, SADMC_E_ARG, "null argument or empty ladder");
+  if (canonical_steps <= 0) return fail(c, SADMC_E_ARG, "canonical_steps must be > 0");
+  if (p->precision != SADMC_FP64 || p->refresh != 0 || p->move_kind != SADMC_MOVE_CUBE)
+    return fail(c, SADMC_E_ARG, "parallel tempering runs the cube move in FP64 with the error-estimate energy rule");
+  for (int r = 0; r < R; r++)
+    if (!(temps[r] > 0)) return fail(c, SADMC_E_ARG, "replica temperatures must be > 0");
+  PtHost t;
+  t.L = L;
+  t.R = R;
+  const int n = p->n;
+  if (p->system == SADMC_ISING) {
+    if (n < 2 || n > 8) return fail(c, SADMC_E_ARG, "Ising lattice width must be 2..8");
+    t.sys = SYS_ISING;
+  } else if (p->system == SADMC_LJ || p->system == SADMC_WCA) {
+    if (n < 2 || n > 1024) return fail(c, SADMC_E_ARG, "atom count must be 2..1024");
+    if (!geom) return fail(c, SADMC_E_ARG, "LJ/WCA need per-ladder geometry");
+    if (!(p->move_param > 0)) return fail(c, SADMC_E_ARG, "the move half-width must be > 0");
+    t.sys = p->system == SADMC_LJ ? SYS_LJ : SYS_WCA;
+    int G = p->lanes > 0 ? p->lanes : auto_lanes(n);
+    if (G < 2 || G > 64 || (G & (G - 1))) return fail(c, SADMC_E_ARG, "lanes must be a power of two in 2..64");
+    t.G = G;
+    t.K = (n + G - 1) / G;
+    if (t.K > 16) return fail(c, SADMC_E_ARG, "too few lanes for this many atoms (max 16 atoms per lane)");
+    t.KMAX = t.K <= 1 ? 1 : t.K <= 2 ? 2 : t.K <= 4 ? 4 : t.K <= 8 ? 8 : 16;
+    t.npad = G * t.K;
+  } else {
+    return fail(c, SADMC_E_ARG, "unknown system");
+  }
+  if ((int64_t)R * t.G > 512) return fail(c, SADMC_E_ARG, "replicas * lanes must be <= 512 (one workgroup per ladder)");
+  if (pt_lds_bytes(R, t.npad) > 65536) return fail(c, SADMC_E_ARG, "a ladder's systems do not fit 64 KiB of LDS");
+  t.threads = ((R * t.G + 63) / 64) * 64;
+  const int64_t S = (int64_t)L * R;
+  // initial configurations: one per ladder, copied to every replica
+  std::vector<uint64_t> bits;
+  std::vector<double> pos;
+  if (t.sys == SYS_ISING) {
+    bits.assign(S, 0);
+    std::vector<int8_t> sp((size_t)n * n);
+    for (int l = 0; l < L; l++) {
+      if (spins)
+        std::memcpy(sp.data(), spins + (size_t)l * n * n, (size_t)n * n);
+      else
+        sadmc_init_spins(n, p->ising_init, seeds[l], sp.data());
+      uint64_t b = 0;
+      for (int k = 0; k < n * n; k++)
+        if (sp[k] > 0) b |= 1ULL << k;
+      for (int r = 0; r < R; r++) bits[(size_t)l * R + r] = b;
+    }
+  } else {
+    pos.assign((size_t)S * 3 * t.npad, 0.0);
+    std::vector<double> cf((size_t)3 * n);
+    for (int l = 0; l < L; l++) {
+      if (coords) {
+        std::memcpy(cf.data(), coords + (size_t)l * 3 * n, sizeof(double) * 3 * n);
+      } else if (sadmc_init_coords(p->system, n, geom[l], seeds[l], cf.data())) {
+        return fail(c, SADMC_E_INIT, "initial configuration failed");
+      }
+      for (int r = 0; r < R; r++)
+        for (int d = 0; d < 3; d++)
+          for (int j = 0; j < n; j++) pos[(((size_t)l * R + r) * 3 + d) * t.npad + j] = cf[(size_t)d * n + j];
+    }
+  }
+  // replicas: clones of seed_from_u64(seed); the ladder's generator jumped
+  std::vector<sadmc_replica> rep(S);
+  std::vector<int32_t> slot_of(S);
+  std::vector<uint64_t> lrng(2 * (size_t)L);
+  for (int l = 0; l < L; l++) {
+    uint64_t s0, s1;
+    rng_seed(seeds[l], s0, s1);
+    for (int r = 0; r < R; r++) {
+      sadmc_replica &q = rep[(size_t)l * R + r];
+      std::memset(&q, 0, sizeof q);
+      q.rng[0] = s0;
+      q.rng[1] = s1;
+      q.T = temps[r];
+      q.geom = geom ? geom[l] : 0.0;
+      slot_of[(size_t)l * R + r] = r;
+    }
+    rng_jump(s0, s1);
+    lrng[2 * l] = s0;
+    lrng[2 * l + 1] = s1;
+  }
+  HIPCHECK(c, hipSetDevice(c->device));
+  HIPCHECK(c, hipMalloc(&t.d_rep, sizeof(sadmc_replica) * S));
+  HIPCHECK(c, hipMalloc(&t.d_slot_e, sizeof(double) * 2 * S));
+  HIPCHECK(c, hipMalloc(&t.d_slot_of, sizeof(int32_t) * S));
+  HIPCHECK(c, hipMalloc(&t.d_lrng, sizeof(uint64_t) * 2 * L));
+  HIPCHECK(c, hipMemcpyAsync(t.d_rep, rep.data(), sizeof(sadmc_replica) * S, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(t.d_slot_of, slot_of.data(), sizeof(int32_t) * S, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(t.d_lrng, lrng.data(), sizeof(uint64_t) * 2 * L, hipMemcpyHostToDevice, c->stream));
+  if (t.sys == SYS_ISING) {
+    HIPCHECK(c, hipMalloc(&t.d_spins, sizeof(uint64_t) * S));
+    HIPCHECK(c, hipMemcpyAsync(t.d_spins, bits.data(), sizeof(uint64_t) * S, hipMemcpyHostToDevice, c->stream));
+  } else {
+    HIPCHECK(c, hipMalloc(&t.d_pos, sizeof(double) * pos.size()));
+    HIPCHECK(c, hipMemcpyAsync(t.d_pos, pos.data(), sizeof(double) * pos.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMalloc(&t.d_geom, sizeof(double) * L));
+    HIPCHECK(c, hipMemcpyAsync(t.d_geom, geom, sizeof(double) * L, hipMemcpyHostToDevice, c->stream));
+  }
+  SetArgs &a = t.args;
+  std::memset(&a, 0, sizeof a);
+  a.pos = t.d_pos;
+  a.spins = t.d_spins;
+  a.W = (int32_t)S;
+  a.n = n;
+  a.npad = t.npad;
+  a.G = t.G;
+  a.K = t.K;
+  a.method = SADMC_CANONICAL;
+  a.precision = SADMC_FP64;
+  a.move_kind = SADMC_MOVE_CUBE;
+  a.rc2 = p->rc2;
+  a.width = 1.0;
+  a.inv_width = 1.0;
+  const uint64_t nn = (uint64_t)n;
+  a.zone_n = ~0ULL - ((~0ULL - nn + 1) % nn);
+  a.zone_site = (nn << __builtin_clzll(nn)) - 1;
+  PtArgs &pa = t.pa;
+  pa.rep = t.d_rep;
+  pa.slot_e = t.d_slot_e;
+  pa.slot_of = t.d_slot_of;
+  pa.lrng = t.d_lrng;
+  pa.geom = t.d_geom;
+  pa.R = R;
+  const int64_t steps = (t.sys == SYS_ISING ? (int64_t)n * n : (int64_t)n) * canonical_steps;
+  if (steps > INT32_MAX) return fail(c, SADMC_E_ARG, "too many moves per round");
+  pa.steps = (int32_t)steps;
+  pa.d = p->move_param;
+  pa.sc = t.sys == SYS_ISING ? 0.0 : uniform_scale(-p->move_param, p->move_param);
+  HIPCHECK(c, launch_pt(t.sys, t.KMAX, a, pa, L, t.threads, 0, 1, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  c->pts.push_back(t);
+  if (pt_id) *pt_id = (int32_t)c->pts.size() - 1;
+  return SADMC_OK;
+}
+
+static PtHost *pt_get(sadmc_ctx *c, int32_t pt) {
+  if (!c || pt < 0 || pt >= (int32_t)c->pts.size()) return nullptr;
+  return &c->pts[pt];
+}
+
+int sadmc_pt_run(sadmc_ctx *c, int32_t pt, uint64_t rounds) {
+  PtHost *t = pt_get(c, pt);
+  if (!t) return fail(c, SADMC_E_ARG, "no such tempering ladder set");
+  if (!rounds) return SADMC_OK;
+  HIPCHECK(c, hipSetDevice(c->device));
+  HIPCHECK(c, launch_pt(t->sys, t->KMAX, t->args, t->pa, t->L, t->threads, rounds, 0, c->stream));
+  return SADMC_OK;
+}
+
+int sadmc_pt_replicas(sadmc_ctx *c, int32_t pt, sadmc_replica *out) {
+  PtHost *t = pt_get(c, pt);
+  if (!t || !out) return fail(c, SADMC_E_ARG, "no such tempering ladder set or null output");
+  HIPCHECK(c, hipMemcpyAsync(out, t->d_rep, sizeof(sadmc_replica) * t->L * t->R, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  return SADMC_OK;
+}
+
+static int pt_slot(sadmc_ctx *c, PtHost *t, int32_t l, int32_t r, int64_t *slot) {
+  if (l < 0 || l >= t->L || r < 0 || r >= t->R) return fail(c, SADMC_E_ARG, "ladder or replica out of range");
+  int32_t s = 0;
+  HIPCHECK(c, hipMemcpyAsync(&s, t->d_slot_of + (int64_t)l * t->R + r, sizeof s, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  *slot = (int64_t)l * t->R + s;
+  return SADMC_OK;
+}
+
+int sadmc_pt_coords(sadmc_ctx *c, int32_t pt, int32_t l, int32_t r, double *coords) {
+  PtHost *t = pt_get(c, pt);
+  if (!t || !coords) return fail(c, SADMC_E_ARG, "no such tempering ladder set or null output");
+  if (t->sys == SYS_ISING) return fail(c, SADMC_E_ARG, "Ising replicas have spins, not coordinates");
+  int64_t slot;
+  int rc = pt_slot(c, t, l, r, &slot);
+  if (rc) return rc;
+  const int n = t->args.n;
+  for (int d = 0; d < 3; d++)
+    HIPCHECK(c, hipMemcpyAsync(coords + (size_t)d * n, t->d_pos + (slot * 3 + d) * t->npad, sizeof(double) * n,
+                               hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  return SADMC_OK;
+}
+
+int sadmc_pt_spins(sadmc_ctx *c, int32_t pt, int32_t l, int32_t r, int8_t *spins) {
+  PtHost *t = pt_get(c, pt);
+  if (!t || !spins) return fail(c, SADMC_E_ARG, "no such tempering ladder set or null output");
+  if (t->sys != SYS_ISING) return fail(c, SADMC_E_ARG, "only Ising replicas have spins");
+  int64_t slot;
+  int rc = pt_slot(c, t, l, r, &slot);
+  if (rc) return rc;
+  uint64_t b = 0;
+  HIPCHECK(c, hipMemcpyAsync(&b, t->d_spins + slot, sizeof b, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  const int n = t->args.n;
+  for (int k = 0; k < n * n; k++) spins[k] = ((b >> k) & 1ULL) ? 1 : -1;
+  return SADMC_OK;
+}
+
+int sadmc_pt_ladder_rng(sadmc_ctx *c, int32_t pt, uint64_t *out) {
+  PtHost *t = pt_get(c, pt);
+  if (!t || !out) return fail(c, SADMC_E_ARG, "no such tempering ladder set or null output");
+  HIPCHECK(c, hipMemcpyAsync(out, t->d_lrng, sizeof(uint64_t) * 2 * t->L, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
   return SADMC_OK;
 }
 

@@ -54,6 +54,25 @@ hipError_t launch_prop_update(const float *partial, int nblk, int count, float *
                               uint16_t *wbf, float lr, int step, float *grad_out, float *loss_out, hipStream_t s);
 hipError_t launch_wca(int meth, int kmax, KernelKind kk, const SetArgs &a, const LaunchArgs &la, hipStream_t s);
 
+// Parallel-tempering ladders (k_pt.hip).  The systems stay in their slots
+// (slot s of ladder l is walker l * R + s of the SetArgs); a swap exchanges
+// which replica (temperature, generator, counters) a slot's system belongs
+// to, which is the reference's exchange of systems seen from the systems.
+struct PtArgs {
+  sadmc_replica *rep;  // [L][R] by replica
+  double *slot_e;      // [L][R][2] energy and error estimate of each slot's system
+  int32_t *slot_of;    // [L][R] the slot holding replica r's system
+  uint64_t *lrng;      // [L][2] the ladders' generators
+  const double *geom;  // [L]
+  int32_t R, steps;
+  double d, sc;        // cube half-width and its Uniform scale
+};
+// One workgroup of `threads` lanes per ladder; `init` computes each slot's
+// energy instead of running rounds.
+size_t pt_lds_bytes(int R, int npad);
+hipError_t launch_pt(int sys, int kmax, const SetArgs &a, const PtArgs &p, int ladders, int threads, uint64_t rounds,
+                     int init, hipStream_t s);
+
 #ifdef SADMC_KERNEL_TU
 
 template <int SYS, int KMAX, int PREC, int METH>

@@ -67,6 +67,13 @@ BIN_DTYPE = np.dtype([
 ])
 
 
+# sadmc_replica: one replica of a tempering ladder.
+REPLICA_DTYPE = np.dtype([
+    ("rng", "<u8", 2), ("E", "<f8"), ("error", "<f8"), ("T", "<f8"), ("total_energy", "<f8"),
+    ("total_energy_squared", "<f8"), ("accepted", "<u8"), ("rejected", "<u8"), ("ignored", "<u8"),
+    ("accepted_swaps", "<u8"), ("rejected_swaps", "<u8"), ("geom", "<f8"), ("status", "<i4"), ("reserved0", "<i4"),
+])
+
 class WalkerStateC(C.Structure):
     _fields_ = [(name, C.c_uint8 * WALKER_STATE_DTYPE.itemsize) for name in ("raw",)]
 
@@ -82,6 +89,7 @@ EXPORTS = [
     "sadmc_plan_learned", "sadmc_prop_init", "sadmc_prop_set_weights", "sadmc_prop_get_weights",
     "sadmc_prop_collect", "sadmc_prop_records", "sadmc_prop_load_records", "sadmc_prop_gradient",
     "sadmc_prop_train", "sadmc_mean_entropy",
+    "sadmc_pt_add", "sadmc_pt_run", "sadmc_pt_replicas", "sadmc_pt_coords", "sadmc_pt_spins", "sadmc_pt_ladder_rng",
 ]
 
 PROP_NP, PROP_REC, PROP_NS = 1377, 24, 16
@@ -132,6 +140,12 @@ def lib():
     L.sadmc_prop_gradient.argtypes = [vp, i64, i32, vp, C.POINTER(C.c_float)]
     L.sadmc_prop_train.argtypes = [vp, i32, i32, C.c_float, vp]
     L.sadmc_mean_entropy.argtypes = [vp, i32, d, d, i64, C.POINTER(d), C.POINTER(i64), vp, vp]
+    L.sadmc_pt_add.argtypes = [vp, C.POINTER(SetParams), i32, i32, vp, vp, vp, vp, vp, i32, C.POINTER(i32)]
+    L.sadmc_pt_run.argtypes = [vp, i32, u64]
+    L.sadmc_pt_replicas.argtypes = [vp, i32, vp]
+    L.sadmc_pt_coords.argtypes = [vp, i32, i32, i32, vp]
+    L.sadmc_pt_spins.argtypes = [vp, i32, i32, i32, vp]
+    L.sadmc_pt_ladder_rng.argtypes = [vp, i32, vp]
     for name in EXPORTS:
         if name not in ("sadmc_last_error", "sadmc_destroy", "sadmc_abi_version"):
             getattr(L, name).restype = C.c_int
@@ -252,6 +266,50 @@ class Context:
         E = e0.value + np.arange(nb.value) * w
         keep = cnt > 0
         return E[keep], S[keep], cnt[keep]
+
+    # ---- parallel tempering
+    def pt_add(self, params, temps, seeds, geom=None, coords=None, spins=None, canonical_steps=1):
+        """Ladders (one per seed) of len(temps) replicas; returns the ladder-set id."""
+        seeds = np.ascontiguousarray(seeds, dtype=np.uint64)
+        T = np.ascontiguousarray(temps, dtype=np.float64)
+        g = None if geom is None else np.ascontiguousarray(np.broadcast_to(geom, seeds.shape), dtype=np.float64)
+        c = None if coords is None else np.ascontiguousarray(coords, dtype=np.float64)
+        sp = None if spins is None else np.ascontiguousarray(spins, dtype=np.int8)
+        pid = C.c_int32()
+        self._check(lib().sadmc_pt_add(self.h, C.byref(params), len(seeds), len(T), _p(T), _p(seeds), _p(g), _p(c),
+                                       _p(sp), canonical_steps, C.byref(pid)), "pt_add")
+        if not hasattr(self, "pts"):
+            self.pts = []
+        self.pts.append((params, len(seeds), len(T)))
+        return pid.value
+
+    def pt_run(self, pt, rounds, sync=True):
+        self._check(lib().sadmc_pt_run(self.h, pt, rounds), "pt_run")
+        if sync:
+            self.sync()
+
+    def pt_replicas(self, pt):
+        """Replica records, shape (ladders, replicas)."""
+        _, L, R = self.pts[pt]
+        out = np.zeros(L * R, REPLICA_DTYPE)
+        self._check(lib().sadmc_pt_replicas(self.h, pt, _p(out)), "pt_replicas")
+        return out.reshape(L, R)
+
+    def pt_coords(self, pt, ladder, r):
+        c = np.zeros(3 * self.pts[pt][0].n)
+        self._check(lib().sadmc_pt_coords(self.h, pt, ladder, r, _p(c)), "pt_coords")
+        return c
+
+    def pt_spins(self, pt, ladder, r):
+        n = self.pts[pt][0].n
+        a = np.zeros(n * n, np.int8)
+        self._check(lib().sadmc_pt_spins(self.h, pt, ladder, r, _p(a)), "pt_spins")
+        return a
+
+    def pt_ladder_rng(self, pt):
+        out = np.zeros(2 * self.pts[pt][1], np.uint64)
+        self._check(lib().sadmc_pt_ladder_rng(self.h, pt, _p(out)), "pt_ladder_rng")
+        return out.reshape(-1, 2)
 
     # ---- learned proposal
     def plan_learned(self, s, w):
