@@ -234,8 +234,11 @@ int sadmc_mean_entropy(sadmc_ctx *ctx, int32_t set, double e_a, double e_b, int6
  * own generator is that one jumped by 2^64 (tempering.rs:147-158).  A round
  * (MC::run_once, tempering.rs:272-341) makes canonical_steps *
  * min_moves_to_randomize moves per replica (Replica::run_once, :96-112),
- * cube moves of half-width p->move_param, then tries the odd or even
- * neighbour swaps.  One workgroup per ladder: replicas * lanes <= 1024. */
+ * cube moves of half-width p->move_param (the reference's replicas keep a
+ * translation scale of 1.0, tempering.rs:90; pass 1.0 for that), then tries
+ * the odd or even neighbour swaps.  One workgroup per ladder: replicas * lanes <= 512;
+ * the pair terms of the moves and initial energies are summed in lane-group
+ * order.  Rejected: FP32 pair terms, refresh > 0, the learned move. */
 int sadmc_pt_add(sadmc_ctx *ctx, const sadmc_set_params *p, int32_t ladders, int32_t replicas, const double *temps,
                  const uint64_t *seeds, const double *geom, const double *coords, const int8_t *spins,
                  int32_t canonical_steps, int32_t *pt_id);
