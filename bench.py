@@ -293,6 +293,60 @@ def batched(moves, per_size, launches):
     return out
 
 
+def tempering(rounds, ladders, replicas, lanes, cpu_seconds):
+    """Parallel tempering of LJ31 clusters (sphere R = 3.0): `ladders`
+    independent ladders of `replicas` canonical replicas at temperatures
+    0.05 * 1.15^r, cube moves of half-width 0.1 sigma, one round = 31 moves
+    per replica and one sweep of neighbour swaps.  Value: replica moves/s over
+    one launch of `rounds` rounds (inputs resident on the device).  The CPU
+    baseline runs the oracle's ladder, one ladder per host thread, for about
+    `cpu_seconds`."""
+    temps = 0.05 * 1.15 ** np.arange(replicas)
+    sp = S.make_params(system=S.LJ, n=N_ATOMS, lanes=lanes, move_param=0.1)
+    seeds = np.arange(ladders, dtype=np.uint64) + 1000
+    ctx = S.Context(0)
+    pt = ctx.pt_add(sp, temps, seeds, geom=RADIUS)
+    ctx.pt_run(pt, 20)
+    t0 = time.perf_counter()
+    ctx.pt_run(pt, rounds)
+    wall = time.perf_counter() - t0
+    reps = ctx.pt_replicas(pt)
+    ctx.close()
+    moves = ladders * replicas * N_ATOMS * rounds
+    swaps = reps["accepted_swaps"].sum() / max(1, (reps["accepted_swaps"] + reps["rejected_swaps"]).sum())
+    out = {"moves_per_s": moves / wall, "wall_s": wall, "ladders": ladders, "replicas": replicas, "lanes": lanes,
+           "rounds": rounds, "swap_acceptance": float(swaps),
+           # FP64 pair terms, no bin traffic: the VALU roofline bounds it
+           "roofline": {"bound": "valu-fp64", "achieved": FLOP_PER_MOVE * moves / wall / 1e12,
+                        "peak": FP64_VECTOR_PEAK, "unit": "TFLOP/s",
+                        "frac": FLOP_PER_MOVE * moves / wall / 1e12 / FP64_VECTOR_PEAK}}
+    log(f"[tempering] gpu {out['moves_per_s'] / 1e9:.3f} G replica moves/s, swap acceptance {swaps:.3f}")
+    if cpu_seconds > 0:
+        import threading
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        threads = min(16, os.cpu_count() or 1)
+        op = O.make_params(system=O.LJ, n=N_ATOMS, move_param=0.1, radius=RADIUS, order_g=lanes)
+        pts = [O.Tempering(op, temps, seed=int(s), coords=S.init_coords(S.LJ, N_ATOMS, RADIUS, int(s)))
+               for s in seeds[:threads]]
+        t0 = time.perf_counter()
+        pts[0].run(10)
+        per = max(1, int(10 * cpu_seconds / max(time.perf_counter() - t0, 1e-6) / 2))
+        t0 = time.perf_counter()
+        ths = [threading.Thread(target=p.run, args=(per,)) for p in pts]
+        for t in ths:
+            t.start()
+        for t in ths:
+            t.join()
+        cw = time.perf_counter() - t0
+        cmoves = threads * replicas * N_ATOMS * per
+        out["cpu_baseline"] = {"value": cmoves / cw, "unit": "replica moves/s", "cores": threads, "kind": "port",
+                               "sample": f"{threads} ladders x {per} rounds on {threads} threads"}
+        out["gpu_over_cpu"] = out["moves_per_s"] / out["cpu_baseline"]["value"]
+        log(f"[tempering] cpu {cmoves / cw / 1e6:.2f} M replica moves/s on {threads} threads")
+    return out
+
+
 def copy_probe(nbytes):
     """A plain device-to-device copy of known size, for scaling the HBM byte
     counters (FETCH_SIZE, WRITE_SIZE) of a profiler run."""
@@ -323,6 +377,8 @@ def main():
     ap.add_argument("--learned-cv-system", default="lj38", choices=["lj38", "lj13"])
     ap.add_argument("--batched", type=int, default=0,
                     help="only run the mixed LJ/WCA batch (this many moves per launch)")
+    ap.add_argument("--tempering", type=int, default=0,
+                    help="only run LJ31 parallel tempering ladders for this many rounds")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (walkers are independent chains and shard by walker); "
@@ -336,6 +392,11 @@ def main():
         print(json.dumps({"metric": "mixed LJ/WCA SAD batch moves/s on 1 GPU", "unit": "moves/s",
                           "higher_is_better": True, "walkers_per_size": 4096, "moves_per_launch": a.batched,
                           "launches": a.steps, **res}))
+        return
+    if a.tempering > 0:
+        res = tempering(a.tempering, 512, 16, a.lanes, 0.0 if a.no_cpu else 15.0)
+        print(json.dumps({"metric": "LJ31 parallel-tempering replica moves/s on 1 GPU", "unit": "moves/s",
+                          "higher_is_better": True, "value": res["moves_per_s"], **res}))
         return
     if a.learned_cv > 0:
         lanes = a.lanes if a.learned_cv_system == "lj38" else 4
