@@ -123,6 +123,14 @@ def lib():
     L.or_pt_coords.argtypes = [vp, C.c_int32, vp]
     L.or_pt_rng.argtypes = [vp, u64p]
     L.or_pt_spins.argtypes = [vp, C.c_int32, vp]
+    L.or_bmc_new.argtypes = [vp, vp, vp]; L.or_bmc_new.restype = vp
+    L.or_bmc_new_scripted.argtypes = [vp, C.c_double, vp, C.c_int64]; L.or_bmc_new_scripted.restype = vp
+    L.or_bmc_run.argtypes = [vp, C.c_uint64]; L.or_bmc_run.restype = C.c_int
+    L.or_bmc_state.argtypes = [vp, vp]
+    L.or_bmc_bins.argtypes = [vp] + [vp] * 9
+    L.or_bmc_coords.argtypes = [vp, vp]
+    L.or_bmc_spins.argtypes = [vp, vp]
+    L.or_bmc_free.argtypes = [vp]
     L.or_pt_free.argtypes = [vp]
     L.or_prop_features.argtypes = [C.c_int32, vp, C.c_int32, C.c_double, C.c_double, C.c_double, vp]
     L.or_prop_mlp.argtypes = [vp, vp]; L.or_prop_mlp.restype = C.c_float
@@ -232,6 +240,65 @@ class RNG:
     @property
     def state(self):
         return (self.s[0], self.s[1])
+
+
+BCOUNTS_FIELDS = [("min_total", "<f8"), ("max_total", "<f8"), ("e_max_total", "<f8"), ("e_max_count", "<f8"),
+                  ("min_count", "<u8"), ("max_count", "<u8"), ("total_count", "<u8"), ("exists", "<i4"),
+                  ("reserved", "<i4")]
+BSTATE_DTYPE = np.dtype([
+    ("rng", "<u8", 2), ("E", "<f8"), ("error", "<f8"), ("moves", "<u8"), ("accepted_moves", "<u8"),
+    ("acceptance_rate", "<f8"), ("translation_scale", "<f8"), ("min", "<f8"), ("width", "<f8"), ("min_e", "<f8"),
+    ("max_e", "<f8"), ("len", "<i8"), ("method", "<i4"), ("status", "<i4"), ("num_states", "<u8"), ("tL", "<u8"),
+    ("min_T", "<f8"), ("too_lo", "<f8"), ("too_hi", "<f8"), ("tF", "<f8"), ("latest_parameter", "<f8"),
+    ("t0", "<f8"), ("gamma", "<f8"),
+] + [(name, np.dtype(BCOUNTS_FIELDS)) for name in ("lnw", "energy", "t_found", "hist", "pressure")])
+
+
+class BWalker:
+    """One walker of the binning path (energy_binning.rs over binning/histogram.rs)."""
+
+    def __init__(self, params, coords=None, spins=None, script=None, e0=0.0):
+        self.p = params
+        if script is not None:
+            sc = np.ascontiguousarray(script, dtype=np.float64)
+            self.h = lib().or_bmc_new_scripted(C.byref(params), e0, _ptr(sc), len(sc))
+            return
+        c = None if coords is None else np.ascontiguousarray(coords, dtype=np.float64)
+        sp = None if spins is None else np.ascontiguousarray(spins, dtype=np.int8)
+        self.h = lib().or_bmc_new(C.byref(params), _ptr(c), _ptr(sp))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_bmc_free(self.h)
+            self.h = None
+
+    def run(self, moves):
+        return lib().or_bmc_run(self.h, moves)
+
+    def state(self):
+        st = np.zeros((), BSTATE_DTYPE)
+        lib().or_bmc_state(self.h, _ptr(st))
+        return st
+
+    def bins(self):
+        n = int(self.state()["len"])
+        out = {k: np.zeros(n) for k in ("lnw_total", "energy_total", "t_found_total", "pressure_total")}
+        out.update({k: np.zeros(n, np.uint64) for k in ("lnw_count", "energy_count", "t_found_count", "hist_count",
+                                                         "pressure_count")})
+        lib().or_bmc_bins(self.h, *[_ptr(out[k]) for k in (
+            "lnw_total", "lnw_count", "energy_total", "energy_count", "t_found_total", "t_found_count", "hist_count",
+            "pressure_total", "pressure_count")])
+        return out
+
+    def coords(self):
+        c = np.zeros(3 * self.p.n)
+        lib().or_bmc_coords(self.h, _ptr(c))
+        return c
+
+    def spins(self):
+        s = np.zeros(self.p.n * self.p.n, np.int8)
+        lib().or_bmc_spins(self.h, _ptr(s))
+        return s
 
 
 class Walker:

@@ -1688,3 +1688,492 @@ double or_bench(const or_params *p, int32_t threads, uint64_t moves, uint64_t se
   free(t); free(a);
   return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
 }
+
+/* ===================================================================== */
+/* EnergyMC over a histogram Binning (energy_binning.rs, binning/histogram.rs) */
+/* ===================================================================== */
+
+/* BinCounts (binning/histogram.rs:11-33): one array of totals and counts with
+ * the running statistics the reference keeps beside them. */
+typedef struct {
+  double *total;
+  uint64_t *count;
+  double min_total, max_total, e_max_total, e_max_count;
+  uint64_t min_count, max_count, total_count;
+  int exists;
+} bcounts;
+
+struct or_bmc {
+  or_mc *m;  /* the system, its generator and energy */
+  or_params bp;
+  /* histogram::Bins (binning/histogram.rs:98-111) */
+  double min, width, min_e, max_e;
+  int64_t len, cap;
+  bcounts lnw, en, tf, hist, pr;
+  /* EnergyMC (energy_binning.rs:77-100) */
+  uint64_t moves, accepted;
+  double acc_rate, tscale;
+  int method;
+  uint64_t num_states, tL;
+  double min_T, too_lo, too_hi, tF, latest;
+  double t0;
+  double gamma;
+  int inv_t, has_min_gamma;
+  double min_gamma;
+  int status;
+};
+
+/* f64::max / f64::min as folded from NaN (binning/histogram.rs:366-372):
+ * a NaN accumulator takes the other value. */
+static inline double fold_max(double r, double x) { return (isnan(r) || x > r) ? x : r; }
+static inline double fold_min(double r, double x) { return (isnan(r) || x < r) ? x : r; }
+
+static void bc_new(bcounts *b, int64_t sz, int64_t cap) {
+  b->total = (double *)calloc((size_t)(cap > 0 ? cap : 1), sizeof(double));
+  b->count = (uint64_t *)calloc((size_t)(cap > 0 ? cap : 1), sizeof(uint64_t));
+  (void)sz;
+  b->min_total = 0.0;
+  b->max_total = 0.0;
+  b->e_max_total = -INFINITY;
+  b->e_max_count = -INFINITY;
+  b->min_count = 0;
+  b->max_count = 0;
+  b->total_count = 0;
+  b->exists = 1;
+}
+
+static void bc_free(bcounts *b) {
+  free(b->total);
+  free(b->count);
+}
+
+static double bc_max_of(const bcounts *b, int64_t len) {
+  double r = NAN;
+  for (int64_t i = 0; i < len; i++) r = fold_max(r, b->total[i]);
+  return r;
+}
+static double bc_min_of(const bcounts *b, int64_t len) {
+  double r = NAN;
+  for (int64_t i = 0; i < len; i++) r = fold_min(r, b->total[i]);
+  return r;
+}
+static uint64_t bc_min_count(const bcounts *b, int64_t len) {
+  uint64_t r = UINT64_MAX;
+  for (int64_t i = 0; i < len; i++) r = b->count[i] < r ? b->count[i] : r;
+  return r;
+}
+
+static void bmc_grow(or_bmc *b, int64_t need) {
+  if (need <= b->cap) return;
+  int64_t nc = b->cap ? b->cap : 16;
+  while (nc < need) nc *= 2;
+  bcounts *all[5] = {&b->lnw, &b->en, &b->tf, &b->hist, &b->pr};
+  for (int k = 0; k < 5; k++) {
+    if (!all[k]->exists) continue;
+    all[k]->total = (double *)realloc(all[k]->total, sizeof(double) * nc);
+    all[k]->count = (uint64_t *)realloc(all[k]->count, sizeof(uint64_t) * nc);
+  }
+  b->cap = nc;
+}
+
+/* histogram.rs:47-58: insert_zero / push_zero on lnw and every extra. */
+static void bmc_insert_zero(or_bmc *b) {
+  bmc_grow(b, b->len + 1);
+  bcounts *all[5] = {&b->lnw, &b->en, &b->tf, &b->hist, &b->pr};
+  for (int k = 0; k < 5; k++) {
+    bcounts *c = all[k];
+    if (!c->exists) continue;
+    memmove(c->total + 1, c->total, sizeof(double) * (size_t)b->len);
+    memmove(c->count + 1, c->count, sizeof(uint64_t) * (size_t)b->len);
+    c->total[0] = 0.0;
+    c->count[0] = 0;
+    c->min_total = 0.0;
+    c->min_count = 0;
+  }
+  b->len += 1;
+}
+static void bmc_push_zero(or_bmc *b) {
+  bmc_grow(b, b->len + 1);
+  bcounts *all[5] = {&b->lnw, &b->en, &b->tf, &b->hist, &b->pr};
+  for (int k = 0; k < 5; k++) {
+    bcounts *c = all[k];
+    if (!c->exists) continue;
+    c->total[b->len] = 0.0;
+    c->count[b->len] = 0;
+    c->min_total = 0.0;
+    c->min_count = 0;
+  }
+  b->len += 1;
+}
+
+/* histogram.rs:129-140; -1 stands for the usize::MAX of the out-of-range
+ * cases (e below min, or the top edge of an empty histogram). */
+static int64_t bmc_index(const or_bmc *b, double e) {
+  if (e < b->min) return -1;
+  double i = (e - b->min) / b->width;
+  if (i == (double)b->len) return b->len == 0 ? -1 : (int64_t)i - 1;
+  if (!(i < 9.2e18)) return INT64_MAX;
+  return (int64_t)i;
+}
+static inline double bmc_energy(const or_bmc *b, int64_t i) { return b->min + ((double)i + 0.5) * b->width; }
+static inline double bc_total_at(const bcounts *c, int64_t len, int64_t idx) {
+  return (idx >= 0 && idx < len) ? c->total[idx] : 0.0;
+}
+static inline uint64_t bc_count_at(const bcounts *c, int64_t len, int64_t idx) {
+  return (idx >= 0 && idx < len) ? c->count[idx] : 0;
+}
+static inline double bmc_get_lnw(const or_bmc *b, double e) { return bc_total_at(&b->lnw, b->len, bmc_index(b, e)); }
+/* get_count as an integer count: count/width compares with 0 and with
+ * other counts/width exactly as the counts do. */
+static inline uint64_t bmc_get_count(const or_bmc *b, double e) {
+  return bc_count_at(&b->lnw, b->len, bmc_index(b, e));
+}
+
+/* histogram.rs:141-158. */
+static void bmc_prep(or_bmc *b, double e) {
+  if (b->len == 0) b->min = floor(e / b->width) * b->width;
+  while (e < b->min) {
+    bmc_insert_zero(b);
+    b->min -= b->width;
+  }
+  while (e >= b->min + b->width * (double)b->len) bmc_push_zero(b);
+}
+
+/* BinCounts::increment_count (histogram.rs:59-81). */
+static void bc_increment(bcounts *c, int64_t len, double e, int64_t idx, double value) {
+  c->total_count += 1;
+  uint64_t old_count = c->count[idx];
+  c->count[idx] += 1;
+  double old_total = c->total[idx];
+  c->total[idx] += value;
+  if (c->total[idx] > c->max_total && bc_max_of(c, len) == c->total[idx]) {
+    c->max_total = c->total[idx];
+    c->e_max_total = e;
+  }
+  if (old_total == c->min_total) c->min_total = bc_min_of(c, len);
+  if (old_count == c->min_count) c->min_count = bc_min_count(c, len);
+  if (c->count[idx] > c->max_count) {
+    c->max_count = c->count[idx];
+    c->e_max_count = e;
+  }
+}
+
+/* Bins::accumulate_extra (histogram.rs:219-245); a missing extra is created
+ * with the current length (BinCounts::new). */
+static void bmc_accumulate(or_bmc *b, bcounts *c, double e, double value) {
+  bmc_prep(b, e);
+  int64_t idx = bmc_index(b, e);
+  if (!c->exists) {
+    bc_new(c, b->len, b->cap);
+    memset(c->total, 0, sizeof(double) * (size_t)b->cap);
+    memset(c->count, 0, sizeof(uint64_t) * (size_t)b->cap);
+  }
+  c->total_count += 1;
+  c->count[idx] += 1;
+  if (c->count[idx] == c->min_count + 1) c->min_count = bc_min_count(c, b->len);
+  if (c->count[idx] > c->max_count) c->max_count = c->count[idx];
+  double old_total = c->total[idx];
+  c->total[idx] = old_total + value;
+  if (c->total[idx] > c->max_total) c->max_total = c->total[idx];
+  if (old_total == c->min_total) c->min_total = bc_min_of(c, b->len);
+}
+
+/* Bins::zero_out_extra (histogram.rs:246-259). */
+static void bc_zero_out(bcounts *c, int64_t len) {
+  if (!c->exists) return;
+  memset(c->count, 0, sizeof(uint64_t) * (size_t)len);
+  for (int64_t i = 0; i < len; i++) c->total[i] = 0.0;
+  c->min_total = 0.0;
+  c->max_total = -INFINITY;
+  c->min_count = 0;
+  c->max_count = 0;
+  c->total_count = 0;
+}
+
+/* Bins::increment_count (histogram.rs:171-181). */
+static void bmc_increment(or_bmc *b, double e, double gamma) {
+  if (e > b->max_e) b->max_e = e;
+  if (e < b->min_e) b->min_e = e;
+  bmc_prep(b, e);
+  int64_t idx = bmc_index(b, e);
+  bc_increment(&b->lnw, b->len, e, idx, gamma);
+}
+
+/* count_states (histogram.rs:191-200) with the SAD predicate
+ * e >= too_lo && e <= too_hi (count >= 0 always holds). */
+static uint64_t bmc_count_states(const or_bmc *b) {
+  uint64_t t = 0;
+  for (int64_t i = 0; i < b->len; i++) {
+    double e = bmc_energy(b, i);
+    if (e >= b->too_lo && e <= b->too_hi) t++;
+  }
+  return t;
+}
+
+/* energy_binning.rs:503-530. */
+static double bmc_gamma(const or_bmc *b) {
+  if (b->method == OR_SAD) {
+    double ns = (double)b->num_states;
+    if (b->latest * b->tF * ns == 0.0) return 0.0;
+    double t = (double)b->moves;
+    return (b->latest + t / b->tF) / (b->latest + t / ns * (t / b->tF));
+  }
+  if (b->method == OR_SAMC) {
+    double t = (double)b->moves;
+    return t > b->t0 ? b->t0 / t : 1.0;
+  }
+  return b->gamma;
+}
+
+/* energy_binning.rs:276-319. */
+static int bmc_reject(or_bmc *b, double e1, double e2) {
+  double lnw1 = bmc_get_lnw(b, e1), lnw2 = bmc_get_lnw(b, e2);
+  if (b->method == OR_SAD) {
+    if (e1 < b->too_lo) lnw1 = bmc_get_lnw(b, b->too_lo) + (e1 - b->too_lo) / b->min_T;
+    else if (e1 > b->too_hi) lnw1 = bmc_get_lnw(b, b->too_hi);
+    if (e2 < b->too_lo) lnw2 = bmc_get_lnw(b, b->too_lo) + (e2 - b->too_lo) / b->min_T;
+    else if (e2 > b->too_hi) lnw2 = bmc_get_lnw(b, b->too_hi);
+    int rejected = lnw2 > lnw1 && or_gen_f64(b->m->rng) > exp(lnw1 - lnw2);
+    if (!rejected && bmc_get_count(b, e2) == 0 && e2 < b->too_hi && e2 > b->too_lo) b->tL = b->moves;
+    return rejected;
+  }
+  return lnw2 > lnw1 && or_gen_f64(b->m->rng) > exp(lnw1 - lnw2);
+}
+
+/* energy_binning.rs:320-500. */
+static void bmc_update(or_bmc *b, double energy) {
+  double gamma = bmc_gamma(b);
+  uint64_t old_highest = b->lnw.max_count;
+  uint64_t old_here = bmc_get_count(b, energy);
+  bmc_increment(b, energy, gamma);
+  if (b->method == OR_SAD) {
+    uint64_t here = bmc_get_count(b, energy);
+    if (old_here == 0) bmc_accumulate(b, &b->tf, energy, (double)b->moves);
+    if (here > old_highest) {
+      if (energy > b->too_hi) {
+        double v = bmc_get_lnw(b, b->too_hi);
+        for (int64_t i = 0; i < b->len; i++) {  /* set_lnw, histogram.rs:182-190 */
+          double e = bmc_energy(b, i);
+          if (e > b->too_hi && bmc_get_count(b, e) > 0) {
+            b->lnw.total[i] = v;
+            b->lnw.count[i] = 0;
+          }
+        }
+        b->latest = (energy - b->too_lo) / b->min_T;
+        b->tL = b->moves;
+        b->too_hi = energy;
+        b->num_states = bmc_count_states(b);
+      } else if (energy < b->too_lo) {
+        double v = bmc_get_lnw(b, b->too_lo);
+        double lo = b->too_lo;
+        for (int64_t i = 0; i < b->len; i++) {
+          double e = bmc_energy(b, i);
+          if (e < lo && bmc_get_count(b, e) > 0) {
+            b->lnw.total[i] = v + (e - lo) / b->min_T;
+            b->lnw.count[i] = 0;
+          }
+        }
+        b->latest = (b->too_hi - energy) / b->min_T;
+        b->tL = b->moves;
+        b->too_lo = energy;
+        b->num_states = bmc_count_states(b);
+      }
+    }
+    if (b->tL == b->moves) {
+      double old_tF = b->tF;
+      b->tF = b->tf.exists ? b->tf.max_total : 0.0;
+      if (old_tF != b->tF && b->bp.move_plan == 1) {
+        double s = b->acc_rate / b->bp.move_param;
+        s = s < 0.8 ? 0.8 : (s > 1.2 ? 1.2 : s);
+        b->tscale *= s;
+      }
+    }
+  } else if (b->method == OR_WL) {
+    double w = b->width;
+    double old_lowest = b->hist.exists ? (double)b->hist.min_count / w : 0.0;
+    bmc_accumulate(b, &b->hist, energy, 0.0);
+    if (b->has_min_gamma && b->gamma < b->min_gamma) return;
+    double lowest = (double)b->hist.min_count / w;
+    int ok_lo = !b->bp.has_min_e || bmc_get_count(b, b->bp.min_e) > 0;
+    int ok_hi = !b->bp.has_max_e || bmc_get_count(b, b->bp.max_e) > 0;
+    if (lowest > old_lowest && ok_lo && ok_hi) {
+      double mean = (double)b->hist.total_count / (w * (double)b->len);
+      if ((b->inv_t && lowest > 0.0) || lowest >= 0.8 * mean) {
+        b->gamma *= 0.5;
+        bc_zero_out(&b->hist, b->len);
+        if (b->has_min_gamma && b->gamma < b->min_gamma) b->gamma = 0.0;
+      }
+      if (b->inv_t && b->gamma < (double)b->len / (double)b->moves) {
+        b->method = OR_SAMC;
+        b->t0 = (double)b->len;
+      }
+    }
+  }
+}
+
+/* energy_binning.rs:566-611. */
+static void bmc_move_once(or_bmc *b) {
+  or_mc *m = b->m;
+  b->moves += 1;
+  m->moves = b->moves;
+  if (b->moves % 100000000ULL == 0) {
+    sys_verify_energy(m);
+    if (m->status) {
+      b->status = m->status;
+      return;
+    }
+  }
+  double e1 = m->E;
+  bmc_accumulate(b, &b->en, e1, e1);
+  double pv;
+  if (sys_pressure(m, &pv)) bmc_accumulate(b, &b->pr, e1, pv);
+  double rs = sqrt(1.0 / (double)b->moves);
+  b->acc_rate *= 1.0 - rs;
+  double e2;
+  if (sys_plan_move(m, b->tscale, &e2)) {
+    int oob = 0;
+    if (b->bp.has_max_e) oob = e2 > b->bp.max_e && e2 > e1;
+    if (b->bp.has_min_e) oob = oob || (e2 < b->bp.min_e && e2 < e1);
+    if (!oob && !bmc_reject(b, e1, e2)) {
+      b->accepted += 1;
+      b->acc_rate += rs;
+      sys_confirm(m);
+    }
+  }
+  bmc_update(b, m->E);
+}
+
+/* energy_binning.rs:533-564 (from_params) on a built system. */
+static or_bmc *bmc_from_system(const or_params *bp, or_mc *m) {
+  or_bmc *b = (or_bmc *)calloc(1, sizeof(or_bmc));
+  b->bp = *bp;
+  b->m = m;
+  if (bp->has_max_e) {
+    for (uint64_t k = 0; k < 100000000ULL; k++) {
+      double newe;
+      if (sys_plan_move(m, 0.05, &newe)) {
+        if (newe < m->E) sys_confirm(m);
+        if (m->E < bp->max_e) break;
+      }
+    }
+  }
+  double E = m->E;
+  b->method = bp->method == OR_INV_T_WL ? OR_WL : bp->method;
+  b->min_T = bp->method_param;
+  b->too_lo = E;
+  b->too_hi = E;
+  b->t0 = bp->method_param;
+  b->gamma = 1.0;
+  b->inv_t = bp->method == OR_INV_T_WL;
+  b->has_min_gamma = bp->method == OR_WL && bp->has_min_gamma;
+  b->min_gamma = bp->method_param;
+  b->width = bp->energy_bin > 0 ? bp->energy_bin : 1.0;
+  b->min = (round(E / b->width) - 0.5) * b->width;  /* Bins::new, histogram.rs:162-170 */
+  b->min_e = E;
+  b->max_e = E;
+  b->len = 0;
+  bmc_grow(b, 16);
+  bc_new(&b->lnw, 0, b->cap);
+  b->acc_rate = 0.5;
+  b->tscale = bp->move_plan == 0 ? bp->move_param : 0.05;
+  return b;
+}
+
+static or_params bmc_system_params(const or_params *p) {
+  or_params q = *p;
+  q.has_max_e = 0;
+  q.has_min_e = 0;
+  q.method = OR_CANONICAL;
+  q.method_param = 1.0;
+  return q;
+}
+
+or_bmc *or_bmc_new(const or_params *p, const double *coords, const int8_t *spins) {
+  or_params q = bmc_system_params(p);
+  return bmc_from_system(p, or_mc_new(&q, coords, spins));  /* generator: seed_from_u64(seed) */
+}
+
+/* The same walker over the scripted system (plan_move returns the script's
+ * energies in turn, NaN meaning no move). */
+or_bmc *or_bmc_new_scripted(const or_params *p, double e0, const double *script, int64_t len) {
+  or_params q = bmc_system_params(p);
+  return bmc_from_system(p, or_mc_new_scripted(&q, e0, script, len));
+}
+
+int or_bmc_run(or_bmc *b, uint64_t moves) {
+  for (uint64_t k = 0; k < moves && !b->status; k++) bmc_move_once(b);
+  return b->status;
+}
+
+static void bc_export(const bcounts *c, or_bcounts *o) {
+  o->min_total = c->min_total;
+  o->max_total = c->max_total;
+  o->e_max_total = c->e_max_total;
+  o->e_max_count = c->e_max_count;
+  o->min_count = c->min_count;
+  o->max_count = c->max_count;
+  o->total_count = c->total_count;
+  o->exists = c->exists;
+}
+
+void or_bmc_state(const or_bmc *b, or_bstate *o) {
+  memset(o, 0, sizeof *o);
+  o->rng[0] = b->m->rng[0];
+  o->rng[1] = b->m->rng[1];
+  o->E = b->m->E;
+  o->error = b->m->error;
+  o->moves = b->moves;
+  o->accepted = b->accepted;
+  o->acceptance_rate = b->acc_rate;
+  o->translation_scale = b->tscale;
+  o->min = b->min;
+  o->width = b->width;
+  o->min_e = b->min_e;
+  o->max_e = b->max_e;
+  o->len = b->len;
+  o->method = b->method;
+  o->status = b->status;
+  o->num_states = b->num_states;
+  o->tL = b->tL;
+  o->min_T = b->min_T;
+  o->too_lo = b->too_lo;
+  o->too_hi = b->too_hi;
+  o->tF = b->tF;
+  o->latest_parameter = b->latest;
+  o->t0 = b->t0;
+  o->gamma = b->gamma;
+  bc_export(&b->lnw, &o->lnw);
+  bc_export(&b->en, &o->energy);
+  bc_export(&b->tf, &o->t_found);
+  bc_export(&b->hist, &o->hist);
+  bc_export(&b->pr, &o->pressure);
+}
+
+/* Logical bins [0, len): totals and counts of lnw and the extras (zeros for
+ * an extra that does not exist). */
+void or_bmc_bins(const or_bmc *b, double *lnw_total, uint64_t *lnw_count, double *energy_total,
+                 uint64_t *energy_count, double *tf_total, uint64_t *tf_count, uint64_t *hist_count,
+                 double *p_total, uint64_t *p_count) {
+  int64_t n = b->len;
+  const bcounts *cs[5] = {&b->lnw, &b->en, &b->tf, &b->hist, &b->pr};
+  double *tt[5] = {lnw_total, energy_total, tf_total, NULL, p_total};
+  uint64_t *cc[5] = {lnw_count, energy_count, tf_count, hist_count, p_count};
+  for (int k = 0; k < 5; k++) {
+    for (int64_t i = 0; i < n; i++) {
+      if (tt[k]) tt[k][i] = cs[k]->exists ? cs[k]->total[i] : 0.0;
+      if (cc[k]) cc[k][i] = cs[k]->exists ? cs[k]->count[i] : 0;
+    }
+  }
+}
+
+void or_bmc_coords(const or_bmc *b, double *c) { or_mc_coords(b->m, c); }
+void or_bmc_spins(const or_bmc *b, int8_t *s) { or_mc_spins(b->m, s); }
+
+void or_bmc_free(or_bmc *b) {
+  if (!b) return;
+  bcounts *all[5] = {&b->lnw, &b->en, &b->tf, &b->hist, &b->pr};
+  for (int k = 0; k < 5; k++)
+    if (all[k]->exists) bc_free(all[k]);
+  or_mc_free(b->m);
+  free(b);
+}

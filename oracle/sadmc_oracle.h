@@ -211,6 +211,45 @@ void or_pt_spins(const or_pt *pt, int32_t r, int8_t *spins);
 void or_pt_rng(const or_pt *pt, uint64_t out[2]);
 void or_pt_free(or_pt *pt);
 
+/* ---- EnergyMC over a histogram Binning (src/mc/energy_binning.rs with
+ * src/mc/binning/histogram.rs) --------------------------------------------
+ * The `binning` binary's walker: SAD (un-rounded too_lo/too_hi, num_states
+ * from count_states, set_lnw zeroing counts), SAMC, WL and 1/t-WL over a
+ * histogram of width energy_bin (<= 0: 1.0) that grows at both ends, with
+ * the "energy" extra every move, the "t_found" extra (SAD), the "hist" extra
+ * (WL) and the WCA "pressure" extra every N^2 moves.  The method comes from
+ * p->method (OR_CANONICAL is not a method of this walker); p->max_e /
+ * p->min_e are max/min_allowed_energy; the generator is seed_from_u64(seed)
+ * and a max_allowed_energy triggers the greedy descent of from_params. */
+typedef struct {
+  double min_total, max_total, e_max_total, e_max_count;
+  uint64_t min_count, max_count, total_count;
+  int32_t exists, reserved;
+} or_bcounts;
+typedef struct {
+  uint64_t rng[2];
+  double E, error;
+  uint64_t moves, accepted;
+  double acceptance_rate, translation_scale;
+  double min, width, min_e, max_e;
+  int64_t len;
+  int32_t method, status;
+  uint64_t num_states, tL;
+  double min_T, too_lo, too_hi, tF, latest_parameter, t0, gamma;
+  or_bcounts lnw, energy, t_found, hist, pressure;
+} or_bstate;
+typedef struct or_bmc or_bmc;
+or_bmc *or_bmc_new(const or_params *p, const double *coords, const int8_t *spins);
+or_bmc *or_bmc_new_scripted(const or_params *p, double e0, const double *script, int64_t len);
+int or_bmc_run(or_bmc *b, uint64_t moves);
+void or_bmc_state(const or_bmc *b, or_bstate *out);
+void or_bmc_bins(const or_bmc *b, double *lnw_total, uint64_t *lnw_count, double *energy_total,
+                 uint64_t *energy_count, double *tf_total, uint64_t *tf_count, uint64_t *hist_count,
+                 double *p_total, uint64_t *p_count);
+void or_bmc_coords(const or_bmc *b, double *coords);
+void or_bmc_spins(const or_bmc *b, int8_t *spins);
+void or_bmc_free(or_bmc *b);
+
 /* CPU baseline: run `threads` independent walkers (one per thread) for
  * `moves` moves each; returns wall seconds (init excluded). */
 double or_bench(const or_params *p, int32_t threads, uint64_t moves, uint64_t seed0);
