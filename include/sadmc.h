@@ -226,6 +226,69 @@ int sadmc_prop_train(sadmc_ctx *ctx, int32_t steps, int32_t batch, float lr, flo
 int sadmc_mean_entropy(sadmc_ctx *ctx, int32_t set, double e_a, double e_b, int64_t max_bins, double *e_first,
                        int64_t *nbins, double *S, int64_t *count);
 
+/* ---- EnergyMC over a histogram Binning (src/mc/energy_binning.rs with
+ * src/mc/binning/histogram.rs: the `binning` binary) -----------------------
+ * A separate kind of walker set: SAD (un-rounded too_lo/too_hi, num_states
+ * by count_states, set_lnw zeroing counts), SAMC, WL and 1/t-WL over a
+ * histogram of width p->energy_bin (<= 0: 1.0, BinningParams' default)
+ * that grows at both ends inside a fixed window of bins per walker.  Every
+ * move accumulates the "energy" extra, SAD the "t_found" extra, WL the
+ * "hist" extra, WCA with collect_pressure the "pressure" extra every N^2
+ * moves, each with BinCounts' running statistics (histogram.rs:11-33).
+ * Method from p->method (SADMC_CANONICAL rejected), FP64, refresh 0, cube
+ * moves; p->min_e/max_e are min/max_allowed_energy (a max triggers the
+ * greedy descent of from_params, energy_binning.rs:537-550). */
+typedef struct {
+  double min_total, max_total, e_max_total, e_max_count;
+  uint64_t min_count, max_count, total_count;
+  int32_t exists, reserved;
+} sadmc_bcounts;
+
+typedef struct {
+  uint64_t rng[2];
+  double E, error;
+  uint64_t moves, accepted_moves;
+  double acceptance_rate, translation_scale;
+  double min, width, min_e, max_e; /* histogram::Bins (histogram.rs:98-111) */
+  int64_t len;
+  int32_t method, status;          /* SADMC_SAD/SAMC/WL (1/t-WL may switch to SAMC); SADMC_WS_* */
+  uint64_t num_states, tL;
+  double min_T, too_lo, too_hi, tF, latest_parameter, t0, gamma;
+  sadmc_bcounts lnw, energy, t_found, hist, pressure;
+  /* placement of the logical bins in the walker's window */
+  int64_t base, cap;
+  double geom;
+  int32_t inv_t, has_min_gamma;
+  double min_gamma;
+  uint64_t reserved[3];
+} sadmc_bwalker_state;
+
+/* One logical bin: lnw and the extras' totals and counts. */
+typedef struct {
+  double lnw_total;
+  uint64_t lnw_count;
+  double energy_total;
+  uint64_t energy_count;
+  double t_found_total;
+  uint64_t t_found_count;
+  uint64_t hist_count;
+  double pressure_total;
+  uint64_t pressure_count;
+  uint64_t reserved;
+} sadmc_bbin;
+
+/* EnergyMC::from_params for W walkers (runs on the device at once). */
+int sadmc_bin_add_set(sadmc_ctx *ctx, const sadmc_set_params *p, int32_t W, const uint64_t *seeds,
+                      const double *geom, const double *coords, const int8_t *spins, int32_t *set_id);
+/* move_once x moves for every walker of the set (synchronous; fails with
+ * SADMC_E_CAPACITY / SADMC_E_VERIFY when a walker stopped). */
+int sadmc_bin_run(sadmc_ctx *ctx, int32_t set, uint64_t moves);
+int sadmc_bin_state(sadmc_ctx *ctx, int32_t set, int32_t w, sadmc_bwalker_state *out);
+/* The logical bins [0, len) of walker w; SADMC_E_CAPACITY if max_bins < len. */
+int sadmc_bin_bins(sadmc_ctx *ctx, int32_t set, int32_t w, int64_t max_bins, sadmc_bbin *out);
+int sadmc_bin_coords(sadmc_ctx *ctx, int32_t set, int32_t w, double *coords);
+int sadmc_bin_spins(sadmc_ctx *ctx, int32_t set, int32_t w, int8_t *spins);
+
 /* ---- parallel tempering (src/mc/tempering.rs) ---------------------------
  * `ladders` independent ladders of `replicas` canonical replicas each, at
  * temperatures temps[0..replicas).  Ladder l starts every replica from one

@@ -1860,9 +1860,13 @@ static void bc_increment(bcounts *c, int64_t len, double e, int64_t idx, double 
 
 /* Bins::accumulate_extra (histogram.rs:219-245); a missing extra is created
  * with the current length (BinCounts::new). */
-static void bmc_accumulate(or_bmc *b, bcounts *c, double e, double value) {
+static int bmc_accumulate(or_bmc *b, bcounts *c, double e, double value) {
   bmc_prep(b, e);
   int64_t idx = bmc_index(b, e);
+  if (idx < 0 || idx >= b->len) {  /* the reference's assert (histogram.rs:222) */
+    b->status = ST_INDEX;
+    return 0;
+  }
   if (!c->exists) {
     bc_new(c, b->len, b->cap);
     memset(c->total, 0, sizeof(double) * (size_t)b->cap);
@@ -1876,6 +1880,7 @@ static void bmc_accumulate(or_bmc *b, bcounts *c, double e, double value) {
   c->total[idx] = old_total + value;
   if (c->total[idx] > c->max_total) c->max_total = c->total[idx];
   if (old_total == c->min_total) c->min_total = bc_min_of(c, b->len);
+  return 1;
 }
 
 /* Bins::zero_out_extra (histogram.rs:246-259). */
@@ -1891,12 +1896,17 @@ static void bc_zero_out(bcounts *c, int64_t len) {
 }
 
 /* Bins::increment_count (histogram.rs:171-181). */
-static void bmc_increment(or_bmc *b, double e, double gamma) {
+static int bmc_increment(or_bmc *b, double e, double gamma) {
   if (e > b->max_e) b->max_e = e;
   if (e < b->min_e) b->min_e = e;
   bmc_prep(b, e);
   int64_t idx = bmc_index(b, e);
+  if (idx < 0 || idx >= b->len) {  /* an index panic in the reference */
+    b->status = ST_INDEX;
+    return 0;
+  }
   bc_increment(&b->lnw, b->len, e, idx, gamma);
+  return 1;
 }
 
 /* count_states (histogram.rs:191-200) with the SAD predicate
@@ -1945,10 +1955,10 @@ static void bmc_update(or_bmc *b, double energy) {
   double gamma = bmc_gamma(b);
   uint64_t old_highest = b->lnw.max_count;
   uint64_t old_here = bmc_get_count(b, energy);
-  bmc_increment(b, energy, gamma);
+  if (!bmc_increment(b, energy, gamma)) return;
   if (b->method == OR_SAD) {
     uint64_t here = bmc_get_count(b, energy);
-    if (old_here == 0) bmc_accumulate(b, &b->tf, energy, (double)b->moves);
+    if (old_here == 0 && !bmc_accumulate(b, &b->tf, energy, (double)b->moves)) return;
     if (here > old_highest) {
       if (energy > b->too_hi) {
         double v = bmc_get_lnw(b, b->too_hi);
@@ -1991,7 +2001,7 @@ static void bmc_update(or_bmc *b, double energy) {
   } else if (b->method == OR_WL) {
     double w = b->width;
     double old_lowest = b->hist.exists ? (double)b->hist.min_count / w : 0.0;
-    bmc_accumulate(b, &b->hist, energy, 0.0);
+    if (!bmc_accumulate(b, &b->hist, energy, 0.0)) return;
     if (b->has_min_gamma && b->gamma < b->min_gamma) return;
     double lowest = (double)b->hist.min_count / w;
     int ok_lo = !b->bp.has_min_e || bmc_get_count(b, b->bp.min_e) > 0;
@@ -2024,9 +2034,9 @@ static void bmc_move_once(or_bmc *b) {
     }
   }
   double e1 = m->E;
-  bmc_accumulate(b, &b->en, e1, e1);
+  if (!bmc_accumulate(b, &b->en, e1, e1)) return;
   double pv;
-  if (sys_pressure(m, &pv)) bmc_accumulate(b, &b->pr, e1, pv);
+  if (sys_pressure(m, &pv) && !bmc_accumulate(b, &b->pr, e1, pv)) return;
   double rs = sqrt(1.0 / (double)b->moves);
   b->acc_rate *= 1.0 - rs;
   double e2;

@@ -1,0 +1,645 @@
+// EnergyMC over a histogram Binning (src/mc/energy_binning.rs with
+// src/mc/binning/histogram.rs): the walker of the reference's `binning`
+// binary, one walker per lane group like the main engine.
+//
+// The system part (plan_move, confirm, energies, pressure) is the main
+// engine's; the bookkeeping is this file's.  Every lane of a group holds the
+// walker's scalars (identical values), the bins live in HBM as one 80-byte
+// record per bin, and the O(bins) scans the reference makes (BinCounts'
+// min/max recomputations, set_lnw, count_states) are split over the group's
+// lanes and combined with shuffles so that the result is the sequential one.
+#define SADMC_KERNEL_TU
+#include "sadmc_kernels.h"
+
+namespace sadmc {
+
+namespace {
+
+enum { BC_LNW = 0, BC_EN = 1, BC_TF = 2, BC_HIST = 3, BC_PR = 4 };
+
+// BinCounts' running statistics (histogram.rs:11-33).
+struct BC {
+  double min_total, max_total, e_max_total, e_max_count;
+  uint64_t min_count, max_count, total_count;
+  int32_t exists;
+};
+
+// f64::max / f64::min folded from NaN (histogram.rs:366-372): the first
+// element attaining the extreme.  Partial folds of interleaved lanes are
+// combined by value, then by the first index, which gives the same element.
+struct Ext {
+  double v;
+  int64_t i;
+};
+
+__device__ __forceinline__ Ext ext_pick(Ext a, Ext b, bool want_max) {
+  if (b.i < 0) return a;
+  if (a.i < 0) return b;
+  const bool better = want_max ? b.v > a.v : b.v < a.v;
+  if (better) return b;
+  if (b.v == a.v && b.i < a.i) return b;
+  return a;
+}
+
+__device__ __forceinline__ double shfl_f64(double v, int d) {
+  return __builtin_bit_cast(double, (uint64_t)__shfl_xor((long long)__builtin_bit_cast(uint64_t, v), d, 64));
+}
+
+}  // namespace
+
+template <int SYS, int KMAX>
+struct BinWalker {
+  Engine<SYS, KMAX, 0, M_CAN> e;
+  const BinArgs *Bp;
+  sadmc_bbin *B;  // this walker's window of bins
+  int lane, G;
+  uint64_t moves, accepted;
+  double acc_rate, tscale, usc;
+  double min, width, min_e, max_e;
+  int64_t len, base, cap;
+  int32_t method, status;
+  uint64_t num_states, tL;
+  double min_T, too_lo, too_hi, tF, latest, t0, gamma;
+  int32_t inv_t, has_min_gamma;
+  double min_gamma;
+  BC bc[5];
+
+  SADMC_DF const BinArgs &args() const { return *Bp; }
+
+  // ------------------------------------------------------------ state I/O
+  SADMC_DF static void bc_load(BC &c, const sadmc_bcounts &s) {
+    c.min_total = s.min_total;
+    c.max_total = s.max_total;
+    c.e_max_total = s.e_max_total;
+    c.e_max_count = s.e_max_count;
+    c.min_count = s.min_count;
+    c.max_count = s.max_count;
+    c.total_count = s.total_count;
+    c.exists = s.exists;
+  }
+  SADMC_DF static void bc_store(const BC &c, sadmc_bcounts &s) {
+    s.min_total = c.min_total;
+    s.max_total = c.max_total;
+    s.e_max_total = c.e_max_total;
+    s.e_max_count = c.e_max_count;
+    s.min_count = c.min_count;
+    s.max_count = c.max_count;
+    s.total_count = c.total_count;
+    s.exists = c.exists;
+    s.reserved = 0;
+  }
+
+  SADMC_DF void load(const BinArgs &ba, const SetArgs &sa, int walker, int lane_in_group, double *lds_group) {
+    Bp = &ba;
+    lane = lane_in_group;
+    G = sa.G;
+    e.load_system(sa, walker, lane_in_group, lds_group);
+    const sadmc_bwalker_state &w = ba.ws[walker];
+    e.s0 = w.rng[0];
+    e.s1 = w.rng[1];
+    e.E = w.E;
+    e.err = w.error;
+    if (SYS != SYS_ISING) {
+      if (SYS == SYS_LJ) e.R2 = w.geom * w.geom;
+      e.L = w.geom;
+      e.halfL = 0.5 * w.geom;
+    }
+    e.status = 0;
+    moves = w.moves;
+    accepted = w.accepted_moves;
+    acc_rate = w.acceptance_rate;
+    tscale = w.translation_scale;
+    usc = SYS == SYS_ISING ? 0.0 : uniform_scale(-tscale, tscale);
+    min = w.min;
+    width = w.width;
+    min_e = w.min_e;
+    max_e = w.max_e;
+    len = w.len;
+    base = w.base;
+    cap = w.cap;
+    method = w.method;
+    status = w.status;
+    num_states = w.num_states;
+    tL = w.tL;
+    min_T = w.min_T;
+    too_lo = w.too_lo;
+    too_hi = w.too_hi;
+    tF = w.tF;
+    latest = w.latest_parameter;
+    t0 = w.t0;
+    gamma = w.gamma;
+    inv_t = w.inv_t;
+    has_min_gamma = w.has_min_gamma;
+    min_gamma = w.min_gamma;
+    bc_load(bc[BC_LNW], w.lnw);
+    bc_load(bc[BC_EN], w.energy);
+    bc_load(bc[BC_TF], w.t_found);
+    bc_load(bc[BC_HIST], w.hist);
+    bc_load(bc[BC_PR], w.pressure);
+    B = ba.bins + (int64_t)walker * ba.cap;
+  }
+
+  SADMC_DF void store(int walker) {
+    if (SYS != SYS_ISING) {
+      // the positions of the group's atoms, from the LDS mirror
+      const SetArgs &sa = e.args();
+      double *P = sa.pos + (int64_t)walker * 3 * sa.npad;
+      for (int j = lane; j < sa.npad; j += G) {
+        P[j] = e.lx[j];
+        P[sa.npad + j] = e.ly[j];
+        P[2 * sa.npad + j] = e.lz[j];
+      }
+    }
+    if (lane != 0) return;
+    if (SYS == SYS_ISING) e.args().spins[walker] = e.spins;
+    sadmc_bwalker_state &w = args().ws[walker];
+    w.rng[0] = e.s0;
+    w.rng[1] = e.s1;
+    w.E = e.E;
+    w.error = e.err;
+    w.moves = moves;
+    w.accepted_moves = accepted;
+    w.acceptance_rate = acc_rate;
+    w.translation_scale = tscale;
+    w.min = min;
+    w.width = width;
+    w.min_e = min_e;
+    w.max_e = max_e;
+    w.len = len;
+    w.base = base;
+    w.method = method;
+    w.status = status;
+    w.num_states = num_states;
+    w.tL = tL;
+    w.min_T = min_T;
+    w.too_lo = too_lo;
+    w.too_hi = too_hi;
+    w.tF = tF;
+    w.latest_parameter = latest;
+    w.t0 = t0;
+    w.gamma = gamma;
+    bc_store(bc[BC_LNW], w.lnw);
+    bc_store(bc[BC_EN], w.energy);
+    bc_store(bc[BC_TF], w.t_found);
+    bc_store(bc[BC_HIST], w.hist);
+    bc_store(bc[BC_PR], w.pressure);
+    if (status) *args().flag = 1;
+  }
+
+  // ------------------------------------------------------- bin records
+  SADMC_DF static double &tot(sadmc_bbin &b, int k) {
+    return k == BC_LNW ? b.lnw_total : k == BC_EN ? b.energy_total : k == BC_TF ? b.t_found_total : b.pressure_total;
+  }
+  SADMC_DF static uint64_t &cnt(sadmc_bbin &b, int k) {
+    return k == BC_LNW   ? b.lnw_count
+           : k == BC_EN  ? b.energy_count
+           : k == BC_TF  ? b.t_found_count
+           : k == BC_HIST ? b.hist_count
+                          : b.pressure_count;
+  }
+  // the "hist" extra only ever adds 0.0 to totals that start at 0.0
+  SADMC_DF double total_at(int k, int64_t i) const { return k == BC_HIST ? 0.0 : tot(B[base + i], k); }
+  SADMC_DF uint64_t count_at(int k, int64_t i) const { return cnt(B[base + i], k); }
+
+  // histogram.rs:129-140; -1 for the out-of-range cases
+  SADMC_DF int64_t index(double en) const {
+    if (en < min) return -1;
+    const double i = (en - min) / width;
+    if (i == (double)len) return len == 0 ? -1 : (int64_t)i - 1;
+    if (!(i < 9.2e18)) return INT64_MAX;
+    return (int64_t)i;
+  }
+  SADMC_DF double energy_of(int64_t i) const { return min + ((double)i + 0.5) * width; }
+  SADMC_DF double get_lnw(double en) const {
+    const int64_t i = index(en);
+    return (i >= 0 && i < len) ? B[base + i].lnw_total : 0.0;
+  }
+  SADMC_DF uint64_t get_count(double en) const {
+    const int64_t i = index(en);
+    return (i >= 0 && i < len) ? B[base + i].lnw_count : 0;
+  }
+
+  // ---------------------------------------------------- group reductions
+  SADMC_DF Ext group_ext(Ext x, bool want_max) const {
+    for (int d = 1; d < G; d <<= 1) {
+      Ext o;
+      o.v = shfl_f64(x.v, d);
+      o.i = (int64_t)__shfl_xor((long long)x.i, d, 64);
+      x = ext_pick(x, o, want_max);
+    }
+    return x;
+  }
+  SADMC_DF uint64_t group_min_u64(uint64_t v) const {
+    for (int d = 1; d < G; d <<= 1) {
+      const uint64_t o = (uint64_t)__shfl_xor((long long)v, d, 64);
+      v = o < v ? o : v;
+    }
+    return v;
+  }
+  SADMC_DF uint64_t group_sum_u64(uint64_t v) const {
+    for (int d = 1; d < G; d <<= 1) v += (uint64_t)__shfl_xor((long long)v, d, 64);
+    return v;
+  }
+  // max_of / min_of of array k's totals, with bin `at` holding `val` (the
+  // value this move just wrote there)
+  SADMC_DF double scan_total(int k, bool want_max, int64_t at, double val) const {
+    Ext x{0.0, -1};
+    for (int64_t j = lane; j < len; j += G) {
+      const double v = j == at ? val : total_at(k, j);
+      x = ext_pick(x, Ext{v, j}, want_max);
+    }
+    x = group_ext(x, want_max);
+    return x.i < 0 ? __builtin_nan("") : x.v;
+  }
+  SADMC_DF uint64_t scan_min_count(int k, int64_t at, uint64_t val) const {
+    uint64_t m = ~0ULL;
+    for (int64_t j = lane; j < len; j += G) {
+      const uint64_t v = j == at ? val : count_at(k, j);
+      m = v < m ? v : m;
+    }
+    return group_min_u64(m);
+  }
+
+  // ------------------------------------------------------------ growth
+  SADMC_DF void zero_bin(int64_t p) {
+    {
+      sadmc_bbin z;
+      z.lnw_total = 0.0;
+      z.lnw_count = 0;
+      z.energy_total = 0.0;
+      z.energy_count = 0;
+      z.t_found_total = 0.0;
+      z.t_found_count = 0;
+      z.hist_count = 0;
+      z.pressure_total = 0.0;
+      z.pressure_count = 0;
+      z.reserved = 0;
+      B[p] = z;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  }
+  SADMC_DF void grown() {
+#pragma unroll
+    for (int k = 0; k < 5; k++)
+      if (bc[k].exists) {
+        bc[k].min_total = 0.0;
+        bc[k].min_count = 0;
+      }
+    len += 1;
+  }
+  // histogram.rs:141-158 (insert_zero / push_zero on lnw and every extra)
+  SADMC_DF bool prep(double en) {
+    if (len == 0) {
+      min = floor(en / width) * width;
+      int64_t b0 = (int64_t)floor((min - args().e_lo) / width) + 4;
+      base = b0 < 0 ? 0 : (b0 >= cap ? cap - 1 : b0);
+    }
+    while (en < min) {
+      if (base == 0) {
+        status = SADMC_WS_CAPACITY;
+        return false;
+      }
+      base -= 1;
+      zero_bin(base);
+      grown();
+      min -= width;
+    }
+    while (en >= min + width * (double)len) {
+      if (base + len >= cap) {
+        status = SADMC_WS_CAPACITY;
+        return false;
+      }
+      zero_bin(base + len);
+      grown();
+    }
+    return true;
+  }
+
+  // BinCounts::increment_count (histogram.rs:59-81) on lnw
+  SADMC_DF void lnw_increment(double en, int64_t idx, double value) {
+    BC &c = bc[BC_LNW];
+    sadmc_bbin &r = B[base + idx];
+    c.total_count += 1;
+    const uint64_t old_count = r.lnw_count;
+    const uint64_t new_count = old_count + 1;
+    const double old_total = r.lnw_total;
+    const double new_total = old_total + value;
+    r.lnw_count = new_count;
+    r.lnw_total = new_total;
+    if (new_total > c.max_total && scan_total(BC_LNW, true, idx, new_total) == new_total) {
+      c.max_total = new_total;
+      c.e_max_total = en;
+    }
+    if (old_total == c.min_total) c.min_total = scan_total(BC_LNW, false, idx, new_total);
+    if (old_count == c.min_count) c.min_count = scan_min_count(BC_LNW, idx, new_count);
+    if (new_count > c.max_count) {
+      c.max_count = new_count;
+      c.e_max_count = en;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  }
+
+  // Bins::accumulate_extra (histogram.rs:219-245)
+  SADMC_DF bool accumulate(int k, double en, double value) {
+    if (!prep(en)) return false;
+    const int64_t idx = index(en);
+    if (idx < 0 || idx >= len) {  // the reference's assert (histogram.rs:222)
+      status = SADMC_WS_INDEX;
+      return false;
+    }
+    BC &c = bc[k];
+    if (!c.exists) {  // BinCounts::new over the current bins
+      c.exists = 1;
+      c.min_total = 0.0;
+      c.max_total = 0.0;
+      c.e_max_total = -__builtin_inf();
+      c.e_max_count = -__builtin_inf();
+      c.min_count = 0;
+      c.max_count = 0;
+      c.total_count = 0;
+    }
+    sadmc_bbin &r = B[base + idx];
+    c.total_count += 1;
+    const uint64_t new_count = cnt(r, k) + 1;
+    cnt(r, k) = new_count;
+    if (new_count == c.min_count + 1) c.min_count = scan_min_count(k, idx, new_count);
+    if (new_count > c.max_count) c.max_count = new_count;
+    const double old_total = k == BC_HIST ? 0.0 : tot(r, k);
+    const double new_total = old_total + value;
+    if (k != BC_HIST) tot(r, k) = new_total;
+    if (new_total > c.max_total) c.max_total = new_total;
+    if (old_total == c.min_total) c.min_total = scan_total(k, false, idx, new_total);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    return true;
+  }
+
+  // Bins::zero_out_extra (histogram.rs:246-259) on "hist"
+  SADMC_DF void zero_out_hist() {
+    BC &c = bc[BC_HIST];
+    if (!c.exists) return;
+    for (int64_t j = lane; j < len; j += G) B[base + j].hist_count = 0;
+    c.min_total = 0.0;
+    c.max_total = -__builtin_inf();
+    c.min_count = 0;
+    c.max_count = 0;
+    c.total_count = 0;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  }
+
+  // count_states with e >= too_lo && e <= too_hi (histogram.rs:191-200)
+  SADMC_DF uint64_t count_states() const {
+    uint64_t t = 0;
+    for (int64_t j = lane; j < len; j += G) {
+      const double en = energy_of(j);
+      if (en >= too_lo && en <= too_hi) t += 1;
+    }
+    return group_sum_u64(t);
+  }
+
+  // -------------------------------------------------------------- method
+  // energy_binning.rs:503-530
+  SADMC_DF double gamma_now() const {
+    if (method == SADMC_SAD) {
+      const double ns = (double)num_states;
+      if (latest * tF * ns == 0.0) return 0.0;
+      const double t = (double)moves;
+      return (latest + t / tF) / (latest + t / ns * (t / tF));
+    }
+    if (method == SADMC_SAMC) {
+      const double t = (double)moves;
+      return t > t0 ? t0 / t : 1.0;
+    }
+    return gamma;
+  }
+
+  // energy_binning.rs:276-319
+  SADMC_DF bool reject(double e1, double e2) {
+    double lnw1 = get_lnw(e1), lnw2 = get_lnw(e2);
+    if (method == SADMC_SAD) {
+      if (e1 < too_lo)
+        lnw1 = get_lnw(too_lo) + (e1 - too_lo) / min_T;
+      else if (e1 > too_hi)
+        lnw1 = get_lnw(too_hi);
+      if (e2 < too_lo)
+        lnw2 = get_lnw(too_lo) + (e2 - too_lo) / min_T;
+      else if (e2 > too_hi)
+        lnw2 = get_lnw(too_hi);
+      const bool rejected = lnw2 > lnw1 && gen_f64(e.s0, e.s1) > exp(lnw1 - lnw2);
+      if (!rejected && get_count(e2) == 0 && e2 < too_hi && e2 > too_lo) tL = moves;
+      return rejected;
+    }
+    return lnw2 > lnw1 && gen_f64(e.s0, e.s1) > exp(lnw1 - lnw2);
+  }
+
+  // set_lnw (histogram.rs:182-190) for SAD's two range extensions: bins
+  // above `edge` (up) or below it (down) with a nonzero count take `v` (up)
+  // or v + (e - edge)/min_T (down) and lose their count.
+  SADMC_DF void set_lnw_sad(bool up, double edge, double v) {
+    for (int64_t j = lane; j < len; j += G) {
+      const double en = energy_of(j);
+      const bool side = up ? en > edge : en < edge;
+      if (side && get_count(en) > 0) {
+        sadmc_bbin &r = B[base + j];
+        r.lnw_total = up ? v : v + (en - edge) / min_T;
+        r.lnw_count = 0;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  }
+
+  // energy_binning.rs:320-500
+  SADMC_DF bool update(double en) {
+    const double g = gamma_now();
+    const uint64_t old_highest = bc[BC_LNW].max_count;
+    const uint64_t old_here = get_count(en);
+    if (en > max_e) max_e = en;
+    if (en < min_e) min_e = en;
+    if (!prep(en)) return false;
+    const int64_t idx = index(en);
+    if (idx < 0 || idx >= len) {  // an index panic in the reference
+      status = SADMC_WS_INDEX;
+      return false;
+    }
+    lnw_increment(en, idx, g);
+    if (method == SADMC_SAD) {
+      const uint64_t here = get_count(en);
+      if (old_here == 0 && !accumulate(BC_TF, en, (double)moves)) return false;
+      if (here > old_highest) {
+        if (en > too_hi) {
+          set_lnw_sad(true, too_hi, get_lnw(too_hi));
+          latest = (en - too_lo) / min_T;
+          tL = moves;
+          too_hi = en;
+          num_states = count_states();
+        } else if (en < too_lo) {
+          set_lnw_sad(false, too_lo, get_lnw(too_lo));
+          latest = (too_hi - en) / min_T;
+          tL = moves;
+          too_lo = en;
+          num_states = count_states();
+        }
+      }
+      if (tL == moves) {
+        const double old_tF = tF;
+        tF = bc[BC_TF].exists ? bc[BC_TF].max_total : 0.0;
+        if (old_tF != tF && args().move_plan == SADMC_ACCEPTANCE_RATE) {
+          double s = acc_rate / args().move_param;
+          s = s < 0.8 ? 0.8 : (s > 1.2 ? 1.2 : s);
+          tscale *= s;
+          if (SYS != SYS_ISING) usc = uniform_scale(-tscale, tscale);
+        }
+      }
+    } else if (method == SADMC_WL) {
+      const double old_lowest = bc[BC_HIST].exists ? (double)bc[BC_HIST].min_count / width : 0.0;
+      if (!accumulate(BC_HIST, en, 0.0)) return false;
+      if (has_min_gamma && gamma < min_gamma) return true;
+      const double lowest = (double)bc[BC_HIST].min_count / width;
+      const bool ok_lo = !args().has_min_e || get_count(args().min_e) > 0;
+      const bool ok_hi = !args().has_max_e || get_count(args().max_e) > 0;
+      if (lowest > old_lowest && ok_lo && ok_hi) {
+        const double mean = (double)bc[BC_HIST].total_count / (width * (double)len);
+        if ((inv_t && lowest > 0.0) || lowest >= 0.8 * mean) {
+          gamma *= 0.5;
+          zero_out_hist();
+          if (has_min_gamma && gamma < min_gamma) gamma = 0.0;
+        }
+        if (inv_t && gamma < (double)len / (double)moves) {
+          method = SADMC_SAMC;
+          t0 = (double)len;
+        }
+      }
+    }
+    return true;
+  }
+
+  // energy_binning.rs:566-611
+  SADMC_DF void move_once() {
+    moves += 1;
+    if (SYS != SYS_ISING && moves % 100000000ULL == 0) {
+      e.verify_energy();
+      if (e.status) {
+        status = e.status;
+        return;
+      }
+    }
+    const double e1 = e.E;
+    if (!accumulate(BC_EN, e1, e1)) return;
+    if (SYS == SYS_WCA && args().collect_pressure && moves % ((uint64_t)e.n * (uint64_t)e.n) == 0) {
+      const double pv = e.pressure();
+      if (!accumulate(BC_PR, e1, pv)) return;
+    }
+    const double rs = sqrt(1.0 / (double)moves);
+    acc_rate *= 1.0 - rs;
+    double e2;
+    if (e.cube_plan(tscale, usc, e2)) {
+      bool oob = false;
+      if (args().has_max_e) oob = e2 > args().max_e && e2 > e1;
+      if (args().has_min_e) oob = oob || (e2 < args().min_e && e2 < e1);
+      if (!oob && !reject(e1, e2)) {
+        accepted += 1;
+        acc_rate += rs;
+        e.confirm();
+      }
+    }
+    update(e.E);
+  }
+
+  // energy_binning.rs:533-564: the greedy descent below max_allowed_energy
+  // and the initial method state (the histogram stays empty until the
+  // first move).
+  SADMC_DF void from_params() {
+    e.E = e.full_energy();
+    e.err = 0.0;
+    if (args().has_max_e) {
+      const double sc = SYS == SYS_ISING ? 0.0 : uniform_scale(-0.05, 0.05);
+      for (uint64_t k = 0; k < 100000000ULL; k++) {
+        double newe;
+        if (e.cube_plan(0.05, sc, newe)) {
+          if (newe < e.E) e.confirm();
+          if (e.E < args().max_e) break;
+        }
+      }
+    }
+    const double E = e.E;
+    too_lo = E;
+    too_hi = E;
+    num_states = 0;
+    tL = 0;
+    tF = 0.0;
+    latest = 0.0;
+    min = (round(E / width) - 0.5) * width;  // Bins::new (histogram.rs:162-170)
+    min_e = E;
+    max_e = E;
+    len = 0;
+    moves = 0;
+    accepted = 0;
+    acc_rate = 0.5;
+    tscale = args().move_plan == SADMC_TRANSLATION_SCALE ? args().move_param : 0.05;
+    // ln w's BinCounts::new(0); the extras do not exist yet (all zero)
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+      bc[k].exists = 0;
+      bc[k].min_total = 0.0;
+      bc[k].max_total = 0.0;
+      bc[k].e_max_total = 0.0;
+      bc[k].e_max_count = 0.0;
+      bc[k].min_count = 0;
+      bc[k].max_count = 0;
+      bc[k].total_count = 0;
+    }
+    bc[BC_LNW].exists = 1;
+    bc[BC_LNW].e_max_total = -__builtin_inf();
+    bc[BC_LNW].e_max_count = -__builtin_inf();
+    status = e.status;
+  }
+};
+
+template <int SYS, int KMAX>
+__global__ __launch_bounds__(64) void k_bin(BinArgs ba, SetArgs sa, uint64_t nmoves, int init) {
+  extern __shared__ double lds[];
+  const int G = sa.G;
+  const int group = threadIdx.x / G;
+  const int lane = threadIdx.x & (G - 1);
+  const int walker = blockIdx.x * (64 / G) + group;
+  if (walker >= sa.W) return;
+  BinWalker<SYS, KMAX> b;
+  b.load(ba, sa, walker, lane, lds + group * 3 * sa.npad);
+  if (init) {
+    b.from_params();
+  } else {
+    for (uint64_t k = 0; k < nmoves && !b.status; k++) b.move_once();
+  }
+  b.store(walker);
+}
+
+template <int SYS, int KMAX>
+static hipError_t launch_bin_t(const BinArgs &ba, const SetArgs &sa, uint64_t moves, int init, hipStream_t s) {
+  const int per = 64 / sa.G;
+  const int blocks = (sa.W + per - 1) / per;
+  const size_t lds = SYS == SYS_ISING ? 0 : sizeof(double) * 3 * (size_t)per * sa.npad;
+  hipLaunchKernelGGL((k_bin<SYS, KMAX>), dim3(blocks), dim3(64), lds, s, ba, sa, moves, init);
+  return hipGetLastError();
+}
+
+template <int SYS>
+static hipError_t launch_bin_s(int kmax, const BinArgs &ba, const SetArgs &sa, uint64_t moves, int init,
+                               hipStream_t s) {
+  switch (kmax) {
+    case 1: return launch_bin_t<SYS, 1>(ba, sa, moves, init, s);
+    case 2: return launch_bin_t<SYS, 2>(ba, sa, moves, init, s);
+    case 4: return launch_bin_t<SYS, 4>(ba, sa, moves, init, s);
+    case 8: return launch_bin_t<SYS, 8>(ba, sa, moves, init, s);
+    case 16: return launch_bin_t<SYS, 16>(ba, sa, moves, init, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_binning(int sys, int kmax, const BinArgs &ba, const SetArgs &sa, uint64_t moves, int init,
+                          hipStream_t s) {
+  if (sys == SYS_ISING) return launch_bin_t<SYS_ISING, 1>(ba, sa, moves, init, s);
+  if (sys == SYS_LJ) return launch_bin_s<SYS_LJ>(kmax, ba, sa, moves, init, s);
+  if (sys == SYS_WCA) return launch_bin_s<SYS_WCA>(kmax, ba, sa, moves, init, s);
+  return hipErrorInvalidValue;
+}
+
+}  // namespace sadmc

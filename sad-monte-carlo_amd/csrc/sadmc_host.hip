@@ -80,6 +80,21 @@ struct PtHost {
   uint64_t *d_spins = nullptr, *d_lrng = nullptr;
 };
 
+// A walker set of the binning path on the device.
+struct BinSetHost {
+  sadmc_set_params p;
+  int32_t W = 0;
+  int G = 1, K = 1, KMAX = 1, npad = 0, sys = 0;
+  int64_t cap = 0;
+  SetArgs args;
+  BinArgs ba;
+  sadmc_bwalker_state *d_ws = nullptr;
+  sadmc_bbin *d_bins = nullptr;
+  double *d_pos = nullptr;
+  uint64_t *d_spins = nullptr;
+  int32_t *d_flag = nullptr;
+};
+
 }  // namespace
 
 struct sadmc_ctx {
@@ -110,6 +125,7 @@ struct sadmc_ctx {
   float *d_grad = nullptr, *d_loss = nullptr;
   int loss_cap = 0;
   std::vector<PtHost> pts;
+  std::vector<BinSetHost> bsets;
 };
 
 static int fail(sadmc_ctx *c, int code, const std::string &msg) {
@@ -335,6 +351,9 @@ void sadmc_destroy(sadmc_ctx *c) {
   for (void *p : {(void *)c->d_pw, (void *)c->d_pm, (void *)c->d_pv, (void *)c->d_pbf, (void *)c->d_rec,
                   (void *)c->d_partial, (void *)c->d_grad, (void *)c->d_loss})
     (void)hipFree(p);
+  for (auto &b : c->bsets)
+    for (void *p : {(void *)b.d_ws, (void *)b.d_bins, (void *)b.d_pos, (void *)b.d_spins, (void *)b.d_flag})
+      (void)hipFree(p);
   for (auto &t : c->pts)
     for (void *p : {(void *)t.d_rep, (void *)t.d_slot_e, (void *)t.d_pos, (void *)t.d_geom, (void *)t.d_slot_of,
                     (void *)t.d_spins, (void *)t.d_lrng})
@@ -1344,6 +1363,213 @@ int sadmc_pt_ladder_rng(sadmc_ctx *c, int32_t pt, uint64_t *out) {
   if (!t || !out) return fail(c, SADMC_E_ARG, "no such tempering ladder set or null output");
   HIPCHECK(c, hipMemcpyAsync(out, t->d_lrng, sizeof(uint64_t) * 2 * t->L, hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(c, hipStreamSynchronize(c->stream));
+  return SADMC_OK;
+}
+
+// --------------------------------------------------- binning-path walkers
+
+int sadmc_bin_add_set(sadmc_ctx *c, const sadmc_set_params *p, int32_t W, const uint64_t *seeds, const double *geom,
+                      const double *coords, const int8_t *spins, int32_t *set_id) {
+  if (!c || !p || W <= 0 || !seeds) return fail(c, SADMC_E_ARG, "null argument or no walkers");
+  if (p->method != SADMC_SAD && p->method != SADMC_SAMC && p->method != SADMC_WL && p->method != SADMC_INV_T_WL)
+    return fail(c, SADMC_E_ARG, "the binning walker runs SAD, SAMC, WL or 1/t-WL");
+  if (p->precision != SADMC_FP64 || p->refresh != 0 || p->move_kind != SADMC_MOVE_CUBE)
+    return fail(c, SADMC_E_ARG, "the binning walker runs the cube move in FP64 with the error-estimate energy rule");
+  BinSetHost b;
+  b.p = *p;
+  b.W = W;
+  const int n = p->n;
+  if (p->system == SADMC_ISING) {
+    if (n < 2 || n > 8) return fail(c, SADMC_E_ARG, "Ising lattice width must be 2..8");
+    b.sys = SYS_ISING;
+  } else if (p->system == SADMC_LJ || p->system == SADMC_WCA) {
+    if (n < 2 || n > 1024) return fail(c, SADMC_E_ARG, "atom count must be 2..1024");
+    if (!geom) return fail(c, SADMC_E_ARG, "LJ/WCA need per-walker geometry");
+    if (p->move_plan == SADMC_TRANSLATION_SCALE && !(p->move_param > 0))
+      return fail(c, SADMC_E_ARG, "translation scale must be > 0");
+    b.sys = p->system == SADMC_LJ ? SYS_LJ : SYS_WCA;
+    int G = p->lanes > 0 ? p->lanes : auto_lanes(n);
+    if (G < 2 || G > 64 || (G & (G - 1))) return fail(c, SADMC_E_ARG, "lanes must be a power of two in 2..64");
+    b.G = G;
+    b.K = (n + G - 1) / G;
+    if (b.K > 16) return fail(c, SADMC_E_ARG, "too few lanes for this many atoms (max 16 atoms per lane)");
+    b.KMAX = b.K <= 1 ? 1 : b.K <= 2 ? 2 : b.K <= 4 ? 4 : b.K <= 8 ? 8 : 16;
+    b.npad = G * b.K;
+  } else {
+    return fail(c, SADMC_E_ARG, "unknown system");
+  }
+  // the window: BinningParams' default width 1.0, bounds as for the main sets
+  sadmc_set_params q = *p;
+  const double width = p->energy_bin > 0 ? p->energy_bin : 1.0;
+  q.energy_bin = width;
+  double e_lo, e_hi;
+  sadmc_bin_window(&q, geom ? geom[0] : 0.0, &e_lo, &e_hi, &b.cap);
+  std::vector<uint64_t> bits;
+  std::vector<double> pos;
+  if (b.sys == SYS_ISING) {
+    bits.assign(W, 0);
+    std::vector<int8_t> sp((size_t)n * n);
+    for (int w = 0; w < W; w++) {
+      if (spins)
+        std::memcpy(sp.data(), spins + (size_t)w * n * n, (size_t)n * n);
+      else
+        sadmc_init_spins(n, p->ising_init, seeds[w], sp.data());
+      for (int k = 0; k < n * n; k++)
+        if (sp[k] > 0) bits[w] |= 1ULL << k;
+    }
+  } else {
+    pos.assign((size_t)W * 3 * b.npad, 0.0);
+    std::vector<double> cf((size_t)3 * n);
+    for (int w = 0; w < W; w++) {
+      if (coords) {
+        std::memcpy(cf.data(), coords + (size_t)w * 3 * n, sizeof(double) * 3 * n);
+      } else if (sadmc_init_coords(p->system, n, geom[w], seeds[w], cf.data())) {
+        return fail(c, SADMC_E_INIT, "initial configuration failed");
+      }
+      for (int d = 0; d < 3; d++)
+        for (int j = 0; j < n; j++) pos[((size_t)w * 3 + d) * b.npad + j] = cf[(size_t)d * n + j];
+    }
+  }
+  std::vector<sadmc_bwalker_state> ws(W);
+  for (int w = 0; w < W; w++) {
+    sadmc_bwalker_state &s = ws[w];
+    std::memset(&s, 0, sizeof s);
+    rng_seed(seeds[w], s.rng[0], s.rng[1]);
+    s.width = width;
+    s.cap = b.cap;
+    s.geom = geom ? geom[w] : 0.0;
+    s.method = p->method == SADMC_INV_T_WL ? SADMC_WL : p->method;
+    s.inv_t = p->method == SADMC_INV_T_WL;
+    s.has_min_gamma = p->method == SADMC_WL && p->has_min_gamma;
+    s.min_gamma = p->method_param;
+    s.min_T = p->method_param;
+    s.t0 = p->method_param;
+    s.gamma = 1.0;
+  }
+  HIPCHECK(c, hipSetDevice(c->device));
+  HIPCHECK(c, hipMalloc(&b.d_ws, sizeof(sadmc_bwalker_state) * W));
+  HIPCHECK(c, hipMalloc(&b.d_bins, sizeof(sadmc_bbin) * (size_t)W * b.cap));
+  HIPCHECK(c, hipMalloc(&b.d_flag, sizeof(int32_t)));
+  HIPCHECK(c, hipMemsetAsync(b.d_bins, 0, sizeof(sadmc_bbin) * (size_t)W * b.cap, c->stream));
+  HIPCHECK(c, hipMemsetAsync(b.d_flag, 0, sizeof(int32_t), c->stream));
+  HIPCHECK(c, hipMemcpyAsync(b.d_ws, ws.data(), sizeof(sadmc_bwalker_state) * W, hipMemcpyHostToDevice, c->stream));
+  if (b.sys == SYS_ISING) {
+    HIPCHECK(c, hipMalloc(&b.d_spins, sizeof(uint64_t) * W));
+    HIPCHECK(c, hipMemcpyAsync(b.d_spins, bits.data(), sizeof(uint64_t) * W, hipMemcpyHostToDevice, c->stream));
+  } else {
+    HIPCHECK(c, hipMalloc(&b.d_pos, sizeof(double) * pos.size()));
+    HIPCHECK(c, hipMemcpyAsync(b.d_pos, pos.data(), sizeof(double) * pos.size(), hipMemcpyHostToDevice, c->stream));
+  }
+  SetArgs &a = b.args;
+  std::memset(&a, 0, sizeof a);
+  a.pos = b.d_pos;
+  a.spins = b.d_spins;
+  a.flag = b.d_flag;
+  a.W = W;
+  a.n = n;
+  a.npad = b.npad;
+  a.G = b.G;
+  a.K = b.K;
+  a.method = SADMC_CANONICAL;
+  a.precision = SADMC_FP64;
+  a.move_kind = SADMC_MOVE_CUBE;
+  a.collect_pressure = p->collect_pressure;
+  a.rc2 = p->rc2;
+  a.width = 1.0;
+  a.inv_width = 1.0;
+  const uint64_t nn = (uint64_t)n;
+  a.zone_n = ~0ULL - ((~0ULL - nn + 1) % nn);
+  a.zone_site = (nn << __builtin_clzll(nn)) - 1;
+  BinArgs &ba = b.ba;
+  ba.ws = b.d_ws;
+  ba.bins = b.d_bins;
+  ba.flag = b.d_flag;
+  ba.cap = b.cap;
+  ba.e_lo = e_lo;
+  ba.move_plan = p->move_plan;
+  ba.has_min_e = p->has_min_e;
+  ba.has_max_e = p->has_max_e;
+  ba.collect_pressure = p->collect_pressure;
+  ba.move_param = p->move_param;
+  ba.min_e = p->min_e;
+  ba.max_e = p->max_e;
+  HIPCHECK(c, launch_binning(b.sys, b.KMAX, ba, a, 0, 1, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  c->bsets.push_back(b);
+  if (set_id) *set_id = (int32_t)c->bsets.size() - 1;
+  return SADMC_OK;
+}
+
+static BinSetHost *bset_get(sadmc_ctx *c, int32_t set) {
+  if (!c || set < 0 || set >= (int32_t)c->bsets.size()) return nullptr;
+  return &c->bsets[set];
+}
+
+int sadmc_bin_run(sadmc_ctx *c, int32_t set, uint64_t moves) {
+  BinSetHost *b = bset_get(c, set);
+  if (!b) return fail(c, SADMC_E_ARG, "no such binning walker set");
+  HIPCHECK(c, hipSetDevice(c->device));
+  if (moves) HIPCHECK(c, launch_binning(b->sys, b->KMAX, b->ba, b->args, moves, 0, c->stream));
+  int32_t flag = 0;
+  HIPCHECK(c, hipMemcpyAsync(&flag, b->d_flag, sizeof flag, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  if (!flag) return SADMC_OK;
+  std::vector<sadmc_bwalker_state> ws(b->W);
+  HIPCHECK(c, hipMemcpy(ws.data(), b->d_ws, sizeof(sadmc_bwalker_state) * b->W, hipMemcpyDeviceToHost));
+  for (int w = 0; w < b->W; w++) {
+    const int st = ws[w].status;
+    if (!st) continue;
+    char buf[160];
+    std::snprintf(buf, sizeof buf, "binning set %d walker %d stopped with status %d at move %llu", set, w, st,
+                  (unsigned long long)ws[w].moves);
+    return fail(c, st == SADMC_WS_CAPACITY ? SADMC_E_CAPACITY : st == SADMC_WS_VERIFY ? SADMC_E_VERIFY : SADMC_E_INDEX,
+                buf);
+  }
+  return SADMC_OK;
+}
+
+int sadmc_bin_state(sadmc_ctx *c, int32_t set, int32_t w, sadmc_bwalker_state *out) {
+  BinSetHost *b = bset_get(c, set);
+  if (!b || !out || w < 0 || w >= b->W) return fail(c, SADMC_E_ARG, "no such binning walker or null output");
+  HIPCHECK(c, hipMemcpyAsync(out, b->d_ws + w, sizeof *out, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  return SADMC_OK;
+}
+
+int sadmc_bin_bins(sadmc_ctx *c, int32_t set, int32_t w, int64_t max_bins, sadmc_bbin *out) {
+  sadmc_bwalker_state st;
+  int rc = sadmc_bin_state(c, set, w, &st);
+  if (rc) return rc;
+  if (st.len > max_bins || !out) return fail(c, SADMC_E_CAPACITY, "output holds fewer bins than the walker has");
+  BinSetHost *b = bset_get(c, set);
+  if (st.len > 0)
+    HIPCHECK(c, hipMemcpyAsync(out, b->d_bins + (int64_t)w * b->cap + st.base, sizeof(sadmc_bbin) * st.len,
+                               hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  return SADMC_OK;
+}
+
+int sadmc_bin_coords(sadmc_ctx *c, int32_t set, int32_t w, double *coords) {
+  BinSetHost *b = bset_get(c, set);
+  if (!b || !coords || w < 0 || w >= b->W) return fail(c, SADMC_E_ARG, "no such binning walker or null output");
+  if (b->sys == SYS_ISING) return fail(c, SADMC_E_ARG, "Ising walkers have spins, not coordinates");
+  const int n = b->p.n;
+  for (int d = 0; d < 3; d++)
+    HIPCHECK(c, hipMemcpyAsync(coords + (size_t)d * n, b->d_pos + ((int64_t)w * 3 + d) * b->npad, sizeof(double) * n,
+                               hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  return SADMC_OK;
+}
+
+int sadmc_bin_spins(sadmc_ctx *c, int32_t set, int32_t w, int8_t *spins) {
+  BinSetHost *b = bset_get(c, set);
+  if (!b || !spins || w < 0 || w >= b->W) return fail(c, SADMC_E_ARG, "no such binning walker or null output");
+  if (b->sys != SYS_ISING) return fail(c, SADMC_E_ARG, "only Ising walkers have spins");
+  uint64_t bits = 0;
+  HIPCHECK(c, hipMemcpyAsync(&bits, b->d_spins + w, sizeof bits, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  const int n = b->p.n;
+  for (int k = 0; k < n * n; k++) spins[k] = ((bits >> k) & 1ULL) ? 1 : -1;
   return SADMC_OK;
 }
 

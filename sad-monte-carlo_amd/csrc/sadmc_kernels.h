@@ -67,6 +67,21 @@ struct PtArgs {
   int32_t R, steps;
   double d, sc;        // cube half-width and its Uniform scale
 };
+// Walker sets of the binning path (k_binning.hip); the system part uses
+// the SetArgs of the set (positions, spins, lanes), the rest is here.
+struct BinArgs {
+  sadmc_bwalker_state *ws;  // [W]
+  sadmc_bbin *bins;         // [W][cap]
+  int32_t *flag;            // set to 1 when a walker stops with a status
+  int64_t cap;
+  double e_lo;              // energy of physical bin 0's lower edge, for placement
+  int32_t move_plan, has_min_e, has_max_e, collect_pressure;
+  double move_param, min_e, max_e;
+};
+// `init` runs from_params (energy, descent, method state) instead of moves.
+hipError_t launch_binning(int sys, int kmax, const BinArgs &ba, const SetArgs &sa, uint64_t moves, int init,
+                          hipStream_t s);
+
 // One workgroup of `threads` lanes per ladder; `init` computes each slot's
 // energy instead of running rounds.
 size_t pt_lds_bytes(int R, int npad);

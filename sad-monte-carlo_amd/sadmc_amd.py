@@ -74,6 +74,28 @@ REPLICA_DTYPE = np.dtype([
     ("accepted_swaps", "<u8"), ("rejected_swaps", "<u8"), ("geom", "<f8"), ("status", "<i4"), ("reserved0", "<i4"),
 ])
 
+# sadmc_bcounts / sadmc_bwalker_state / sadmc_bbin: walkers of the binning path.
+BCOUNTS_DTYPE = np.dtype([
+    ("min_total", "<f8"), ("max_total", "<f8"), ("e_max_total", "<f8"), ("e_max_count", "<f8"),
+    ("min_count", "<u8"), ("max_count", "<u8"), ("total_count", "<u8"), ("exists", "<i4"), ("reserved", "<i4"),
+])
+BWALKER_STATE_DTYPE = np.dtype([
+    ("rng", "<u8", 2), ("E", "<f8"), ("error", "<f8"), ("moves", "<u8"), ("accepted_moves", "<u8"),
+    ("acceptance_rate", "<f8"), ("translation_scale", "<f8"), ("min", "<f8"), ("width", "<f8"), ("min_e", "<f8"),
+    ("max_e", "<f8"), ("len", "<i8"), ("method", "<i4"), ("status", "<i4"), ("num_states", "<u8"), ("tL", "<u8"),
+    ("min_T", "<f8"), ("too_lo", "<f8"), ("too_hi", "<f8"), ("tF", "<f8"), ("latest_parameter", "<f8"),
+    ("t0", "<f8"), ("gamma", "<f8"),
+    ("lnw", BCOUNTS_DTYPE), ("energy", BCOUNTS_DTYPE), ("t_found", BCOUNTS_DTYPE), ("hist", BCOUNTS_DTYPE),
+    ("pressure", BCOUNTS_DTYPE),
+    ("base", "<i8"), ("cap", "<i8"), ("geom", "<f8"), ("inv_t", "<i4"), ("has_min_gamma", "<i4"),
+    ("min_gamma", "<f8"), ("reserved", "<u8", 3),
+])
+BBIN_DTYPE = np.dtype([
+    ("lnw_total", "<f8"), ("lnw_count", "<u8"), ("energy_total", "<f8"), ("energy_count", "<u8"),
+    ("t_found_total", "<f8"), ("t_found_count", "<u8"), ("hist_count", "<u8"), ("pressure_total", "<f8"),
+    ("pressure_count", "<u8"), ("reserved", "<u8"),
+])
+
 class WalkerStateC(C.Structure):
     _fields_ = [(name, C.c_uint8 * WALKER_STATE_DTYPE.itemsize) for name in ("raw",)]
 
@@ -90,6 +112,7 @@ EXPORTS = [
     "sadmc_prop_collect", "sadmc_prop_records", "sadmc_prop_load_records", "sadmc_prop_gradient",
     "sadmc_prop_train", "sadmc_mean_entropy",
     "sadmc_pt_add", "sadmc_pt_run", "sadmc_pt_replicas", "sadmc_pt_coords", "sadmc_pt_spins", "sadmc_pt_ladder_rng",
+    "sadmc_bin_add_set", "sadmc_bin_run", "sadmc_bin_state", "sadmc_bin_bins", "sadmc_bin_coords", "sadmc_bin_spins",
 ]
 
 PROP_NP, PROP_REC, PROP_NS = 1377, 24, 16
@@ -146,6 +169,12 @@ def lib():
     L.sadmc_pt_coords.argtypes = [vp, i32, i32, i32, vp]
     L.sadmc_pt_spins.argtypes = [vp, i32, i32, i32, vp]
     L.sadmc_pt_ladder_rng.argtypes = [vp, i32, vp]
+    L.sadmc_bin_add_set.argtypes = [vp, C.POINTER(SetParams), i32, vp, vp, vp, vp, C.POINTER(i32)]
+    L.sadmc_bin_run.argtypes = [vp, i32, u64]
+    L.sadmc_bin_state.argtypes = [vp, i32, i32, vp]
+    L.sadmc_bin_bins.argtypes = [vp, i32, i32, i64, vp]
+    L.sadmc_bin_coords.argtypes = [vp, i32, i32, vp]
+    L.sadmc_bin_spins.argtypes = [vp, i32, i32, vp]
     for name in EXPORTS:
         if name not in ("sadmc_last_error", "sadmc_destroy", "sadmc_abi_version"):
             getattr(L, name).restype = C.c_int
@@ -266,6 +295,45 @@ class Context:
         E = e0.value + np.arange(nb.value) * w
         keep = cnt > 0
         return E[keep], S[keep], cnt[keep]
+
+    # ---- walkers of the binning path (energy_binning.rs)
+    def bin_add_set(self, params, seeds, geom=None, coords=None, spins=None):
+        seeds = np.ascontiguousarray(seeds, dtype=np.uint64)
+        g = None if geom is None else np.ascontiguousarray(np.broadcast_to(geom, seeds.shape), dtype=np.float64)
+        c = None if coords is None else np.ascontiguousarray(coords, dtype=np.float64)
+        sp = None if spins is None else np.ascontiguousarray(spins, dtype=np.int8)
+        sid = C.c_int32()
+        self._check(lib().sadmc_bin_add_set(self.h, C.byref(params), len(seeds), _p(seeds), _p(g), _p(c), _p(sp),
+                                            C.byref(sid)), "bin_add_set")
+        if not hasattr(self, "bsets"):
+            self.bsets = []
+        self.bsets.append((params, len(seeds)))
+        return sid.value
+
+    def bin_run(self, s, moves):
+        self._check(lib().sadmc_bin_run(self.h, s, moves), "bin_run")
+
+    def bin_state(self, s, w):
+        st = np.zeros((), BWALKER_STATE_DTYPE)
+        self._check(lib().sadmc_bin_state(self.h, s, w, _p(st)), "bin_state")
+        return st
+
+    def bin_bins(self, s, w):
+        n = int(self.bin_state(s, w)["len"])
+        out = np.zeros(max(n, 1), BBIN_DTYPE)
+        self._check(lib().sadmc_bin_bins(self.h, s, w, len(out), _p(out)), "bin_bins")
+        return out[:n]
+
+    def bin_coords(self, s, w):
+        c = np.zeros(3 * self.bsets[s][0].n)
+        self._check(lib().sadmc_bin_coords(self.h, s, w, _p(c)), "bin_coords")
+        return c
+
+    def bin_spins(self, s, w):
+        n = self.bsets[s][0].n
+        a = np.zeros(n * n, np.int8)
+        self._check(lib().sadmc_bin_spins(self.h, s, w, _p(a)), "bin_spins")
+        return a
 
     # ---- parallel tempering
     def pt_add(self, params, temps, seeds, geom=None, coords=None, spins=None, canonical_steps=1):

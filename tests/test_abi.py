@@ -41,6 +41,8 @@ def test_struct_layouts_match_python_mirrors():
 int main(void) {
   printf("%zu %zu %zu %zu %zu\n", sizeof(sadmc_walker_state), sizeof(sadmc_bin), sizeof(sadmc_set_params),
          offsetof(sadmc_walker_state, cap), offsetof(sadmc_set_params, move_kind));
+  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(sadmc_replica), sizeof(sadmc_bwalker_state), sizeof(sadmc_bbin),
+         sizeof(sadmc_bcounts), offsetof(sadmc_bwalker_state, pressure), offsetof(sadmc_bwalker_state, min_gamma));
   return 0;
 }
 """
@@ -50,7 +52,17 @@ int main(void) {
         open(c, "w").write(src)
         subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", exe, c], check=True)
         out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()
-    ws, b, sp, cap_off, mk_off = map(int, out)
+    ws, b, sp, cap_off, mk_off = map(int, out[:5])
+    rep, bws, bb, bc, pr_off, mg_off = map(int, out[5:])
+    assert rep == S.REPLICA_DTYPE.itemsize
+    assert bws == S.BWALKER_STATE_DTYPE.itemsize and bb == S.BBIN_DTYPE.itemsize == 80
+    assert bc == S.BCOUNTS_DTYPE.itemsize == 64
+    assert pr_off == S.BWALKER_STATE_DTYPE.fields["pressure"][1]
+    assert mg_off == S.BWALKER_STATE_DTYPE.fields["min_gamma"][1]
+    # the binning walker's state starts with the oracle's record, field for field
+    import oracle as Orc
+    for name in Orc.BSTATE_DTYPE.names:
+        assert S.BWALKER_STATE_DTYPE.fields[name][1] == Orc.BSTATE_DTYPE.fields[name][1], name
     assert ws == S.WALKER_STATE_DTYPE.itemsize
     assert b == S.BIN_DTYPE.itemsize == 64
     assert sp == __import__("ctypes").sizeof(S.SetParams)
