@@ -347,6 +347,75 @@ def tempering(rounds, ladders, replicas, lanes, cpu_seconds):
     return out
 
 
+def binning_path(moves, walkers, lanes, cpu_seconds):
+    """The binning binary's walker (energy_binning.rs over a histogram) on the
+    LJ31 configuration: 16384 walkers, N = 31 in a sphere of radius 3.0, cube
+    moves 0.1, SAD min_T = 0.05, bins 0.01, max_allowed_energy 0, FP64.
+    One untimed launch of 2000 moves, then one timed launch of `moves` moves
+    per walker.  CPU baseline: the oracle's walker, one walker per host
+    thread, for about `cpu_seconds`."""
+    sp = S.make_params(system=S.LJ, n=N_ATOMS, lanes=lanes, method=S.SAD, method_param=0.05, move_param=0.1,
+                       energy_bin=0.01, has_max_e=1, max_e=0.0)
+    seeds = np.arange(walkers, dtype=np.uint64)
+    ctx = S.Context(0)
+    s = ctx.bin_add_set(sp, seeds, geom=RADIUS)
+    ctx.bin_run(s, WARMUP_MOVES)
+    t0 = time.perf_counter()
+    ctx.bin_run(s, moves)
+    wall = time.perf_counter() - t0
+    lens = [int(ctx.bin_state(s, w)["len"]) for w in range(0, walkers, max(1, walkers // 64))]
+    ctx.close()
+    total = walkers * moves
+    gbs = BYTES_PER_MOVE * total / wall / 1e9
+    out = {"moves_per_s": total / wall, "wall_s": wall, "walkers": walkers, "lanes": lanes, "moves": moves,
+           "warmup_moves": WARMUP_MOVES, "mean_bins": float(np.mean(lens)),
+           "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": gbs / HBM_PEAK_GBS, "bytes_per_move": BYTES_PER_MOVE}}
+    log(f"[binning] gpu {out['moves_per_s'] / 1e9:.3f} G moves/s ({wall:.2f} s)")
+    if cpu_seconds > 0:
+        # the same segment of each walker's run (moves WARMUP_MOVES .. WARMUP_MOVES + moves) for as many
+        # walkers per thread as fit in about cpu_seconds; the rate of a thread is its timed moves over its
+        # timed seconds, the threads run side by side
+        import threading
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        threads = min(16, os.cpu_count() or 1)
+
+        def params(seed):
+            return O.make_params(system=O.LJ, n=N_ATOMS, method=O.SAD, method_param=0.05, move_param=0.1,
+                                 energy_bin=0.01, has_max_e=1, max_e=0.0, radius=RADIUS, order_g=lanes,
+                                 seed=int(seed))
+        probe = O.BWalker(params(0))
+        probe.run(WARMUP_MOVES)
+        t0 = time.perf_counter()
+        probe.run(moves)
+        per_walker = time.perf_counter() - t0
+        del probe
+        reps = max(1, int(round(cpu_seconds / max(per_walker, 1e-6))))
+        timed = [0.0] * threads
+
+        def work(t):
+            for r in range(reps):
+                w = O.BWalker(params(t + threads * r))
+                w.run(WARMUP_MOVES)
+                t1 = time.perf_counter()
+                w.run(moves)
+                timed[t] += time.perf_counter() - t1
+                del w
+        ths = [threading.Thread(target=work, args=(t,)) for t in range(threads)]
+        for t in ths:
+            t.start()
+        for t in ths:
+            t.join()
+        rate = sum(reps * moves / x for x in timed)
+        out["cpu_baseline"] = {"value": rate, "unit": "moves/s", "cores": threads, "kind": "port",
+                               "sample": f"{threads} threads x {reps} walkers, each timed over moves "
+                                         f"{WARMUP_MOVES}..{WARMUP_MOVES + moves} (the GPU's timed segment)"}
+        out["gpu_over_cpu"] = out["moves_per_s"] / rate
+        log(f"[binning] cpu {rate / 1e6:.2f} M moves/s on {threads} threads")
+    return out
+
+
 def copy_probe(nbytes):
     """A plain device-to-device copy of known size, for scaling the HBM byte
     counters (FETCH_SIZE, WRITE_SIZE) of a profiler run."""
@@ -377,6 +446,8 @@ def main():
     ap.add_argument("--learned-cv-system", default="lj38", choices=["lj38", "lj13"])
     ap.add_argument("--batched", type=int, default=0,
                     help="only run the mixed LJ/WCA batch (this many moves per launch)")
+    ap.add_argument("--binning", type=int, default=0,
+                    help="only run the binning-path LJ31 walkers for this many moves per walker")
     ap.add_argument("--tempering", type=int, default=0,
                     help="only run LJ31 parallel tempering ladders for this many rounds")
     a = ap.parse_args()
@@ -392,6 +463,11 @@ def main():
         print(json.dumps({"metric": "mixed LJ/WCA SAD batch moves/s on 1 GPU", "unit": "moves/s",
                           "higher_is_better": True, "walkers_per_size": 4096, "moves_per_launch": a.batched,
                           "launches": a.steps, **res}))
+        return
+    if a.binning > 0:
+        res = binning_path(a.binning, a.walkers, a.lanes, 0.0 if a.no_cpu else 15.0)
+        print(json.dumps({"metric": "LJ31 SAD moves/s of the binning-path walker on 1 GPU", "unit": "moves/s",
+                          "higher_is_better": True, "value": res["moves_per_s"], **res}))
         return
     if a.tempering > 0:
         res = tempering(a.tempering, 512, 16, a.lanes, 0.0 if a.no_cpu else 15.0)

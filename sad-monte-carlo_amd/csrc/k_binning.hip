@@ -5,9 +5,11 @@
 // The system part (plan_move, confirm, energies, pressure) is the main
 // engine's; the bookkeeping is this file's.  Every lane of a group holds the
 // walker's scalars (identical values), the bins live in HBM as one 80-byte
-// record per bin, and the O(bins) scans the reference makes (BinCounts'
-// min/max recomputations, set_lnw, count_states) are split over the group's
-// lanes and combined with shuffles so that the result is the sequential one.
+// record per bin.  The O(bins) scans the reference makes are either replaced
+// by exact trackers of the extremes (BinCounts' min/max recomputations) or
+// split over the group's lanes and combined with shuffles (set_lnw,
+// count_states, the trackers' own rescans), so that every result is the
+// sequential one.
 #define SADMC_KERNEL_TU
 #include "sadmc_kernels.h"
 
@@ -23,23 +25,6 @@ struct BC {
   uint64_t min_count, max_count, total_count;
   int32_t exists;
 };
-
-// f64::max / f64::min folded from NaN (histogram.rs:366-372): the first
-// element attaining the extreme.  Partial folds of interleaved lanes are
-// combined by value, then by the first index, which gives the same element.
-struct Ext {
-  double v;
-  int64_t i;
-};
-
-__device__ __forceinline__ Ext ext_pick(Ext a, Ext b, bool want_max) {
-  if (b.i < 0) return a;
-  if (a.i < 0) return b;
-  const bool better = want_max ? b.v > a.v : b.v < a.v;
-  if (better) return b;
-  if (b.v == a.v && b.i < a.i) return b;
-  return a;
-}
 
 __device__ __forceinline__ double shfl_f64(double v, int d) {
   return __builtin_bit_cast(double, (uint64_t)__shfl_xor((long long)__builtin_bit_cast(uint64_t, v), d, 64));
@@ -63,6 +48,14 @@ struct BinWalker {
   int32_t inv_t, has_min_gamma;
   double min_gamma;
   BC bc[5];
+  // Exact extremes of each array kept beside the reference's statistics,
+  // so that its min_of / max_of / min() rescans (histogram.rs:64-80,
+  // 226-244) become lookups: the smallest total and how many bins hold it,
+  // the smallest count and its multiplicity, and the largest ln w total.
+  // A full rescan is needed only when the last bin holding an extreme
+  // leaves it, after set_lnw / zero_out, and at the start of a launch.
+  double tmin[5], lmax;
+  uint64_t nmin[5], cmin[5], ncmin[5];
 
   SADMC_DF const BinArgs &args() const { return *Bp; }
 
@@ -137,6 +130,9 @@ struct BinWalker {
     bc_load(bc[BC_HIST], w.hist);
     bc_load(bc[BC_PR], w.pressure);
     B = ba.bins + (int64_t)walker * ba.cap;
+#pragma unroll
+    for (int k = 0; k < 5; k++)
+      if (bc[k].exists) rescan_all(k);
   }
 
   SADMC_DF void store(int walker) {
@@ -220,44 +216,143 @@ struct BinWalker {
   }
 
   // ---------------------------------------------------- group reductions
-  SADMC_DF Ext group_ext(Ext x, bool want_max) const {
-    for (int d = 1; d < G; d <<= 1) {
-      Ext o;
-      o.v = shfl_f64(x.v, d);
-      o.i = (int64_t)__shfl_xor((long long)x.i, d, 64);
-      x = ext_pick(x, o, want_max);
-    }
-    return x;
-  }
-  SADMC_DF uint64_t group_min_u64(uint64_t v) const {
-    for (int d = 1; d < G; d <<= 1) {
-      const uint64_t o = (uint64_t)__shfl_xor((long long)v, d, 64);
-      v = o < v ? o : v;
-    }
-    return v;
-  }
   SADMC_DF uint64_t group_sum_u64(uint64_t v) const {
     for (int d = 1; d < G; d <<= 1) v += (uint64_t)__shfl_xor((long long)v, d, 64);
     return v;
   }
-  // max_of / min_of of array k's totals, with bin `at` holding `val` (the
-  // value this move just wrote there)
-  SADMC_DF double scan_total(int k, bool want_max, int64_t at, double val) const {
-    Ext x{0.0, -1};
-    for (int64_t j = lane; j < len; j += G) {
-      const double v = j == at ? val : total_at(k, j);
-      x = ext_pick(x, Ext{v, j}, want_max);
+  // Rescans, four bins per lane per step so that their loads overlap.
+  // Smallest total of array k and its multiplicity (+inf, 0 when empty).
+  SADMC_DF void rescan_total(int k) {
+    double m = __builtin_inf();
+    uint64_t c = 0;
+    if (k != BC_HIST) {
+      for (int64_t j0 = lane; j0 < len; j0 += 4 * G) {
+        double v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int64_t j = j0 + u * G;
+          v[u] = j < len ? total_at(k, j) : __builtin_inf();
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          if (j0 + u * G >= len) continue;
+          if (v[u] < m) {
+            m = v[u];
+            c = 1;
+          } else if (v[u] == m) {
+            c += 1;
+          }
+        }
+      }
+    } else if (lane < len) {
+      m = 0.0;
+      c = (uint64_t)((len - lane + G - 1) / G);
     }
-    x = group_ext(x, want_max);
-    return x.i < 0 ? __builtin_nan("") : x.v;
+    for (int d = 1; d < G; d <<= 1) {
+      const double om = shfl_f64(m, d);
+      const uint64_t oc = (uint64_t)__shfl_xor((long long)c, d, 64);
+      if (om < m) {
+        m = om;
+        c = oc;
+      } else if (om == m) {
+        c += oc;
+      }
+    }
+    tmin[k] = m;
+    nmin[k] = c;
   }
-  SADMC_DF uint64_t scan_min_count(int k, int64_t at, uint64_t val) const {
-    uint64_t m = ~0ULL;
-    for (int64_t j = lane; j < len; j += G) {
-      const uint64_t v = j == at ? val : count_at(k, j);
-      m = v < m ? v : m;
+  SADMC_DF void rescan_count(int k) {
+    uint64_t m = ~0ULL, c = 0;
+    for (int64_t j0 = lane; j0 < len; j0 += 4 * G) {
+      uint64_t v[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int64_t j = j0 + u * G;
+        v[u] = j < len ? count_at(k, j) : ~0ULL;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        if (j0 + u * G >= len) continue;
+        if (v[u] < m) {
+          m = v[u];
+          c = 1;
+        } else if (v[u] == m) {
+          c += 1;
+        }
+      }
     }
-    return group_min_u64(m);
+    for (int d = 1; d < G; d <<= 1) {
+      const uint64_t om = (uint64_t)__shfl_xor((long long)m, d, 64);
+      const uint64_t oc = (uint64_t)__shfl_xor((long long)c, d, 64);
+      if (om < m) {
+        m = om;
+        c = oc;
+      } else if (om == m) {
+        c += oc;
+      }
+    }
+    cmin[k] = m;
+    ncmin[k] = c;
+  }
+  SADMC_DF void rescan_lmax() {
+    double m = -__builtin_inf();
+    for (int64_t j0 = lane; j0 < len; j0 += 4 * G) {
+      double v[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int64_t j = j0 + u * G;
+        v[u] = j < len ? B[base + j].lnw_total : -__builtin_inf();
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) m = v[u] > m ? v[u] : m;
+    }
+    for (int d = 1; d < G; d <<= 1) {
+      const double om = shfl_f64(m, d);
+      m = om > m ? om : m;
+    }
+    lmax = m;
+  }
+  SADMC_DF void rescan_all(int k) {
+    rescan_total(k);
+    rescan_count(k);
+    if (k == BC_LNW) rescan_lmax();
+  }
+  // One bin of array k goes from v to w (totals) / from a to b (counts).
+  SADMC_DF void track_total(int k, double v, double w) {
+    if (w < tmin[k]) {
+      tmin[k] = w;
+      nmin[k] = 1;
+    } else if (w == tmin[k]) {
+      if (v != w) nmin[k] += 1;
+    } else if (v == tmin[k]) {
+      if (--nmin[k] == 0) rescan_total(k);
+    }
+  }
+  SADMC_DF void track_count(int k, uint64_t a, uint64_t b) {
+    if (b < cmin[k]) {
+      cmin[k] = b;
+      ncmin[k] = 1;
+    } else if (b == cmin[k]) {
+      if (a != b) ncmin[k] += 1;
+    } else if (a == cmin[k]) {
+      if (--ncmin[k] == 0) rescan_count(k);
+    }
+  }
+  // a zero bin joins array k
+  SADMC_DF void track_new_zero(int k) {
+    if (0.0 < tmin[k]) {
+      tmin[k] = 0.0;
+      nmin[k] = 1;
+    } else if (0.0 == tmin[k]) {
+      nmin[k] += 1;
+    }
+    if (0 < cmin[k]) {
+      cmin[k] = 0;
+      ncmin[k] = 1;
+    } else {
+      ncmin[k] += 1;
+    }
+    if (k == BC_LNW && !(lmax >= 0.0)) lmax = 0.0;
   }
 
   // ------------------------------------------------------------ growth
@@ -284,6 +379,7 @@ struct BinWalker {
       if (bc[k].exists) {
         bc[k].min_total = 0.0;
         bc[k].min_count = 0;
+        track_new_zero(k);
       }
     len += 1;
   }
@@ -326,12 +422,18 @@ struct BinWalker {
     const double new_total = old_total + value;
     r.lnw_count = new_count;
     r.lnw_total = new_total;
-    if (new_total > c.max_total && scan_total(BC_LNW, true, idx, new_total) == new_total) {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    track_total(BC_LNW, old_total, new_total);
+    track_count(BC_LNW, old_count, new_count);
+    if (new_total > lmax) lmax = new_total;
+    // max_of(total) == total[idx], min_of(total) and min(count) after the
+    // update are the tracked extremes
+    if (new_total > c.max_total && lmax == new_total) {
       c.max_total = new_total;
       c.e_max_total = en;
     }
-    if (old_total == c.min_total) c.min_total = scan_total(BC_LNW, false, idx, new_total);
-    if (old_count == c.min_count) c.min_count = scan_min_count(BC_LNW, idx, new_count);
+    if (old_total == c.min_total) c.min_total = tmin[BC_LNW];
+    if (old_count == c.min_count) c.min_count = cmin[BC_LNW];
     if (new_count > c.max_count) {
       c.max_count = new_count;
       c.e_max_count = en;
@@ -349,6 +451,10 @@ struct BinWalker {
     }
     BC &c = bc[k];
     if (!c.exists) {  // BinCounts::new over the current bins
+      tmin[k] = 0.0;
+      nmin[k] = (uint64_t)len;
+      cmin[k] = 0;
+      ncmin[k] = (uint64_t)len;
       c.exists = 1;
       c.min_total = 0.0;
       c.max_total = 0.0;
@@ -360,15 +466,19 @@ struct BinWalker {
     }
     sadmc_bbin &r = B[base + idx];
     c.total_count += 1;
-    const uint64_t new_count = cnt(r, k) + 1;
-    cnt(r, k) = new_count;
-    if (new_count == c.min_count + 1) c.min_count = scan_min_count(k, idx, new_count);
-    if (new_count > c.max_count) c.max_count = new_count;
+    const uint64_t old_count = cnt(r, k);
+    const uint64_t new_count = old_count + 1;
     const double old_total = k == BC_HIST ? 0.0 : tot(r, k);
     const double new_total = old_total + value;
+    cnt(r, k) = new_count;
     if (k != BC_HIST) tot(r, k) = new_total;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    track_count(k, old_count, new_count);
+    if (k != BC_HIST) track_total(k, old_total, new_total);
+    if (new_count == c.min_count + 1) c.min_count = cmin[k];
+    if (new_count > c.max_count) c.max_count = new_count;
     if (new_total > c.max_total) c.max_total = new_total;
-    if (old_total == c.min_total) c.min_total = scan_total(k, false, idx, new_total);
+    if (old_total == c.min_total) c.min_total = k == BC_HIST ? 0.0 : tmin[k];
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     return true;
   }
@@ -383,6 +493,8 @@ struct BinWalker {
     c.min_count = 0;
     c.max_count = 0;
     c.total_count = 0;
+    cmin[BC_HIST] = 0;
+    ncmin[BC_HIST] = (uint64_t)len;
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   }
 
@@ -445,6 +557,7 @@ struct BinWalker {
       }
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    rescan_all(BC_LNW);
   }
 
   // energy_binning.rs:320-500
