@@ -60,6 +60,8 @@ enum sadmc_move_plan { SADMC_TRANSLATION_SCALE = 0, SADMC_ACCEPTANCE_RATE = 1 };
 enum sadmc_precision { SADMC_FP64 = 0, SADMC_FP32_PAIRS = 1 };
 /* Displacement proposal for LJ/WCA. */
 enum sadmc_move_kind { SADMC_MOVE_CUBE = 0, SADMC_MOVE_LEARNED = 1 };
+/* The Binning of binning-path walkers (src/mc/binning.rs:48-61). */
+enum sadmc_binning { SADMC_BIN_HISTOGRAM = 0, SADMC_BIN_LINEAR = 1 };
 
 /* Parameters shared by every walker of a set.  Mirrors the reference's
  * EnergyMCParams (energy.rs:81-97) plus the system parameters
@@ -86,7 +88,7 @@ typedef struct {
   double e_lo_bound;     /* bin window: lowest energy a walker may reach */
   double e_hi_bound;     /* bin window: highest energy a walker may reach */
   int32_t move_kind;     /* sadmc_move_kind */
-  int32_t reserved0;
+  int32_t binning;       /* binning-path walkers: sadmc_binning (0 histogram) */
   double reserved[4];
 } sadmc_set_params;
 
@@ -230,8 +232,10 @@ int sadmc_mean_entropy(sadmc_ctx *ctx, int32_t set, double e_a, double e_b, int6
  * src/mc/binning/histogram.rs: the `binning` binary) -----------------------
  * A separate kind of walker set: SAD (un-rounded too_lo/too_hi, num_states
  * by count_states, set_lnw zeroing counts), SAMC, WL and 1/t-WL over a
- * histogram of width p->energy_bin (<= 0: 1.0, BinningParams' default)
- * that grows at both ends inside a fixed window of bins per walker.  Every
+ * histogram (p->binning = SADMC_BIN_HISTOGRAM, binning/histogram.rs) or a
+ * linear interpolation (SADMC_BIN_LINEAR, binning/linear.rs) of width
+ * p->energy_bin (<= 0: 1.0, BinningParams' default) that grows at both ends
+ * inside a fixed window of bins per walker.  Every
  * move accumulates the "energy" extra, SAD the "t_found" extra, WL the
  * "hist" extra, WCA with collect_pressure the "pressure" extra every N^2
  * moves, each with BinCounts' running statistics (histogram.rs:11-33).
@@ -263,7 +267,10 @@ typedef struct {
   uint64_t reserved[3];
 } sadmc_bwalker_state;
 
-/* One logical bin: lnw and the extras' totals and counts. */
+/* One logical bin: lnw and the extras' totals and counts.  With
+ * SADMC_BIN_LINEAR the counts are fractional (linear.rs:24-31): every
+ * *_count field here, and min_count / max_count of sadmc_bcounts, then hold
+ * the bits of an IEEE double. */
 typedef struct {
   double lnw_total;
   uint64_t lnw_count;

@@ -27,6 +27,7 @@ class Params(C.Structure):
         ("refresh", C.c_int32), ("order_g", C.c_int32), ("ising_init", C.c_int32),
         ("collect_pressure", C.c_int32), ("seed", C.c_uint64),
         ("move_kind", C.c_int32), ("drop_hastings", C.c_int32), ("mlp", C.c_void_p),
+        ("binning", C.c_int32),
     ]
 
 
@@ -128,6 +129,8 @@ def lib():
     L.or_bmc_run.argtypes = [vp, C.c_uint64]; L.or_bmc_run.restype = C.c_int
     L.or_bmc_state.argtypes = [vp, vp]
     L.or_bmc_bins.argtypes = [vp] + [vp] * 9
+    L.or_bmc_lnw_at.argtypes = [vp, C.c_double]; L.or_bmc_lnw_at.restype = C.c_double
+    L.or_bmc_count_at.argtypes = [vp, C.c_double]; L.or_bmc_count_at.restype = C.c_double
     L.or_bmc_coords.argtypes = [vp, vp]
     L.or_bmc_spins.argtypes = [vp, vp]
     L.or_bmc_free.argtypes = [vp]
@@ -289,6 +292,12 @@ class BWalker:
             "lnw_total", "lnw_count", "energy_total", "energy_count", "t_found_total", "t_found_count", "hist_count",
             "pressure_total", "pressure_count")])
         return out
+
+    def lnw_at(self, e):
+        return lib().or_bmc_lnw_at(self.h, e)
+
+    def count_at(self, e):
+        return lib().or_bmc_count_at(self.h, e)
 
     def coords(self):
         c = np.zeros(3 * self.p.n)

@@ -1698,8 +1698,10 @@ double or_bench(const or_params *p, int32_t threads, uint64_t moves, uint64_t se
 typedef struct {
   double *total;
   uint64_t *count;
+  double *fc;  /* linear.rs:24-31: fractional counts (linear binning) */
   double min_total, max_total, e_max_total, e_max_count;
   uint64_t min_count, max_count, total_count;
+  double fmin_count, fmax_count;
   int exists;
 } bcounts;
 
@@ -1721,6 +1723,7 @@ struct or_bmc {
   int inv_t, has_min_gamma;
   double min_gamma;
   int status;
+  int linear;  /* binning/linear.rs instead of binning/histogram.rs */
 };
 
 /* f64::max / f64::min as folded from NaN (binning/histogram.rs:366-372):
@@ -1731,7 +1734,10 @@ static inline double fold_min(double r, double x) { return (isnan(r) || x < r) ?
 static void bc_new(bcounts *b, int64_t sz, int64_t cap) {
   b->total = (double *)calloc((size_t)(cap > 0 ? cap : 1), sizeof(double));
   b->count = (uint64_t *)calloc((size_t)(cap > 0 ? cap : 1), sizeof(uint64_t));
+  b->fc = (double *)calloc((size_t)(cap > 0 ? cap : 1), sizeof(double));
   (void)sz;
+  b->fmin_count = 0.0;
+  b->fmax_count = 0.0;
   b->min_total = 0.0;
   b->max_total = 0.0;
   b->e_max_total = -INFINITY;
@@ -1745,6 +1751,7 @@ static void bc_new(bcounts *b, int64_t sz, int64_t cap) {
 static void bc_free(bcounts *b) {
   free(b->total);
   free(b->count);
+  free(b->fc);
 }
 
 static double bc_max_of(const bcounts *b, int64_t len) {
@@ -1772,6 +1779,7 @@ static void bmc_grow(or_bmc *b, int64_t need) {
     if (!all[k]->exists) continue;
     all[k]->total = (double *)realloc(all[k]->total, sizeof(double) * nc);
     all[k]->count = (uint64_t *)realloc(all[k]->count, sizeof(uint64_t) * nc);
+    all[k]->fc = (double *)realloc(all[k]->fc, sizeof(double) * nc);
   }
   b->cap = nc;
 }
@@ -1785,10 +1793,13 @@ static void bmc_insert_zero(or_bmc *b) {
     if (!c->exists) continue;
     memmove(c->total + 1, c->total, sizeof(double) * (size_t)b->len);
     memmove(c->count + 1, c->count, sizeof(uint64_t) * (size_t)b->len);
+    memmove(c->fc + 1, c->fc, sizeof(double) * (size_t)b->len);
     c->total[0] = 0.0;
     c->count[0] = 0;
+    c->fc[0] = 0.0;
     c->min_total = 0.0;
     c->min_count = 0;
+    c->fmin_count = 0.0;
   }
   b->len += 1;
 }
@@ -1800,8 +1811,10 @@ static void bmc_push_zero(or_bmc *b) {
     if (!c->exists) continue;
     c->total[b->len] = 0.0;
     c->count[b->len] = 0;
+    c->fc[b->len] = 0.0;
     c->min_total = 0.0;
     c->min_count = 0;
+    c->fmin_count = 0.0;
   }
   b->len += 1;
 }
@@ -1829,14 +1842,111 @@ static inline uint64_t bmc_get_count(const or_bmc *b, double e) {
   return bc_count_at(&b->lnw, b->len, bmc_index(b, e));
 }
 
-/* histogram.rs:141-158. */
+/* The fractional index of linear.rs:201-203 and its interpretation
+ * (linear.rs:84-98): the bins and weights in the order the reference's
+ * iterator yields them. */
+typedef struct {
+  int n;
+  int64_t i[2];
+  double f[2];
+} lin_idx;
+
+static lin_idx lin_interpret(int64_t len, double fidx) {
+  lin_idx r;
+  r.n = 0;
+  if (fidx < -1.0) return r;
+  if (fidx < 0.0) {
+    r.n = 1;
+    r.i[0] = 0;
+    r.f[0] = 1.0 - (-fidx);
+  } else if (fidx < (double)len - 1.0) {
+    int64_t i = (int64_t)fidx;
+    double o = fidx - (double)i;
+    r.n = 2;
+    r.i[0] = i + 1;
+    r.f[0] = o;
+    r.i[1] = i;
+    r.f[1] = 1.0 - o;
+  } else if (fidx < (double)len) {
+    r.n = 1;
+    r.i[0] = len - 1;
+    r.f[0] = 1.0 - (fidx - ((double)len - 1.0));
+  }
+  return r;
+}
+
+/* BinCounts::get_total / get_count (linear.rs:139-152). */
+static double lin_get(const double *a, int64_t len, double fidx) {
+  lin_idx r = lin_interpret(len, fidx);
+  double t = 0.0;
+  for (int k = 0; k < r.n; k++) t += a[r.i[k]] * r.f[k];
+  return t;
+}
+
+static inline double lin_fidx(const or_bmc *b, double e) { return (e - b->min) / b->width; }
+
+/* get_lnw and get_count (as PerEnergy, count / width) of either binning. */
+static double bm_get_lnw(const or_bmc *b, double e) {
+  if (b->linear) return b->len ? lin_get(b->lnw.total, b->len, lin_fidx(b, e)) : 0.0;
+  return bmc_get_lnw(b, e);
+}
+static double bm_get_count(const or_bmc *b, double e) {
+  if (b->linear) return (b->len ? lin_get(b->lnw.fc, b->len, lin_fidx(b, e)) : 0.0) / b->width;
+  return (double)bmc_get_count(b, e) / b->width;
+}
+
+/* histogram.rs:141-158 / linear.rs:204-222 (the linear binning keeps one
+ * more bin above the energy). */
 static void bmc_prep(or_bmc *b, double e) {
   if (b->len == 0) b->min = floor(e / b->width) * b->width;
   while (e < b->min) {
     bmc_insert_zero(b);
     b->min -= b->width;
   }
-  while (e >= b->min + b->width * (double)b->len) bmc_push_zero(b);
+  if (b->linear) {
+    while (e >= b->min + b->width * ((double)b->len - 1.0)) bmc_push_zero(b);
+  } else {
+    while (e >= b->min + b->width * (double)b->len) bmc_push_zero(b);
+  }
+}
+
+static double fc_min_of(const bcounts *c, int64_t len) {
+  double r = NAN;
+  for (int64_t i = 0; i < len; i++) r = fold_min(r, c->fc[i]);
+  return r;
+}
+
+/* BinCounts::increment_count of linear.rs:99-137 on ln w. */
+static void lin_increment(or_bmc *b, double elo, double ehi, double fidx, double value) {
+  bcounts *c = &b->lnw;
+  int64_t len = b->len;
+  int64_t idx = (int64_t)fidx;
+  double offset = fidx - (double)idx;
+  c->total_count += 1;
+  double oc = c->fc[idx], opc = c->fc[idx + 1];
+  c->fc[idx] += 1.0 - offset;
+  c->fc[idx + 1] += offset;
+  double ot = c->total[idx], opt = c->total[idx + 1];
+  c->total[idx] += value * (1.0 - offset);
+  c->total[idx + 1] += value * offset;
+  if (c->total[idx] > c->max_total && bc_max_of(c, len) == c->total[idx]) {
+    c->max_total = c->total[idx];
+    c->e_max_total = elo;
+  }
+  if (c->total[idx + 1] > c->max_total && bc_max_of(c, len) == c->total[idx + 1]) {
+    c->max_total = c->total[idx + 1];
+    c->e_max_total = ehi;
+  }
+  if (ot == c->min_total || opt == c->min_total) c->min_total = bc_min_of(c, len);
+  if (oc == c->fmin_count || opc == c->fmin_count) c->fmin_count = fc_min_of(c, len);
+  if (c->fc[idx] > c->fmax_count) {
+    c->fmax_count = c->fc[idx];
+    c->e_max_count = elo;
+  }
+  if (c->fc[idx + 1] > c->fmax_count) {
+    c->fmax_count = c->fc[idx + 1];
+    c->e_max_count = ehi;
+  }
 }
 
 /* BinCounts::increment_count (histogram.rs:59-81). */
@@ -1860,7 +1970,10 @@ static void bc_increment(bcounts *c, int64_t len, double e, int64_t idx, double 
 
 /* Bins::accumulate_extra (histogram.rs:219-245); a missing extra is created
  * with the current length (BinCounts::new). */
+static int lin_accumulate(or_bmc *b, bcounts *c, double e, double value);
+
 static int bmc_accumulate(or_bmc *b, bcounts *c, double e, double value) {
+  if (b->linear) return lin_accumulate(b, c, e, value);
   bmc_prep(b, e);
   int64_t idx = bmc_index(b, e);
   if (idx < 0 || idx >= b->len) {  /* the reference's assert (histogram.rs:222) */
@@ -1883,15 +1996,47 @@ static int bmc_accumulate(or_bmc *b, bcounts *c, double e, double value) {
   return 1;
 }
 
+/* Bins::accumulate_extra of linear.rs:267-310. */
+static int lin_accumulate(or_bmc *b, bcounts *c, double e, double value) {
+  bmc_prep(b, e);
+  double fidx = lin_fidx(b, e);
+  int64_t i = (int64_t)fidx;
+  if (!(fidx >= 0.0) || i + 1 >= b->len) {  /* an assert / index panic in the reference */
+    b->status = ST_INDEX;
+    return 0;
+  }
+  if (!c->exists) {
+    bc_new(c, b->len, b->cap);
+  }
+  double off = fidx - (double)i;
+  c->total_count += 1;
+  double oc = c->fc[i], opc = c->fc[i + 1];
+  c->fc[i] += 1.0 - off;
+  c->fc[i + 1] += off;
+  if (oc == c->fmin_count || opc == c->fmin_count) c->fmin_count = fc_min_of(c, b->len);
+  if (c->fc[i] > c->fmax_count) c->fmax_count = c->fc[i];
+  if (c->fc[i + 1] > c->fmax_count) c->fmax_count = c->fc[i + 1];
+  double ot = c->total[i], opt = c->total[i + 1];
+  c->total[i] = ot + value * (1.0 - off);
+  c->total[i + 1] = opt + value * off;
+  if (c->total[i] > c->max_total) c->max_total = c->total[i];
+  if (c->total[i + 1] > c->max_total) c->max_total = c->total[i + 1];
+  if (ot == c->min_total || opt == c->min_total) c->min_total = bc_min_of(c, b->len);
+  return 1;
+}
+
 /* Bins::zero_out_extra (histogram.rs:246-259). */
 static void bc_zero_out(bcounts *c, int64_t len) {
   if (!c->exists) return;
   memset(c->count, 0, sizeof(uint64_t) * (size_t)len);
   for (int64_t i = 0; i < len; i++) c->total[i] = 0.0;
+  for (int64_t i = 0; i < len; i++) c->fc[i] = 0.0;
   c->min_total = 0.0;
   c->max_total = -INFINITY;
   c->min_count = 0;
   c->max_count = 0;
+  c->fmin_count = 0.0;
+  c->fmax_count = 0.0;
   c->total_count = 0;
 }
 
@@ -1900,6 +2045,17 @@ static int bmc_increment(or_bmc *b, double e, double gamma) {
   if (e > b->max_e) b->max_e = e;
   if (e < b->min_e) b->min_e = e;
   bmc_prep(b, e);
+  if (b->linear) {  /* linear.rs:237-253 */
+    double fidx = lin_fidx(b, e);
+    int64_t ii = (int64_t)fidx;
+    if (!(fidx >= 0.0) || ii + 1 >= b->len) {
+      b->status = ST_INDEX;
+      return 0;
+    }
+    double rg = gamma * 1.0 / b->width;
+    lin_increment(b, bmc_energy(b, ii), bmc_energy(b, ii + 1), fidx, rg);
+    return 1;
+  }
   int64_t idx = bmc_index(b, e);
   if (idx < 0 || idx >= b->len) {  /* an index panic in the reference */
     b->status = ST_INDEX;
@@ -1937,14 +2093,14 @@ static double bmc_gamma(const or_bmc *b) {
 
 /* energy_binning.rs:276-319. */
 static int bmc_reject(or_bmc *b, double e1, double e2) {
-  double lnw1 = bmc_get_lnw(b, e1), lnw2 = bmc_get_lnw(b, e2);
+  double lnw1 = bm_get_lnw(b, e1), lnw2 = bm_get_lnw(b, e2);
   if (b->method == OR_SAD) {
-    if (e1 < b->too_lo) lnw1 = bmc_get_lnw(b, b->too_lo) + (e1 - b->too_lo) / b->min_T;
-    else if (e1 > b->too_hi) lnw1 = bmc_get_lnw(b, b->too_hi);
-    if (e2 < b->too_lo) lnw2 = bmc_get_lnw(b, b->too_lo) + (e2 - b->too_lo) / b->min_T;
-    else if (e2 > b->too_hi) lnw2 = bmc_get_lnw(b, b->too_hi);
+    if (e1 < b->too_lo) lnw1 = bm_get_lnw(b, b->too_lo) + (e1 - b->too_lo) / b->min_T;
+    else if (e1 > b->too_hi) lnw1 = bm_get_lnw(b, b->too_hi);
+    if (e2 < b->too_lo) lnw2 = bm_get_lnw(b, b->too_lo) + (e2 - b->too_lo) / b->min_T;
+    else if (e2 > b->too_hi) lnw2 = bm_get_lnw(b, b->too_hi);
     int rejected = lnw2 > lnw1 && or_gen_f64(b->m->rng) > exp(lnw1 - lnw2);
-    if (!rejected && bmc_get_count(b, e2) == 0 && e2 < b->too_hi && e2 > b->too_lo) b->tL = b->moves;
+    if (!rejected && bm_get_count(b, e2) == 0.0 && e2 < b->too_hi && e2 > b->too_lo) b->tL = b->moves;
     return rejected;
   }
   return lnw2 > lnw1 && or_gen_f64(b->m->rng) > exp(lnw1 - lnw2);
@@ -1953,20 +2109,21 @@ static int bmc_reject(or_bmc *b, double e1, double e2) {
 /* energy_binning.rs:320-500. */
 static void bmc_update(or_bmc *b, double energy) {
   double gamma = bmc_gamma(b);
-  uint64_t old_highest = b->lnw.max_count;
-  uint64_t old_here = bmc_get_count(b, energy);
+  double old_highest = (b->linear ? b->lnw.fmax_count : (double)b->lnw.max_count) / b->width;
+  double old_here = bm_get_count(b, energy);
   if (!bmc_increment(b, energy, gamma)) return;
   if (b->method == OR_SAD) {
-    uint64_t here = bmc_get_count(b, energy);
-    if (old_here == 0 && !bmc_accumulate(b, &b->tf, energy, (double)b->moves)) return;
+    double here = bm_get_count(b, energy);
+    if (old_here == 0.0 && !bmc_accumulate(b, &b->tf, energy, (double)b->moves)) return;
     if (here > old_highest) {
       if (energy > b->too_hi) {
-        double v = bmc_get_lnw(b, b->too_hi);
-        for (int64_t i = 0; i < b->len; i++) {  /* set_lnw, histogram.rs:182-190 */
+        double v = bm_get_lnw(b, b->too_hi);
+        for (int64_t i = 0; i < b->len; i++) {  /* set_lnw, histogram.rs:182-190, linear.rs:254-262 */
           double e = bmc_energy(b, i);
-          if (e > b->too_hi && bmc_get_count(b, e) > 0) {
+          if (e > b->too_hi && bm_get_count(b, e) > 0.0) {
             b->lnw.total[i] = v;
             b->lnw.count[i] = 0;
+            b->lnw.fc[i] = 0.0;
           }
         }
         b->latest = (energy - b->too_lo) / b->min_T;
@@ -1974,13 +2131,14 @@ static void bmc_update(or_bmc *b, double energy) {
         b->too_hi = energy;
         b->num_states = bmc_count_states(b);
       } else if (energy < b->too_lo) {
-        double v = bmc_get_lnw(b, b->too_lo);
+        double v = bm_get_lnw(b, b->too_lo);
         double lo = b->too_lo;
         for (int64_t i = 0; i < b->len; i++) {
           double e = bmc_energy(b, i);
-          if (e < lo && bmc_get_count(b, e) > 0) {
+          if (e < lo && bm_get_count(b, e) > 0.0) {
             b->lnw.total[i] = v + (e - lo) / b->min_T;
             b->lnw.count[i] = 0;
+            b->lnw.fc[i] = 0.0;
           }
         }
         b->latest = (b->too_hi - energy) / b->min_T;
@@ -2000,12 +2158,12 @@ static void bmc_update(or_bmc *b, double energy) {
     }
   } else if (b->method == OR_WL) {
     double w = b->width;
-    double old_lowest = b->hist.exists ? (double)b->hist.min_count / w : 0.0;
+    double old_lowest = b->hist.exists ? (b->linear ? b->hist.fmin_count : (double)b->hist.min_count) / w : 0.0;
     if (!bmc_accumulate(b, &b->hist, energy, 0.0)) return;
     if (b->has_min_gamma && b->gamma < b->min_gamma) return;
-    double lowest = (double)b->hist.min_count / w;
-    int ok_lo = !b->bp.has_min_e || bmc_get_count(b, b->bp.min_e) > 0;
-    int ok_hi = !b->bp.has_max_e || bmc_get_count(b, b->bp.max_e) > 0;
+    double lowest = (b->linear ? b->hist.fmin_count : (double)b->hist.min_count) / w;
+    int ok_lo = !b->bp.has_min_e || bm_get_count(b, b->bp.min_e) > 0.0;
+    int ok_hi = !b->bp.has_max_e || bm_get_count(b, b->bp.max_e) > 0.0;
     if (lowest > old_lowest && ok_lo && ok_hi) {
       double mean = (double)b->hist.total_count / (w * (double)b->len);
       if ((b->inv_t && lowest > 0.0) || lowest >= 0.8 * mean) {
@@ -2078,7 +2236,8 @@ static or_bmc *bmc_from_system(const or_params *bp, or_mc *m) {
   b->has_min_gamma = bp->method == OR_WL && bp->has_min_gamma;
   b->min_gamma = bp->method_param;
   b->width = bp->energy_bin > 0 ? bp->energy_bin : 1.0;
-  b->min = (round(E / b->width) - 0.5) * b->width;  /* Bins::new, histogram.rs:162-170 */
+  b->linear = bp->binning == 1;
+  b->min = (round(E / b->width) - 0.5) * b->width;  /* Bins::new, histogram.rs:162-170, linear.rs:226-236 */
   b->min_e = E;
   b->max_e = E;
   b->len = 0;
@@ -2115,13 +2274,13 @@ int or_bmc_run(or_bmc *b, uint64_t moves) {
   return b->status;
 }
 
-static void bc_export(const bcounts *c, or_bcounts *o) {
+static void bc_export(const bcounts *c, or_bcounts *o, int linear) {
   o->min_total = c->min_total;
   o->max_total = c->max_total;
   o->e_max_total = c->e_max_total;
   o->e_max_count = c->e_max_count;
-  o->min_count = c->min_count;
-  o->max_count = c->max_count;
+  o->min_count = linear ? f64_bits(c->fmin_count) : c->min_count;
+  o->max_count = linear ? f64_bits(c->fmax_count) : c->max_count;
   o->total_count = c->total_count;
   o->exists = c->exists;
 }
@@ -2152,11 +2311,11 @@ void or_bmc_state(const or_bmc *b, or_bstate *o) {
   o->latest_parameter = b->latest;
   o->t0 = b->t0;
   o->gamma = b->gamma;
-  bc_export(&b->lnw, &o->lnw);
-  bc_export(&b->en, &o->energy);
-  bc_export(&b->tf, &o->t_found);
-  bc_export(&b->hist, &o->hist);
-  bc_export(&b->pr, &o->pressure);
+  bc_export(&b->lnw, &o->lnw, b->linear);
+  bc_export(&b->en, &o->energy, b->linear);
+  bc_export(&b->tf, &o->t_found, b->linear);
+  bc_export(&b->hist, &o->hist, b->linear);
+  bc_export(&b->pr, &o->pressure, b->linear);
 }
 
 /* Logical bins [0, len): totals and counts of lnw and the extras (zeros for
@@ -2171,10 +2330,14 @@ void or_bmc_bins(const or_bmc *b, double *lnw_total, uint64_t *lnw_count, double
   for (int k = 0; k < 5; k++) {
     for (int64_t i = 0; i < n; i++) {
       if (tt[k]) tt[k][i] = cs[k]->exists ? cs[k]->total[i] : 0.0;
-      if (cc[k]) cc[k][i] = cs[k]->exists ? cs[k]->count[i] : 0;
+      if (cc[k]) cc[k][i] = cs[k]->exists ? (b->linear ? f64_bits(cs[k]->fc[i]) : cs[k]->count[i]) : 0;
     }
   }
 }
+
+/* get_lnw and get_count (PerEnergy) at any energy, as the walker sees them. */
+double or_bmc_lnw_at(const or_bmc *b, double e) { return bm_get_lnw(b, e); }
+double or_bmc_count_at(const or_bmc *b, double e) { return bm_get_count(b, e); }
 
 void or_bmc_coords(const or_bmc *b, double *c) { or_mc_coords(b->m, c); }
 void or_bmc_spins(const or_bmc *b, int8_t *s) { or_mc_spins(b->m, s); }

@@ -79,6 +79,7 @@ typedef struct {
   int32_t move_kind;        /* 0 cube move, 1 learned Gaussian move (LJ) */
   int32_t drop_hastings;    /* test hook: omit ln H (a planted mistake) */
   const float *mlp;         /* learned move: OR_PROP_NP weights (BF16 values as float) */
+  int32_t binning;          /* binning walker: 0 histogram (histogram.rs), 1 linear (linear.rs) */
 } or_params;
 
 typedef struct {
@@ -217,7 +218,9 @@ void or_pt_free(or_pt *pt);
  * from count_states, set_lnw zeroing counts), SAMC, WL and 1/t-WL over a
  * histogram of width energy_bin (<= 0: 1.0) that grows at both ends, with
  * the "energy" extra every move, the "t_found" extra (SAD), the "hist" extra
- * (WL) and the WCA "pressure" extra every N^2 moves.  The method comes from
+ * (WL) and the WCA "pressure" extra every N^2 moves.  p->binning = 1
+ * selects the linear interpolation of binning/linear.rs (fractional counts,
+ * exported as the bits of doubles in the count fields).  The method comes from
  * p->method (OR_CANONICAL is not a method of this walker); p->max_e /
  * p->min_e are max/min_allowed_energy; the generator is seed_from_u64(seed)
  * and a max_allowed_energy triggers the greedy descent of from_params. */
@@ -246,6 +249,8 @@ void or_bmc_state(const or_bmc *b, or_bstate *out);
 void or_bmc_bins(const or_bmc *b, double *lnw_total, uint64_t *lnw_count, double *energy_total,
                  uint64_t *energy_count, double *tf_total, uint64_t *tf_count, uint64_t *hist_count,
                  double *p_total, uint64_t *p_count);
+double or_bmc_lnw_at(const or_bmc *b, double e);
+double or_bmc_count_at(const or_bmc *b, double e);
 void or_bmc_coords(const or_bmc *b, double *coords);
 void or_bmc_spins(const or_bmc *b, int8_t *spins);
 void or_bmc_free(or_bmc *b);

@@ -11,6 +11,8 @@
 // count_states, the trackers' own rescans), so that every result is the
 // sequential one.
 #define SADMC_KERNEL_TU
+#include <type_traits>
+
 #include "sadmc_kernels.h"
 
 namespace sadmc {
@@ -19,12 +21,34 @@ namespace {
 
 enum { BC_LNW = 0, BC_EN = 1, BC_TF = 2, BC_HIST = 3, BC_PR = 4 };
 
-// BinCounts' running statistics (histogram.rs:11-33).
+// BinCounts' running statistics (histogram.rs:11-33, linear.rs:11-33):
+// counts are integers in the histogram, fractions in the linear binning.
+template <typename CT>
 struct BC {
   double min_total, max_total, e_max_total, e_max_count;
-  uint64_t min_count, max_count, total_count;
+  CT min_count, max_count;
+  uint64_t total_count;
   int32_t exists;
 };
+
+// Counts live in the 64-bit count fields of the bin records: as integers,
+// or as the bits of doubles.
+template <typename CT>
+__device__ __forceinline__ uint64_t ct_bits(CT v) {
+  if constexpr (std::is_floating_point<CT>::value) {
+    return __builtin_bit_cast(uint64_t, v);
+  } else {
+    return (uint64_t)v;
+  }
+}
+template <typename CT>
+__device__ __forceinline__ CT ct_from(uint64_t u) {
+  if constexpr (std::is_floating_point<CT>::value) {
+    return __builtin_bit_cast(double, u);
+  } else {
+    return (CT)u;
+  }
+}
 
 __device__ __forceinline__ double shfl_f64(double v, int d) {
   return __builtin_bit_cast(double, (uint64_t)__shfl_xor((long long)__builtin_bit_cast(uint64_t, v), d, 64));
@@ -32,8 +56,11 @@ __device__ __forceinline__ double shfl_f64(double v, int d) {
 
 }  // namespace
 
-template <int SYS, int KMAX>
+// LIN: the linear interpolation of binning/linear.rs instead of the
+// histogram of binning/histogram.rs.
+template <int SYS, int KMAX, bool LIN>
 struct BinWalker {
+  using CT = typename std::conditional<LIN, double, uint64_t>::type;
   Engine<SYS, KMAX, 0, M_CAN> e;
   const BinArgs *Bp;
   sadmc_bbin *B;  // this walker's window of bins
@@ -47,7 +74,7 @@ struct BinWalker {
   double min_T, too_lo, too_hi, tF, latest, t0, gamma;
   int32_t inv_t, has_min_gamma;
   double min_gamma;
-  BC bc[5];
+  BC<CT> bc[5];
   // Exact extremes of each array kept beside the reference's statistics,
   // so that its min_of / max_of / min() rescans (histogram.rs:64-80,
   // 226-244) become lookups: the smallest total and how many bins hold it,
@@ -55,28 +82,33 @@ struct BinWalker {
   // A full rescan is needed only when the last bin holding an extreme
   // leaves it, after set_lnw / zero_out, and at the start of a launch.
   double tmin[5], lmax;
-  uint64_t nmin[5], cmin[5], ncmin[5];
+  uint64_t nmin[5], ncmin[5];
+  CT cmin[5];
+  // an extreme lost its last bin: rescan once the move's stores are all in
+  // (a rescan between the two bins of a linear update would count the
+  // second bin's new value twice)
+  bool stale_t[5], stale_c[5];
 
   SADMC_DF const BinArgs &args() const { return *Bp; }
 
   // ------------------------------------------------------------ state I/O
-  SADMC_DF static void bc_load(BC &c, const sadmc_bcounts &s) {
+  SADMC_DF static void bc_load(BC<CT> &c, const sadmc_bcounts &s) {
     c.min_total = s.min_total;
     c.max_total = s.max_total;
     c.e_max_total = s.e_max_total;
     c.e_max_count = s.e_max_count;
-    c.min_count = s.min_count;
-    c.max_count = s.max_count;
+    c.min_count = ct_from<CT>(s.min_count);
+    c.max_count = ct_from<CT>(s.max_count);
     c.total_count = s.total_count;
     c.exists = s.exists;
   }
-  SADMC_DF static void bc_store(const BC &c, sadmc_bcounts &s) {
+  SADMC_DF static void bc_store(const BC<CT> &c, sadmc_bcounts &s) {
     s.min_total = c.min_total;
     s.max_total = c.max_total;
     s.e_max_total = c.e_max_total;
     s.e_max_count = c.e_max_count;
-    s.min_count = c.min_count;
-    s.max_count = c.max_count;
+    s.min_count = ct_bits<CT>(c.min_count);
+    s.max_count = ct_bits<CT>(c.max_count);
     s.total_count = c.total_count;
     s.exists = c.exists;
     s.reserved = 0;
@@ -131,8 +163,11 @@ struct BinWalker {
     bc_load(bc[BC_PR], w.pressure);
     B = ba.bins + (int64_t)walker * ba.cap;
 #pragma unroll
-    for (int k = 0; k < 5; k++)
+    for (int k = 0; k < 5; k++) {
+      stale_t[k] = false;
+      stale_c[k] = false;
       if (bc[k].exists) rescan_all(k);
+    }
   }
 
   SADMC_DF void store(int walker) {
@@ -193,9 +228,11 @@ struct BinWalker {
            : k == BC_HIST ? b.hist_count
                           : b.pressure_count;
   }
+  SADMC_DF static CT cget(sadmc_bbin &b, int k) { return ct_from<CT>(cnt(b, k)); }
+  SADMC_DF static void cset(sadmc_bbin &b, int k, CT v) { cnt(b, k) = ct_bits<CT>(v); }
   // the "hist" extra only ever adds 0.0 to totals that start at 0.0
   SADMC_DF double total_at(int k, int64_t i) const { return k == BC_HIST ? 0.0 : tot(B[base + i], k); }
-  SADMC_DF uint64_t count_at(int k, int64_t i) const { return cnt(B[base + i], k); }
+  SADMC_DF CT count_at(int k, int64_t i) const { return cget(B[base + i], k); }
 
   // histogram.rs:129-140; -1 for the out-of-range cases
   SADMC_DF int64_t index(double en) const {
@@ -205,14 +242,38 @@ struct BinWalker {
     if (!(i < 9.2e18)) return INT64_MAX;
     return (int64_t)i;
   }
+  SADMC_DF double fidx(double en) const { return (en - min) / width; }  // linear.rs:201-203
   SADMC_DF double energy_of(int64_t i) const { return min + ((double)i + 0.5) * width; }
+  // BinCounts::get_total / get_count of linear.rs:139-152 over the
+  // interpretation of linear.rs:84-98, in the iterator's order
+  SADMC_DF double lin_interp(bool counts, double f) const {
+    double t = 0.0;
+    if (len == 0 || f < -1.0) return t;
+    auto val = [&](int64_t i) -> double {
+      return counts ? (double)count_at(BC_LNW, i) : B[base + i].lnw_total;
+    };
+    if (f < 0.0) {
+      t += val(0) * (1.0 - (-f));
+    } else if (f < (double)len - 1.0) {
+      const int64_t i = (int64_t)f;
+      const double o = f - (double)i;
+      t += val(i + 1) * o;
+      t += val(i) * (1.0 - o);
+    } else if (f < (double)len) {
+      t += val(len - 1) * (1.0 - (f - ((double)len - 1.0)));
+    }
+    return t;
+  }
   SADMC_DF double get_lnw(double en) const {
+    if (LIN) return lin_interp(false, fidx(en));
     const int64_t i = index(en);
     return (i >= 0 && i < len) ? B[base + i].lnw_total : 0.0;
   }
-  SADMC_DF uint64_t get_count(double en) const {
+  // get_count as PerEnergy: count / width
+  SADMC_DF double get_count(double en) const {
+    if (LIN) return lin_interp(true, fidx(en)) / width;
     const int64_t i = index(en);
-    return (i >= 0 && i < len) ? B[base + i].lnw_count : 0;
+    return (double)((i >= 0 && i < len) ? (uint64_t)count_at(BC_LNW, i) : 0) / width;
   }
 
   // ---------------------------------------------------- group reductions
@@ -262,13 +323,20 @@ struct BinWalker {
     nmin[k] = c;
   }
   SADMC_DF void rescan_count(int k) {
-    uint64_t m = ~0ULL, c = 0;
+    CT big;
+    if constexpr (LIN) {
+      big = __builtin_inf();
+    } else {
+      big = ~0ULL;
+    }
+    CT m = big;
+    uint64_t c = 0;
     for (int64_t j0 = lane; j0 < len; j0 += 4 * G) {
-      uint64_t v[4];
+      CT v[4];
 #pragma unroll
       for (int u = 0; u < 4; u++) {
         const int64_t j = j0 + u * G;
-        v[u] = j < len ? count_at(k, j) : ~0ULL;
+        v[u] = j < len ? count_at(k, j) : big;
       }
 #pragma unroll
       for (int u = 0; u < 4; u++) {
@@ -282,7 +350,7 @@ struct BinWalker {
       }
     }
     for (int d = 1; d < G; d <<= 1) {
-      const uint64_t om = (uint64_t)__shfl_xor((long long)m, d, 64);
+      const CT om = ct_from<CT>((uint64_t)__shfl_xor((long long)ct_bits<CT>(m), d, 64));
       const uint64_t oc = (uint64_t)__shfl_xor((long long)c, d, 64);
       if (om < m) {
         m = om;
@@ -317,25 +385,38 @@ struct BinWalker {
     rescan_count(k);
     if (k == BC_LNW) rescan_lmax();
   }
-  // One bin of array k goes from v to w (totals) / from a to b (counts).
+  // One bin of array k goes from v to w (totals) / from a to b (counts);
+  // settle() afterwards, when every bin of the update is stored.
   SADMC_DF void track_total(int k, double v, double w) {
+    if (stale_t[k]) return;
     if (w < tmin[k]) {
       tmin[k] = w;
       nmin[k] = 1;
     } else if (w == tmin[k]) {
       if (v != w) nmin[k] += 1;
     } else if (v == tmin[k]) {
-      if (--nmin[k] == 0) rescan_total(k);
+      if (--nmin[k] == 0) stale_t[k] = true;
     }
   }
-  SADMC_DF void track_count(int k, uint64_t a, uint64_t b) {
+  SADMC_DF void track_count(int k, CT a, CT b) {
+    if (stale_c[k]) return;
     if (b < cmin[k]) {
       cmin[k] = b;
       ncmin[k] = 1;
     } else if (b == cmin[k]) {
       if (a != b) ncmin[k] += 1;
     } else if (a == cmin[k]) {
-      if (--ncmin[k] == 0) rescan_count(k);
+      if (--ncmin[k] == 0) stale_c[k] = true;
+    }
+  }
+  SADMC_DF void settle(int k) {
+    if (stale_t[k]) {
+      rescan_total(k);
+      stale_t[k] = false;
+    }
+    if (stale_c[k]) {
+      rescan_count(k);
+      stale_c[k] = false;
     }
   }
   // a zero bin joins array k
@@ -346,10 +427,10 @@ struct BinWalker {
     } else if (0.0 == tmin[k]) {
       nmin[k] += 1;
     }
-    if (0 < cmin[k]) {
+    if ((CT)0 < cmin[k]) {
       cmin[k] = 0;
       ncmin[k] = 1;
-    } else {
+    } else if ((CT)0 == cmin[k]) {
       ncmin[k] += 1;
     }
     if (k == BC_LNW && !(lmax >= 0.0)) lmax = 0.0;
@@ -383,7 +464,8 @@ struct BinWalker {
       }
     len += 1;
   }
-  // histogram.rs:141-158 (insert_zero / push_zero on lnw and every extra)
+  // histogram.rs:141-158 / linear.rs:204-222 (insert_zero / push_zero on
+  // lnw and every extra; the linear binning keeps one more bin above)
   SADMC_DF bool prep(double en) {
     if (len == 0) {
       min = floor(en / width) * width;
@@ -400,7 +482,7 @@ struct BinWalker {
       grown();
       min -= width;
     }
-    while (en >= min + width * (double)len) {
+    while (LIN ? en >= min + width * ((double)len - 1.0) : en >= min + width * (double)len) {
       if (base + len >= cap) {
         status = SADMC_WS_CAPACITY;
         return false;
@@ -413,18 +495,19 @@ struct BinWalker {
 
   // BinCounts::increment_count (histogram.rs:59-81) on lnw
   SADMC_DF void lnw_increment(double en, int64_t idx, double value) {
-    BC &c = bc[BC_LNW];
+    BC<CT> &c = bc[BC_LNW];
     sadmc_bbin &r = B[base + idx];
     c.total_count += 1;
-    const uint64_t old_count = r.lnw_count;
-    const uint64_t new_count = old_count + 1;
+    const CT old_count = cget(r, BC_LNW);
+    const CT new_count = old_count + 1;
     const double old_total = r.lnw_total;
     const double new_total = old_total + value;
-    r.lnw_count = new_count;
+    cset(r, BC_LNW, new_count);
     r.lnw_total = new_total;
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     track_total(BC_LNW, old_total, new_total);
     track_count(BC_LNW, old_count, new_count);
+    settle(BC_LNW);
     if (new_total > lmax) lmax = new_total;
     // max_of(total) == total[idx], min_of(total) and min(count) after the
     // update are the tracked extremes
@@ -441,15 +524,72 @@ struct BinWalker {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   }
 
-  // Bins::accumulate_extra (histogram.rs:219-245)
+  // BinCounts::increment_count of linear.rs:99-137 on lnw: the weight
+  // splits over bins idx and idx + 1
+  SADMC_DF void lnw_increment_lin(double elo, double ehi, double f, double value) {
+    BC<CT> &c = bc[BC_LNW];
+    const int64_t idx = (int64_t)f;
+    const double offset = f - (double)idx;
+    sadmc_bbin &r0 = B[base + idx];
+    sadmc_bbin &r1 = B[base + idx + 1];
+    c.total_count += 1;
+    const CT oc = cget(r0, BC_LNW), opc = cget(r1, BC_LNW);
+    const CT c0 = oc + (1.0 - offset), c1 = opc + offset;
+    const double ot = r0.lnw_total, opt = r1.lnw_total;
+    const double t0v = ot + value * (1.0 - offset), t1v = opt + value * offset;
+    cset(r0, BC_LNW, c0);
+    cset(r1, BC_LNW, c1);
+    r0.lnw_total = t0v;
+    r1.lnw_total = t1v;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    track_total(BC_LNW, ot, t0v);
+    track_total(BC_LNW, opt, t1v);
+    track_count(BC_LNW, oc, c0);
+    track_count(BC_LNW, opc, c1);
+    settle(BC_LNW);
+    if (t0v > lmax) lmax = t0v;
+    if (t1v > lmax) lmax = t1v;
+    if (t0v > c.max_total && lmax == t0v) {
+      c.max_total = t0v;
+      c.e_max_total = elo;
+    }
+    if (t1v > c.max_total && lmax == t1v) {
+      c.max_total = t1v;
+      c.e_max_total = ehi;
+    }
+    if (ot == c.min_total || opt == c.min_total) c.min_total = tmin[BC_LNW];
+    if (oc == c.min_count || opc == c.min_count) c.min_count = cmin[BC_LNW];
+    if (c0 > c.max_count) {
+      c.max_count = c0;
+      c.e_max_count = elo;
+    }
+    if (c1 > c.max_count) {
+      c.max_count = c1;
+      c.e_max_count = ehi;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  }
+
+  // Bins::accumulate_extra (histogram.rs:219-245, linear.rs:267-310)
   SADMC_DF bool accumulate(int k, double en, double value) {
     if (!prep(en)) return false;
-    const int64_t idx = index(en);
-    if (idx < 0 || idx >= len) {  // the reference's assert (histogram.rs:222)
-      status = SADMC_WS_INDEX;
-      return false;
+    int64_t idx;
+    double f = 0.0;
+    if (LIN) {
+      f = fidx(en);
+      idx = (int64_t)f;
+      if (!(f >= 0.0) || idx + 1 >= len) {  // the reference's assert / index panic
+        status = SADMC_WS_INDEX;
+        return false;
+      }
+    } else {
+      idx = index(en);
+      if (idx < 0 || idx >= len) {  // the reference's assert (histogram.rs:222)
+        status = SADMC_WS_INDEX;
+        return false;
+      }
     }
-    BC &c = bc[k];
+    BC<CT> &c = bc[k];
     if (!c.exists) {  // BinCounts::new over the current bins
       tmin[k] = 0.0;
       nmin[k] = (uint64_t)len;
@@ -464,30 +604,61 @@ struct BinWalker {
       c.max_count = 0;
       c.total_count = 0;
     }
-    sadmc_bbin &r = B[base + idx];
     c.total_count += 1;
-    const uint64_t old_count = cnt(r, k);
-    const uint64_t new_count = old_count + 1;
-    const double old_total = k == BC_HIST ? 0.0 : tot(r, k);
-    const double new_total = old_total + value;
-    cnt(r, k) = new_count;
-    if (k != BC_HIST) tot(r, k) = new_total;
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    track_count(k, old_count, new_count);
-    if (k != BC_HIST) track_total(k, old_total, new_total);
-    if (new_count == c.min_count + 1) c.min_count = cmin[k];
-    if (new_count > c.max_count) c.max_count = new_count;
-    if (new_total > c.max_total) c.max_total = new_total;
-    if (old_total == c.min_total) c.min_total = k == BC_HIST ? 0.0 : tmin[k];
+    if constexpr (!LIN) {
+      sadmc_bbin &r = B[base + idx];
+      const CT old_count = cget(r, k);
+      const CT new_count = old_count + 1;
+      const double old_total = k == BC_HIST ? 0.0 : tot(r, k);
+      const double new_total = old_total + value;
+      cset(r, k, new_count);
+      if (k != BC_HIST) tot(r, k) = new_total;
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      track_count(k, old_count, new_count);
+      if (k != BC_HIST) track_total(k, old_total, new_total);
+      settle(k);
+      if (new_count == c.min_count + 1) c.min_count = cmin[k];
+      if (new_count > c.max_count) c.max_count = new_count;
+      if (new_total > c.max_total) c.max_total = new_total;
+      if (old_total == c.min_total) c.min_total = k == BC_HIST ? 0.0 : tmin[k];
+    } else {
+      const double off = f - (double)idx;
+      sadmc_bbin &r0 = B[base + idx];
+      sadmc_bbin &r1 = B[base + idx + 1];
+      const CT oc = cget(r0, k), opc = cget(r1, k);
+      const CT c0 = oc + (1.0 - off), c1 = opc + off;
+      const double ot = k == BC_HIST ? 0.0 : tot(r0, k), opt = k == BC_HIST ? 0.0 : tot(r1, k);
+      const double t0v = ot + value * (1.0 - off), t1v = opt + value * off;
+      cset(r0, k, c0);
+      cset(r1, k, c1);
+      if (k != BC_HIST) {
+        tot(r0, k) = t0v;
+        tot(r1, k) = t1v;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      track_count(k, oc, c0);
+      track_count(k, opc, c1);
+      if (k != BC_HIST) {
+        track_total(k, ot, t0v);
+        track_total(k, opt, t1v);
+      }
+      settle(k);
+      if (oc == c.min_count || opc == c.min_count) c.min_count = cmin[k];
+      if (c0 > c.max_count) c.max_count = c0;
+      if (c1 > c.max_count) c.max_count = c1;
+      if (t0v > c.max_total) c.max_total = t0v;
+      if (t1v > c.max_total) c.max_total = t1v;
+      if (ot == c.min_total || opt == c.min_total) c.min_total = k == BC_HIST ? 0.0 : tmin[k];
+    }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     return true;
   }
 
   // Bins::zero_out_extra (histogram.rs:246-259) on "hist"
   SADMC_DF void zero_out_hist() {
-    BC &c = bc[BC_HIST];
+    BC<CT> &c = bc[BC_HIST];
     if (!c.exists) return;
-    for (int64_t j = lane; j < len; j += G) B[base + j].hist_count = 0;
+    for (int64_t j = lane; j < len; j += G) cset(B[base + j], BC_HIST, (CT)0);
     c.min_total = 0.0;
     c.max_total = -__builtin_inf();
     c.min_count = 0;
@@ -537,23 +708,26 @@ struct BinWalker {
       else if (e2 > too_hi)
         lnw2 = get_lnw(too_hi);
       const bool rejected = lnw2 > lnw1 && gen_f64(e.s0, e.s1) > exp(lnw1 - lnw2);
-      if (!rejected && get_count(e2) == 0 && e2 < too_hi && e2 > too_lo) tL = moves;
+      if (!rejected && get_count(e2) == 0.0 && e2 < too_hi && e2 > too_lo) tL = moves;
       return rejected;
     }
     return lnw2 > lnw1 && gen_f64(e.s0, e.s1) > exp(lnw1 - lnw2);
   }
 
-  // set_lnw (histogram.rs:182-190) for SAD's two range extensions: bins
+  // set_lnw (histogram.rs:182-190, linear.rs:254-262) for SAD's two range
+  // extensions (in the linear binning a bin's count is read at its centre,
+  // from it and the next bin, before that next bin may be cleared: each
+  // step's loads come before its stores); bins
   // above `edge` (up) or below it (down) with a nonzero count take `v` (up)
   // or v + (e - edge)/min_T (down) and lose their count.
   SADMC_DF void set_lnw_sad(bool up, double edge, double v) {
     for (int64_t j = lane; j < len; j += G) {
       const double en = energy_of(j);
       const bool side = up ? en > edge : en < edge;
-      if (side && get_count(en) > 0) {
+      if (side && get_count(en) > 0.0) {
         sadmc_bbin &r = B[base + j];
         r.lnw_total = up ? v : v + (en - edge) / min_T;
-        r.lnw_count = 0;
+        cset(r, BC_LNW, (CT)0);
       }
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -563,20 +737,30 @@ struct BinWalker {
   // energy_binning.rs:320-500
   SADMC_DF bool update(double en) {
     const double g = gamma_now();
-    const uint64_t old_highest = bc[BC_LNW].max_count;
-    const uint64_t old_here = get_count(en);
+    const double old_highest = (double)bc[BC_LNW].max_count / width;
+    const double old_here = get_count(en);
     if (en > max_e) max_e = en;
     if (en < min_e) min_e = en;
     if (!prep(en)) return false;
-    const int64_t idx = index(en);
-    if (idx < 0 || idx >= len) {  // an index panic in the reference
-      status = SADMC_WS_INDEX;
-      return false;
+    if constexpr (LIN) {  // linear.rs:237-253
+      const double f = fidx(en);
+      const int64_t ii = (int64_t)f;
+      if (!(f >= 0.0) || ii + 1 >= len) {  // an index panic in the reference
+        status = SADMC_WS_INDEX;
+        return false;
+      }
+      lnw_increment_lin(energy_of(ii), energy_of(ii + 1), f, g * 1.0 / width);
+    } else {
+      const int64_t idx = index(en);
+      if (idx < 0 || idx >= len) {  // an index panic in the reference
+        status = SADMC_WS_INDEX;
+        return false;
+      }
+      lnw_increment(en, idx, g);
     }
-    lnw_increment(en, idx, g);
     if (method == SADMC_SAD) {
-      const uint64_t here = get_count(en);
-      if (old_here == 0 && !accumulate(BC_TF, en, (double)moves)) return false;
+      const double here = get_count(en);
+      if (old_here == 0.0 && !accumulate(BC_TF, en, (double)moves)) return false;
       if (here > old_highest) {
         if (en > too_hi) {
           set_lnw_sad(true, too_hi, get_lnw(too_hi));
@@ -607,8 +791,8 @@ struct BinWalker {
       if (!accumulate(BC_HIST, en, 0.0)) return false;
       if (has_min_gamma && gamma < min_gamma) return true;
       const double lowest = (double)bc[BC_HIST].min_count / width;
-      const bool ok_lo = !args().has_min_e || get_count(args().min_e) > 0;
-      const bool ok_hi = !args().has_max_e || get_count(args().max_e) > 0;
+      const bool ok_lo = !args().has_min_e || get_count(args().min_e) > 0.0;
+      const bool ok_hi = !args().has_max_e || get_count(args().max_e) > 0.0;
       if (lowest > old_lowest && ok_lo && ok_hi) {
         const double mean = (double)bc[BC_HIST].total_count / (width * (double)len);
         if ((inv_t && lowest > 0.0) || lowest >= 0.8 * mean) {
@@ -707,7 +891,7 @@ struct BinWalker {
   }
 };
 
-template <int SYS, int KMAX>
+template <int SYS, int KMAX, bool LIN>
 __global__ __launch_bounds__(64) void k_bin(BinArgs ba, SetArgs sa, uint64_t nmoves, int init) {
   extern __shared__ double lds[];
   const int G = sa.G;
@@ -715,7 +899,7 @@ __global__ __launch_bounds__(64) void k_bin(BinArgs ba, SetArgs sa, uint64_t nmo
   const int lane = threadIdx.x & (G - 1);
   const int walker = blockIdx.x * (64 / G) + group;
   if (walker >= sa.W) return;
-  BinWalker<SYS, KMAX> b;
+  BinWalker<SYS, KMAX, LIN> b;
   b.load(ba, sa, walker, lane, lds + group * 3 * sa.npad);
   if (init) {
     b.from_params();
@@ -730,7 +914,10 @@ static hipError_t launch_bin_t(const BinArgs &ba, const SetArgs &sa, uint64_t mo
   const int per = 64 / sa.G;
   const int blocks = (sa.W + per - 1) / per;
   const size_t lds = SYS == SYS_ISING ? 0 : sizeof(double) * 3 * (size_t)per * sa.npad;
-  hipLaunchKernelGGL((k_bin<SYS, KMAX>), dim3(blocks), dim3(64), lds, s, ba, sa, moves, init);
+  if (ba.linear)
+    hipLaunchKernelGGL((k_bin<SYS, KMAX, true>), dim3(blocks), dim3(64), lds, s, ba, sa, moves, init);
+  else
+    hipLaunchKernelGGL((k_bin<SYS, KMAX, false>), dim3(blocks), dim3(64), lds, s, ba, sa, moves, init);
   return hipGetLastError();
 }
 

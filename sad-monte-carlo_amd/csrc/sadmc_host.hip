@@ -1375,6 +1375,8 @@ int sadmc_bin_add_set(sadmc_ctx *c, const sadmc_set_params *p, int32_t W, const 
     return fail(c, SADMC_E_ARG, "the binning walker runs SAD, SAMC, WL or 1/t-WL");
   if (p->precision != SADMC_FP64 || p->refresh != 0 || p->move_kind != SADMC_MOVE_CUBE)
     return fail(c, SADMC_E_ARG, "the binning walker runs the cube move in FP64 with the error-estimate energy rule");
+  if (p->binning != SADMC_BIN_HISTOGRAM && p->binning != SADMC_BIN_LINEAR)
+    return fail(c, SADMC_E_ARG, "unknown binning");
   BinSetHost b;
   b.p = *p;
   b.W = W;
@@ -1493,6 +1495,7 @@ int sadmc_bin_add_set(sadmc_ctx *c, const sadmc_set_params *p, int32_t W, const 
   ba.move_param = p->move_param;
   ba.min_e = p->min_e;
   ba.max_e = p->max_e;
+  ba.linear = p->binning == SADMC_BIN_LINEAR;
   HIPCHECK(c, launch_binning(b.sys, b.KMAX, ba, a, 0, 1, c->stream));
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   c->bsets.push_back(b);

@@ -77,6 +77,7 @@ struct BinArgs {
   double e_lo;              // energy of physical bin 0's lower edge, for placement
   int32_t move_plan, has_min_e, has_max_e, collect_pressure;
   double move_param, min_e, max_e;
+  int32_t linear;           // binning/linear.rs instead of binning/histogram.rs
 };
 // `init` runs from_params (energy, descent, method state) instead of moves.
 hipError_t launch_binning(int sys, int kmax, const BinArgs &ba, const SetArgs &sa, uint64_t moves, int init,

@@ -19,6 +19,7 @@ _LIB = None
 ISING, LJ, WCA = 0, 1, 2
 SAD, SAMC, WL, INV_T_WL, CANONICAL = 0, 1, 2, 3, 4
 TRANSLATION_SCALE, ACCEPTANCE_RATE = 0, 1
+BIN_HISTOGRAM, BIN_LINEAR = 0, 1
 FP64, FP32_PAIRS = 0, 1
 MOVE_CUBE, MOVE_LEARNED = 0, 1
 
@@ -45,7 +46,7 @@ class SetParams(C.Structure):
         ("rc2", C.c_double), ("refresh", C.c_int32), ("ising_init", C.c_int32),
         ("collect_pressure", C.c_int32), ("lanes", C.c_int32),
         ("e_lo_bound", C.c_double), ("e_hi_bound", C.c_double),
-        ("move_kind", C.c_int32), ("reserved0", C.c_int32), ("reserved", C.c_double * 4),
+        ("move_kind", C.c_int32), ("binning", C.c_int32), ("reserved", C.c_double * 4),
     ]
 
 

@@ -113,3 +113,36 @@ def test_binning_argument_errors(gpu_ctx_factory):
     s = ctx.bin_add_set(S.make_params(system=S.ISING, n=4, method=S.SAD, method_param=1.0), [1])
     with pytest.raises(S.SadmcError, match="coordinates"):
         ctx.bin_coords(s, 0)
+
+
+LINEAR_CASES = [
+    ("ising_sad", S.ISING, dict(n=4, ising_init=1, method=S.SAD, method_param=1.0, energy_bin=4.0), None, [9000, 6000]),
+    ("ising_samc", S.ISING, dict(n=4, ising_init=1, method=S.SAMC, method_param=500.0, energy_bin=1.0), None,
+     [15000]),
+    ("ising_wl", S.ISING, dict(n=4, ising_init=1, method=S.WL, energy_bin=4.0, has_min_e=1, min_e=-24.0,
+                               has_max_e=1, max_e=24.0), None, [15000]),
+    ("lj13_sad", S.LJ, dict(n=13, lanes=4, method=S.SAD, method_param=0.05, energy_bin=0.05, move_param=0.1,
+                            has_max_e=1, max_e=0.0), 2.5, [10000, 10000]),
+    ("lj13_sad_acceptance", S.LJ, dict(n=13, lanes=4, method=S.SAD, method_param=0.05, energy_bin=0.05,
+                                       move_plan=S.ACCEPTANCE_RATE, move_param=0.3, has_max_e=1, max_e=0.0),
+     2.5, [20000]),
+    ("lj31_wl", S.LJ, dict(n=31, lanes=8, method=S.INV_T_WL, energy_bin=0.5, move_param=0.1, has_min_e=1,
+                           min_e=-60.0, has_max_e=1, max_e=0.0), 3.0, [10000]),
+]
+
+
+@pytest.mark.parametrize("name,system,kw,geom,chunks", LINEAR_CASES, ids=[c[0] for c in LINEAR_CASES])
+def test_linear_binning_bit_exact(gpu_ctx_factory, name, system, kw, geom, chunks):
+    print(f"case linear binning {name}", flush=True)
+    sp = S.make_params(system=system, binning=S.BIN_LINEAR, **kw)
+    W = 16 if system == S.ISING else 4
+    _run(gpu_ctx_factory, sp, np.arange(W, dtype=np.uint64) * 3 + 2, geom, chunks, f"linear {name}")
+
+
+def test_linear_wca32_with_pressure(gpu_ctx_factory):
+    print("case linear binning wca32 sad pressure", flush=True)
+    n = 32
+    L = (n / 0.6) ** (1.0 / 3.0)
+    sp = S.make_params(system=S.WCA, n=n, lanes=8, method=S.SAD, method_param=0.1, energy_bin=0.1, move_param=0.1,
+                       has_max_e=1, max_e=2.5 * n, collect_pressure=1, binning=S.BIN_LINEAR)
+    _run(gpu_ctx_factory, sp, np.arange(4, dtype=np.uint64) + 9, L, [3100], "linear wca32")

@@ -164,3 +164,50 @@ def test_energy_extra_counts_every_move_lj_and_wca():
     st, bins = bw.state(), bw.bins()
     assert int(st["pressure"]["total_count"]) == 3 == int(bins["pressure_count"].sum())
     assert np.all(bins["pressure_total"][bins["pressure_count"] > 0] > 0.0)
+
+
+# ---- the linear interpolation (binning/linear.rs)
+
+def test_linear_known_answers():
+    # linear.rs:170-186: one increment of weight 1 at E = 1.9 with bins of width 1
+    p = O.make_params(system=O.SCRIPTED, n=1, method=O.SAMC, method_param=1000.0, energy_bin=1.0, binning=1)
+    b = O.BWalker(p, script=np.array([np.nan]), e0=1.9)
+    b.run(1)  # SAMC's gamma is 1 at t = 1 <= t0
+    st, bins = b.state(), b.bins()
+    assert float(st["min"]) == 1.0 and int(st["len"]) == 2  # one more bin kept above the energy
+    assert b.lnw_at(1.95) > b.lnw_at(1.85)
+    assert b.lnw_at(1.0) > 0.0 and b.lnw_at(0.51) > 0.0 and b.lnw_at(2.49) > 0.0
+    assert b.lnw_at(3.01) == 0.0 and b.lnw_at(-0.01) == 0.0
+    # the weight splits 0.1 / 0.9 over the bins at 1 and 2 (fractional counts too)
+    np.testing.assert_allclose(bins["lnw_total"], [0.1, 0.9], rtol=1e-12)
+    np.testing.assert_allclose(bins["lnw_count"].view(np.float64), [0.1, 0.9], rtol=1e-12)
+    assert abs(b.lnw_at(1.95) - (0.9 * 0.95 + 0.1 * 0.05)) < 1e-12
+    assert abs(b.lnw_at(0.51) - 0.1 * 0.51) < 1e-12  # below the first bin: one-sided weight 1 + fidx
+
+
+@pytest.mark.parametrize("name,kw,tol", [
+    ("sad", dict(method=O.SAD, method_param=1.0), 0.35),
+    ("samc", dict(method=O.SAMC, method_param=1000.0), 0.2),
+])
+def test_linear_converges_to_exact_ising_dos(name, kw, tol):
+    lng = _exact_lng()
+    for seed in (1, 2):
+        b = O.BWalker(O.make_params(system=O.ISING, n=4, ising_init=1, seed=seed, energy_bin=4.0, binning=1, **kw))
+        assert b.run(3000000) == 0
+        st = b.state()
+        levels = [e for e in lng if name != "sad" or st["too_lo"] <= e <= st["too_hi"]]
+        diffs = np.array([b.lnw_at(e) - lng[e] for e in levels])
+        assert len(levels) >= 8
+        assert diffs.max() - diffs.min() < tol, (name, seed, np.round(diffs - diffs.mean(), 3))
+
+
+def test_linear_energy_extra_weights_sum_to_moves():
+    p = O.make_params(system=O.LJ, n=13, radius=2.5, method=O.SAD, method_param=0.05, energy_bin=0.05,
+                      move_param=0.1, has_max_e=1, max_e=0.0, order_g=4, seed=5, binning=1)
+    b = O.BWalker(p)
+    b.run(40000)
+    st, bins = b.state(), b.bins()
+    assert int(st["status"]) == 0
+    assert int(st["energy"]["total_count"]) == 40000 == int(st["lnw"]["total_count"])
+    w = bins["energy_count"].view(np.float64)
+    assert abs(w.sum() - 40000) < 1e-6  # each move adds weights 1 - offset and offset
