@@ -347,15 +347,16 @@ def tempering(rounds, ladders, replicas, lanes, cpu_seconds):
     return out
 
 
-def binning_path(moves, walkers, lanes, cpu_seconds):
+def binning_path(moves, walkers, lanes, cpu_seconds, linear=False):
     """The binning binary's walker (energy_binning.rs over a histogram) on the
     LJ31 configuration: 16384 walkers, N = 31 in a sphere of radius 3.0, cube
     moves 0.1, SAD min_T = 0.05, bins 0.01, max_allowed_energy 0, FP64.
     One untimed launch of 2000 moves, then one timed launch of `moves` moves
     per walker.  CPU baseline: the oracle's walker, one walker per host
     thread, for about `cpu_seconds`."""
+    kind = S.BIN_LINEAR if linear else S.BIN_HISTOGRAM
     sp = S.make_params(system=S.LJ, n=N_ATOMS, lanes=lanes, method=S.SAD, method_param=0.05, move_param=0.1,
-                       energy_bin=0.01, has_max_e=1, max_e=0.0)
+                       energy_bin=0.01, has_max_e=1, max_e=0.0, binning=kind)
     seeds = np.arange(walkers, dtype=np.uint64)
     ctx = S.Context(0)
     s = ctx.bin_add_set(sp, seeds, geom=RADIUS)
@@ -368,6 +369,7 @@ def binning_path(moves, walkers, lanes, cpu_seconds):
     total = walkers * moves
     gbs = BYTES_PER_MOVE * total / wall / 1e9
     out = {"moves_per_s": total / wall, "wall_s": wall, "walkers": walkers, "lanes": lanes, "moves": moves,
+           "binning": "linear" if linear else "histogram",
            "warmup_moves": WARMUP_MOVES, "mean_bins": float(np.mean(lens)),
            "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": gbs / HBM_PEAK_GBS, "bytes_per_move": BYTES_PER_MOVE}}
@@ -384,7 +386,7 @@ def binning_path(moves, walkers, lanes, cpu_seconds):
         def params(seed):
             return O.make_params(system=O.LJ, n=N_ATOMS, method=O.SAD, method_param=0.05, move_param=0.1,
                                  energy_bin=0.01, has_max_e=1, max_e=0.0, radius=RADIUS, order_g=lanes,
-                                 seed=int(seed))
+                                 seed=int(seed), binning=1 if linear else 0)
         probe = O.BWalker(params(0))
         probe.run(WARMUP_MOVES)
         t0 = time.perf_counter()
@@ -448,6 +450,7 @@ def main():
                     help="only run the mixed LJ/WCA batch (this many moves per launch)")
     ap.add_argument("--binning", type=int, default=0,
                     help="only run the binning-path LJ31 walkers for this many moves per walker")
+    ap.add_argument("--binning-linear", action="store_true", help="with --binning: the linear interpolation")
     ap.add_argument("--tempering", type=int, default=0,
                     help="only run LJ31 parallel tempering ladders for this many rounds")
     a = ap.parse_args()
@@ -465,7 +468,7 @@ def main():
                           "launches": a.steps, **res}))
         return
     if a.binning > 0:
-        res = binning_path(a.binning, a.walkers, a.lanes, 0.0 if a.no_cpu else 15.0)
+        res = binning_path(a.binning, a.walkers, a.lanes, 0.0 if a.no_cpu else 15.0, a.binning_linear)
         print(json.dumps({"metric": "LJ31 SAD moves/s of the binning-path walker on 1 GPU", "unit": "moves/s",
                           "higher_is_better": True, "value": res["moves_per_s"], **res}))
         return
