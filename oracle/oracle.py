@@ -124,6 +124,16 @@ def lib():
     L.or_pt_coords.argtypes = [vp, C.c_int32, vp]
     L.or_pt_rng.argtypes = [vp, u64p]
     L.or_pt_spins.argtypes = [vp, C.c_int32, vp]
+    L.or_zmc_new.argtypes = [vp, C.c_uint64, C.c_double, C.c_uint64, vp, vp]; L.or_zmc_new.restype = vp
+    L.or_zmc_run.argtypes = [vp, C.c_uint64]
+    L.or_zmc_replicas.argtypes = [vp]; L.or_zmc_replicas.restype = C.c_int32
+    L.or_zmc_replica.argtypes = [vp, C.c_int32, vp]
+    L.or_zmc_coords.argtypes = [vp, C.c_int32, vp]
+    L.or_zmc_spins.argtypes = [vp, C.c_int32, vp]
+    L.or_zmc_rng.argtypes = [vp, u64p]
+    L.or_zmc_median.argtypes = [vp, vp]; L.or_zmc_median.restype = C.c_int32
+    L.or_zmc_moves.argtypes = [vp]; L.or_zmc_moves.restype = C.c_uint64
+    L.or_zmc_free.argtypes = [vp]
     L.or_bmc_new.argtypes = [vp, vp, vp]; L.or_bmc_new.restype = vp
     L.or_bmc_new_scripted.argtypes = [vp, C.c_double, vp, C.c_int64]; L.or_bmc_new_scripted.restype = vp
     L.or_bmc_run.argtypes = [vp, C.c_uint64]; L.or_bmc_run.restype = C.c_int
@@ -498,6 +508,68 @@ class Tempering:
         s = (C.c_uint64 * 2)()
         lib().or_pt_rng(self.h, s)
         return list(s)
+
+
+class ZRep(C.Structure):
+    _fields_ = [("max_energy", C.c_double), ("cutoff_energy", C.c_double), ("rejected_count", C.c_uint64),
+                ("accepted_count", C.c_uint64), ("above_count", C.c_uint64), ("below_count", C.c_uint64),
+                ("upwelling_count", C.c_uint64), ("above_total", C.c_double), ("below_total", C.c_double),
+                ("above_total_squared", C.c_double), ("below_total_squared", C.c_double),
+                ("lowest_max_energy", C.c_double), ("unique_visitors", C.c_uint64), ("collecting_data", C.c_int32),
+                ("reserved", C.c_int32), ("rng", C.c_uint64 * 2), ("translation_scale", C.c_double),
+                ("E", C.c_double), ("error", C.c_double)]
+
+
+class Zmc:
+    """Energy-ceiling replicas (src/mc/energy_replicas.rs)."""
+
+    def __init__(self, params, seed, min_T=0.2, isbnb=64, coords=None, spins=None):
+        c = None if coords is None else np.ascontiguousarray(coords, dtype=np.float64)
+        sp = None if spins is None else np.ascontiguousarray(spins, dtype=np.int8)
+        self.n = params.n
+        self.h = lib().or_zmc_new(C.byref(params), seed, min_T, isbnb, _ptr(c), _ptr(sp))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_zmc_free(self.h)
+            self.h = None
+
+    def run(self, rounds):
+        lib().or_zmc_run(self.h, rounds)
+
+    def replicas(self):
+        out = []
+        for k in range(lib().or_zmc_replicas(self.h)):
+            r = ZRep()
+            lib().or_zmc_replica(self.h, k, C.byref(r))
+            d = {name: getattr(r, name) for name, _ in ZRep._fields_ if name != "reserved"}
+            d["rng"] = list(r.rng)
+            out.append(d)
+        return out
+
+    def coords(self, k):
+        c = np.zeros(3 * self.n)
+        lib().or_zmc_coords(self.h, k, _ptr(c))
+        return c
+
+    def spins(self, k):
+        s = np.zeros(self.n * self.n, np.int8)
+        lib().or_zmc_spins(self.h, k, _ptr(s))
+        return s
+
+    def rng(self):
+        s = (C.c_uint64 * 2)()
+        lib().or_zmc_rng(self.h, s)
+        return list(s)
+
+    def median(self):
+        n = lib().or_zmc_median(self.h, None)
+        a = np.zeros(n)
+        lib().or_zmc_median(self.h, _ptr(a))
+        return a
+
+    def moves(self):
+        return lib().or_zmc_moves(self.h)
 
 
 def rng_jump(state):

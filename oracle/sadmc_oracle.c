@@ -2350,3 +2350,319 @@ void or_bmc_free(or_bmc *b) {
   or_mc_free(b->m);
   free(b);
 }
+
+/* ===================================================================== */
+/* Energy-ceiling replicas ("ZMC", energy_replicas.rs)                   */
+/* ===================================================================== */
+
+/* A deep copy of a system (positions or spins, energy, error estimate). */
+static or_mc *mc_clone(const or_mc *m) {
+  or_mc *c = (or_mc *)malloc(sizeof(or_mc));
+  *c = *m;
+  c->lnw = NULL; c->etot = NULL; c->esq = NULL; c->ptotal = NULL; c->hist = NULL; c->tfound = NULL;
+  c->rt = NULL; c->pcount = NULL; c->vis = NULL; c->wlh = NULL; c->tr_e = NULL; c->tr_a = NULL;
+  c->script = NULL;
+  if (m->x) {
+    c->x = (double *)malloc(sizeof(double) * 3 * m->n);
+    memcpy(c->x, m->x, sizeof(double) * m->n);
+    memcpy(c->x + m->n, m->y, sizeof(double) * m->n);
+    memcpy(c->x + 2 * m->n, m->z, sizeof(double) * m->n);
+    c->y = c->x + m->n;
+    c->z = c->x + 2 * m->n;
+  }
+  if (m->S) {
+    c->S = (int8_t *)malloc((size_t)m->n * m->n);
+    memcpy(c->S, m->S, (size_t)m->n * m->n);
+  }
+  return c;
+}
+
+/* System::randomize: ising.rs:79-85 (the low bit of each draw, in site
+ * order), lj.rs:262-279 (each atom uniform in the sphere by rejection from
+ * Uniform(-1, 1)^3, scaled by the radius).  The energy is recomputed; the
+ * error estimate is left as it was. */
+static double sys_randomize(or_mc *m, uint64_t rng[2]) {
+  if (m->sys == OR_ISING) {
+    for (int64_t k = 0; k < (int64_t)m->n * m->n; k++) m->S[k] = (int8_t)(((int8_t)or_rng_next(rng) & 1) * 2 - 1);
+    m->E = or_ising_energy(m->n, m->S);
+  } else {
+    double R = m->p.radius;
+    for (int i = 0; i < m->n; i++) {
+      double x, y, z;
+      for (;;) {
+        x = or_uniform_f64(rng, -1.0, 1.0);
+        y = or_uniform_f64(rng, -1.0, 1.0);
+        z = or_uniform_f64(rng, -1.0, 1.0);
+        if (norm2(x, y, z) < 1.0) break;
+      }
+      m->x[i] = x * R;
+      m->y[i] = y * R;
+      m->z[i] = z * R;
+    }
+    m->E = sys_compute_energy(m, m->G);
+  }
+  return m->E;
+}
+
+struct or_zmc {
+  or_params p;
+  int R, cap;
+  or_mc **sys;  /* replica r's system (swapped with the systems) */
+  or_zrep *r;
+  uint64_t rng[2];
+  uint64_t moves, isbnb;
+  double min_T;
+  double *med;
+  int med_len;
+};
+
+static double zmc_max_size(const or_mc *m) { return m->sys == OR_ISING ? 0.5 : m->p.radius; }
+static uint64_t zmc_dim(const or_mc *m) { return m->sys == OR_ISING ? sys_moves_to_randomize(m) : 3 * sys_moves_to_randomize(m); }
+
+/* Replica::new (energy_replicas.rs:145-168). */
+static void zrep_new(or_zrep *r, double max_e, double cutoff, const or_mc *sys, const uint64_t rng[2]) {
+  memset(r, 0, sizeof *r);
+  r->max_energy = max_e;
+  r->cutoff_energy = cutoff;
+  r->translation_scale = zmc_max_size(sys);
+  r->lowest_max_energy = max_e;
+  r->unique_visitors = 1;
+  r->collecting_data = 1;
+  r->rng[0] = rng[0];
+  r->rng[1] = rng[1];
+}
+
+static int cmp_double(const void *a, const void *b) {
+  double x = *(const double *)a, y = *(const double *)b;
+  return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+/* MC::from_params (energy_replicas.rs:344-395). */
+or_zmc *or_zmc_new(const or_params *p, uint64_t seed, double min_T, uint64_t isbnb, const double *coords,
+                   const int8_t *spins) {
+  or_params q = *p;
+  q.has_max_e = 0;
+  q.has_min_e = 0;
+  q.method = OR_CANONICAL;
+  q.method_param = 1.0;
+  q.seed = seed;
+  or_zmc *z = (or_zmc *)calloc(1, sizeof(or_zmc));
+  z->p = q;
+  z->min_T = min_T;
+  z->isbnb = isbnb ? isbnb : 64;
+  or_mc *system = or_mc_new(&q, coords, spins);
+  or_rng_seed(seed, z->rng);
+  const int MAX_INIT = 1 << 15;
+  double *energies = (double *)malloc(sizeof(double) * MAX_INIT);
+  for (int k = 0; k < MAX_INIT; k++) energies[k] = sys_randomize(system, z->rng);
+  or_mc *high = mc_clone(system);
+  sys_randomize(high, z->rng);
+  while (system->E > energies[MAX_INIT / 2]) sys_randomize(system, z->rng);
+  qsort(energies, MAX_INIT, sizeof(double), cmp_double);
+  z->cap = 16;
+  z->sys = (or_mc **)calloc(z->cap, sizeof(or_mc *));
+  z->r = (or_zrep *)calloc(z->cap, sizeof(or_zrep));
+  z->sys[0] = high;
+  z->sys[1] = system;
+  zrep_new(&z->r[0], INFINITY, energies[MAX_INIT / 2], high, z->rng);
+  zrep_new(&z->r[1], energies[MAX_INIT / 2], energies[MAX_INIT / 4], system, z->rng);
+  z->R = 2;
+  or_rng_jump(z->rng);
+  z->med = (double *)malloc(sizeof(double) * 4096);
+  z->med[0] = energies[MAX_INIT / 4];
+  z->med_len = 1;
+  free(energies);
+  return z;
+}
+
+/* Replica::run_once (energy_replicas.rs:200-245); LJ and Ising collect no
+ * extra data. */
+static void zrep_run_once(or_zrep *r, or_mc *m, double very_lowest_max) {
+  m->rng[0] = r->rng[0];
+  m->rng[1] = r->rng[1];
+  if (isfinite(r->max_energy)) {
+    double e;
+    if (sys_plan_move(m, r->translation_scale, &e)) {
+      if (e < r->max_energy) {
+        sys_confirm(m);
+        r->accepted_count += 1;
+      } else {
+        r->rejected_count += 1;
+      }
+    } else {
+      r->rejected_count += 1;
+    }
+  } else {
+    sys_randomize(m, m->rng);
+    r->lowest_max_energy = r->max_energy;
+  }
+  r->rng[0] = m->rng[0];
+  r->rng[1] = m->rng[1];
+  double e = m->E;
+  if (r->collecting_data) {
+    if (e > r->cutoff_energy) {
+      r->above_count += 1;
+      r->above_total += e;
+      r->above_total_squared += e * e;
+    } else {
+      r->below_count += 1;
+      r->below_total += e;
+      r->below_total_squared += e * e;
+    }
+    if (r->lowest_max_energy == very_lowest_max) r->upwelling_count += 1;
+  }
+}
+
+/* MedianEstimator (energy_replicas.rs:47-96). */
+static void zmed_add(or_zmc *z, double e) {
+  if (z->med_len < 4096) {
+    z->med[z->med_len++] = e;
+  } else if (or_gen_f64(z->rng) < 1.0 / ((double)z->med_len + 1.0)) {
+    uint64_t i = or_gen_range(z->rng, 0, (uint64_t)z->med_len);
+    z->med[i] = e;
+  }
+}
+static double zmed_median(or_zmc *z) {
+  int n = z->med_len;
+  qsort(z->med, n, sizeof(double), cmp_double);
+  int middle = n / 2;
+  double em = z->med[middle];
+  for (int i = middle + 1; i < n; i++)
+    if (z->med[i] != em) return 0.5 * (em + z->med[i]);
+  for (int i = middle - 1; i >= 0; i--)
+    if (z->med[i] != em) return 0.5 * (em + z->med[i]);
+  return em;
+}
+
+/* Replica::decimate (energy_replicas.rs:169-192). */
+static void zrep_decimate(or_zrep *r) {
+  r->upwelling_count = 0;
+  if (r->above_count > 1) {
+    r->above_total /= (double)r->above_count;
+    r->above_total_squared /= (double)r->above_count;
+    r->above_count = 1;
+  }
+  if (r->below_count > 1) {
+    r->below_total /= (double)r->below_count;
+    r->below_total_squared /= (double)r->below_count;
+    r->below_count = 1;
+  }
+  r->lowest_max_energy = r->max_energy;
+  r->accepted_count = 1;
+  r->rejected_count = 1;
+  r->unique_visitors = 1;
+}
+
+/* Replica::occasional_update (energy_replicas.rs:246-290). */
+static void zrep_occasional(or_zrep *r, const or_mc *m) {
+  if (r->rejected_count > 128 && r->accepted_count > 128 && isfinite(r->max_energy)) {
+    double ratio = (double)r->accepted_count / (double)r->rejected_count;
+    double mx = (double)sys_moves_to_randomize(m);
+    if (ratio < 0.5 || ratio > 2.0 * mx) {
+      double adj = ratio < 0.5 ? ratio / sqrt(mx) : ratio * sqrt(mx);
+      if (adj > 2.0) adj = 2.0;
+      else if (adj < 0.5) adj = 0.5;
+      r->translation_scale *= adj;
+      r->accepted_count = 0;
+      r->rejected_count = 0;
+    }
+  }
+}
+
+/* MC::run_once (energy_replicas.rs:504-641), without the plugins. */
+void or_zmc_run(or_zmc *z, uint64_t rounds) {
+  for (uint64_t round = 0; round < rounds; round++) {
+    uint64_t steps = sys_moves_to_randomize(z->sys[0]);
+    double lowest = z->r[z->R - 1].max_energy;
+    uint64_t these = 0;
+    for (int k = 0; k < z->R; k++) {
+      these += steps;
+      if (isfinite(z->r[k].max_energy)) {
+        for (uint64_t i = 0; i < steps; i++) zrep_run_once(&z->r[k], z->sys[k], lowest);
+      } else {
+        zrep_run_once(&z->r[k], z->sys[k], lowest);
+      }
+    }
+    int first = (or_rng_next(z->rng) >> 63) ? 0 : 1;
+    for (int i = first; i + 1 < z->R; i += 2) {
+      or_zrep *r0 = &z->r[i], *r1 = &z->r[i + 1];
+      if (z->sys[i]->E < r1->max_energy) {
+        or_mc *t = z->sys[i];
+        z->sys[i] = z->sys[i + 1];
+        z->sys[i + 1] = t;
+        double l = r0->lowest_max_energy;
+        r0->lowest_max_energy = r1->lowest_max_energy;
+        r1->lowest_max_energy = l;
+        r0->collecting_data = 1;
+        r1->collecting_data = 1;
+        if (r1->lowest_max_energy > r1->max_energy) {
+          r1->unique_visitors += 1;
+          r1->lowest_max_energy = r1->max_energy;
+        }
+      }
+    }
+    {
+      or_zrep *last = &z->r[z->R - 1];
+      double le = z->sys[z->R - 1]->E;
+      if (le < last->cutoff_energy) zmed_add(z, le);
+    }
+    or_zrep *r = &z->r[z->R - 1];
+    if (r->unique_visitors >= z->isbnb && r->lowest_max_energy == r->max_energy) {
+      double mean_below = r->below_total / (double)r->below_count;
+      if (mean_below + z->min_T < r->cutoff_energy && z->sys[z->R - 1]->E < r->cutoff_energy) {
+        double median_below = zmed_median(z);
+        z->med[0] = median_below;
+        z->med_len = 1;
+        if (z->R == z->cap) {
+          z->cap *= 2;
+          z->sys = (or_mc **)realloc(z->sys, sizeof(or_mc *) * z->cap);
+          z->r = (or_zrep *)realloc(z->r, sizeof(or_zrep) * z->cap);
+        }
+        or_zrep nr = *r;
+        nr.max_energy = r->cutoff_energy;
+        nr.cutoff_energy = median_below;
+        zrep_decimate(&nr);
+        nr.lowest_max_energy = -INFINITY;
+        nr.translation_scale = r->translation_scale * pow(0.5, 1.0 / (double)zmc_dim(z->sys[z->R - 1]));
+        or_rng_jump(nr.rng);
+        z->sys[z->R] = mc_clone(z->sys[z->R - 1]);
+        z->r[z->R] = nr;
+        z->R += 1;
+      }
+    }
+    for (int k = 0; k < z->R; k++) zrep_occasional(&z->r[k], z->sys[k]);
+    z->moves += these;
+  }
+}
+
+int32_t or_zmc_replicas(const or_zmc *z) { return z->R; }
+
+void or_zmc_replica(const or_zmc *z, int32_t k, or_zrep *out) {
+  *out = z->r[k];
+  out->E = z->sys[k]->E;
+  out->error = z->sys[k]->error;
+}
+
+void or_zmc_coords(const or_zmc *z, int32_t k, double *c) { or_mc_coords(z->sys[k], c); }
+void or_zmc_spins(const or_zmc *z, int32_t k, int8_t *s) { or_mc_spins(z->sys[k], s); }
+
+void or_zmc_rng(const or_zmc *z, uint64_t out[2]) {
+  out[0] = z->rng[0];
+  out[1] = z->rng[1];
+}
+
+int32_t or_zmc_median(const or_zmc *z, double *out) {
+  if (out) memcpy(out, z->med, sizeof(double) * z->med_len);
+  return z->med_len;
+}
+
+uint64_t or_zmc_moves(const or_zmc *z) { return z->moves; }
+
+void or_zmc_free(or_zmc *z) {
+  if (!z) return;
+  for (int k = 0; k < z->R; k++) or_mc_free(z->sys[k]);
+  free(z->sys);
+  free(z->r);
+  free(z->med);
+  free(z);
+}

@@ -255,6 +255,44 @@ void or_bmc_coords(const or_bmc *b, double *coords);
 void or_bmc_spins(const or_bmc *b, int8_t *spins);
 void or_bmc_free(or_bmc *b);
 
+/* ---- energy-ceiling replicas (src/mc/energy_replicas.rs, "ZMC") --------
+ * Ising or LJ systems.  from_params (energy_replicas.rs:344-395): 2^15
+ * randomizations of the system with the generator seed_from_u64(seed), a
+ * second randomized system for the unbounded replica, the starting system
+ * randomized until its energy is below the 2^14-th sampled energy (taken
+ * before the sort, as the reference does), replicas [inf, E(1/2)] and
+ * [E(1/2), E(1/4)] of the sorted energies with clones of the generator,
+ * which is then jumped; the median estimator starts at E(1/4).  A round is
+ * MC::run_once (energy_replicas.rs:504-641) without plugins: every bounded
+ * replica makes min_moves_to_randomize moves (accept iff E' < max_energy,
+ * moves of half-width translation_scale, which starts at max_size), the
+ * unbounded one randomizes once; neighbour swaps when the lower system fits
+ * under the upper replica's ceiling; the median estimator; a new replica
+ * after `isbnb` unique visitors; occasional translation-scale updates. */
+typedef struct {
+  double max_energy, cutoff_energy;
+  uint64_t rejected_count, accepted_count, above_count, below_count, upwelling_count;
+  double above_total, below_total, above_total_squared, below_total_squared;
+  double lowest_max_energy;
+  uint64_t unique_visitors;
+  int32_t collecting_data, reserved;
+  uint64_t rng[2];
+  double translation_scale;
+  double E, error;  /* the replica's system */
+} or_zrep;
+typedef struct or_zmc or_zmc;
+or_zmc *or_zmc_new(const or_params *p, uint64_t seed, double min_T, uint64_t isbnb, const double *coords,
+                   const int8_t *spins);
+void or_zmc_run(or_zmc *z, uint64_t rounds);
+int32_t or_zmc_replicas(const or_zmc *z);
+void or_zmc_replica(const or_zmc *z, int32_t k, or_zrep *out);
+void or_zmc_coords(const or_zmc *z, int32_t k, double *coords);
+void or_zmc_spins(const or_zmc *z, int32_t k, int8_t *spins);
+void or_zmc_rng(const or_zmc *z, uint64_t out[2]);
+int32_t or_zmc_median(const or_zmc *z, double *out);
+uint64_t or_zmc_moves(const or_zmc *z);
+void or_zmc_free(or_zmc *z);
+
 /* CPU baseline: run `threads` independent walkers (one per thread) for
  * `moves` moves each; returns wall seconds (init excluded). */
 double or_bench(const or_params *p, int32_t threads, uint64_t moves, uint64_t seed0);
