@@ -296,6 +296,49 @@ int sadmc_bin_bins(sadmc_ctx *ctx, int32_t set, int32_t w, int64_t max_bins, sad
 int sadmc_bin_coords(sadmc_ctx *ctx, int32_t set, int32_t w, double *coords);
 int sadmc_bin_spins(sadmc_ctx *ctx, int32_t set, int32_t w, int8_t *spins);
 
+/* ---- energy-ceiling replicas (src/mc/energy_replicas.rs, "ZMC") ---------
+ * `sims` independent runs of Ising or LJ systems (the replicas binary),
+ * each on one workgroup with up to max_replicas replicas (replicas * lanes
+ * <= 512).  from_params (energy_replicas.rs:344-395) runs on the host when
+ * the set is added: 2^15 randomizations with seed_from_u64(seeds[s]), the
+ * replicas [inf, E(1/2)] and [E(1/2), E(1/4)], the generator jumped, the
+ * median estimator at E(1/4).  A round is MC::run_once
+ * (energy_replicas.rs:504-641) without plugins: N (LJ) or N^2 (Ising)
+ * moves per bounded replica, accepted iff E' < max_energy, cube moves of
+ * half-width translation_scale (max_size at first: the radius, or 0.5), a
+ * randomization for the unbounded replica; neighbour swaps when the lower
+ * system fits under the upper ceiling; the median estimator; a new replica
+ * after `isbnb` (<= 0: 64) unique visitors when the mean below the cutoff is
+ * min_T under it; occasional translation-scale updates.  The lowest replica
+ * cannot split further once max_replicas exist: sadmc_zmc_run then fails
+ * with SADMC_E_CAPACITY. */
+typedef struct {
+  double max_energy, cutoff_energy;
+  uint64_t rejected_count, accepted_count, above_count, below_count, upwelling_count;
+  double above_total, below_total, above_total_squared, below_total_squared;
+  double lowest_max_energy; /* of the replica's system */
+  uint64_t unique_visitors;
+  int32_t collecting_data, reserved;
+  uint64_t rng[2];
+  double translation_scale;
+  double E, error; /* the replica's system */
+} sadmc_zreplica;
+
+int sadmc_zmc_add(sadmc_ctx *ctx, const sadmc_set_params *p, int32_t sims, const uint64_t *seeds,
+                  const double *geom, const double *coords, const int8_t *spins, double min_T, int64_t isbnb,
+                  int32_t max_replicas, int32_t *zmc_id);
+int sadmc_zmc_run(sadmc_ctx *ctx, int32_t zmc, uint64_t rounds);
+/* Replica count of run s, and up to max_out replica records, lowest index
+ * (unbounded) first. */
+int sadmc_zmc_replicas(sadmc_ctx *ctx, int32_t zmc, int32_t s, int32_t max_out, sadmc_zreplica *out,
+                       int32_t *count);
+int sadmc_zmc_coords(sadmc_ctx *ctx, int32_t zmc, int32_t s, int32_t replica, double *coords);
+int sadmc_zmc_spins(sadmc_ctx *ctx, int32_t zmc, int32_t s, int32_t replica, int8_t *spins);
+/* The run's own generator (2 words), its move count and its median
+ * estimator's energies (up to 4096; *len set). */
+int sadmc_zmc_state(sadmc_ctx *ctx, int32_t zmc, int32_t s, uint64_t *rng, uint64_t *moves, double *median,
+                    int32_t *len);
+
 /* ---- parallel tempering (src/mc/tempering.rs) ---------------------------
  * `ladders` independent ladders of `replicas` canonical replicas each, at
  * temperatures temps[0..replicas).  Ladder l starts every replica from one

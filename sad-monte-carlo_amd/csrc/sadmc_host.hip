@@ -80,6 +80,18 @@ struct PtHost {
   uint64_t *d_spins = nullptr, *d_lrng = nullptr;
 };
 
+// Energy-ceiling replica runs on the device.
+struct ZmcHost {
+  int S = 0, RM = 0, G = 1, K = 1, KMAX = 1, npad = 0, threads = 64, sys = 0, n = 0;
+  SetArgs args;
+  ZArgs za;
+  ZSim *d_sim = nullptr;
+  sadmc_zreplica *d_rep = nullptr;
+  double *d_slot = nullptr, *d_med = nullptr, *d_pos = nullptr, *d_geom = nullptr;
+  int32_t *d_slot_of = nullptr;
+  uint64_t *d_spins = nullptr;
+};
+
 // A walker set of the binning path on the device.
 struct BinSetHost {
   sadmc_set_params p;
@@ -126,6 +138,7 @@ struct sadmc_ctx {
   int loss_cap = 0;
   std::vector<PtHost> pts;
   std::vector<BinSetHost> bsets;
+  std::vector<ZmcHost> zmcs;
 };
 
 static int fail(sadmc_ctx *c, int code, const std::string &msg) {
@@ -351,6 +364,10 @@ void sadmc_destroy(sadmc_ctx *c) {
   for (void *p : {(void *)c->d_pw, (void *)c->d_pm, (void *)c->d_pv, (void *)c->d_pbf, (void *)c->d_rec,
                   (void *)c->d_partial, (void *)c->d_grad, (void *)c->d_loss})
     (void)hipFree(p);
+  for (auto &z : c->zmcs)
+    for (void *p : {(void *)z.d_sim, (void *)z.d_rep, (void *)z.d_slot, (void *)z.d_med, (void *)z.d_pos,
+                    (void *)z.d_geom, (void *)z.d_slot_of, (void *)z.d_spins})
+      (void)hipFree(p);
   for (auto &b : c->bsets)
     for (void *p : {(void *)b.d_ws, (void *)b.d_bins, (void *)b.d_pos, (void *)b.d_spins, (void *)b.d_flag})
       (void)hipFree(p);
@@ -1573,6 +1590,336 @@ int sadmc_bin_spins(sadmc_ctx *c, int32_t set, int32_t w, int8_t *spins) {
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   const int n = b->p.n;
   for (int k = 0; k < n * n; k++) spins[k] = ((bits >> k) & 1ULL) ? 1 : -1;
+  return SADMC_OK;
+}
+
+// ------------------------------------------------ energy-ceiling replicas
+
+namespace {
+
+// A system on the host for from_params: positions (SoA) or spins, and the
+// energy summed in the device's lane-group order.
+struct HostSys {
+  int sys = 0, n = 0, G = 1;
+  double radius = 0.0;
+  std::vector<double> c;   // 3n
+  std::vector<int8_t> s;   // n*n
+  double E = 0.0;
+
+  double energy() const {
+    if (sys == SYS_ISING) {
+      const int N = n;
+      double e = 0.0;
+      for (int i1 = 0; i1 < N; i1++)
+        for (int j1 = 0; j1 < N; j1++) {
+          const int j2 = j1 + 1 == N ? 0 : j1 + 1;
+          const int i2 = i1 + 1 == N ? 0 : i1 + 1;
+          const int nt = s[i1 + j2 * N] + s[i2 + j1 * N];
+          e += (double)nt * (double)s[i1 + j1 * N];
+        }
+      return e;
+    }
+    // lane l sums the pairs (i, j < i) of its atoms i = l, l + G, ...; the
+    // lanes meet in an xor butterfly
+    const double *x = c.data(), *y = x + n, *z = x + 2 * n;
+    double acc[64];
+    for (int l = 0; l < G; l++) acc[l] = 0.0;
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < i; j++) acc[i % G] += lj_pair(norm2(x[i] - x[j], y[i] - y[j], z[i] - z[j]));
+    for (int d = 1; d < G; d <<= 1) {
+      double t[64];
+      for (int l = 0; l < G; l++) t[l] = acc[l] + acc[l ^ d];
+      for (int l = 0; l < G; l++) acc[l] = t[l];
+    }
+    return acc[0];
+  }
+  // System::randomize (ising.rs:79-85, lj.rs:262-279)
+  double randomize(uint64_t &s0, uint64_t &s1) {
+    if (sys == SYS_ISING) {
+      for (int k = 0; k < n * n; k++) s[k] = (int8_t)(((rng_next(s0, s1) & 1ULL) ? 1 : 0) * 2 - 1);
+    } else {
+      const double sc = uniform_scale(-1.0, 1.0);
+      double *x = c.data(), *y = x + n, *z = x + 2 * n;
+      for (int i = 0; i < n; i++) {
+        double a, b, d;
+        for (;;) {
+          a = uniform_sample(s0, s1, -1.0, sc);
+          b = uniform_sample(s0, s1, -1.0, sc);
+          d = uniform_sample(s0, s1, -1.0, sc);
+          if (norm2(a, b, d) < 1.0) break;
+        }
+        x[i] = a * radius;
+        y[i] = b * radius;
+        z[i] = d * radius;
+      }
+    }
+    E = energy();
+    return E;
+  }
+};
+
+}  // namespace
+
+int sadmc_zmc_add(sadmc_ctx *c, const sadmc_set_params *p, int32_t S, const uint64_t *seeds, const double *geom,
+                  const double *coords, const int8_t *spins, double min_T, int64_t isbnb, int32_t RM,
+                  int32_t *zmc_id) {
+  if (!c || !p || S <= 0 || !seeds) return fail(c, SADMC_E_ARG, "null argument or no runs");
+  if (RM < 2) return fail(c, SADMC_E_ARG, "max_replicas must be >= 2");
+  if (p->precision != SADMC_FP64 || p->refresh != 0 || p->move_kind != SADMC_MOVE_CUBE)
+    return fail(c, SADMC_E_ARG, "energy-ceiling replicas run the cube move in FP64 with the error-estimate rule");
+  ZmcHost z;
+  z.S = S;
+  z.RM = RM;
+  const int n = p->n;
+  z.n = n;
+  if (p->system == SADMC_ISING) {
+    if (n < 2 || n > 8) return fail(c, SADMC_E_ARG, "Ising lattice width must be 2..8");
+    z.sys = SYS_ISING;
+  } else if (p->system == SADMC_LJ) {
+    if (n < 2 || n > 1024) return fail(c, SADMC_E_ARG, "atom count must be 2..1024");
+    if (!geom) return fail(c, SADMC_E_ARG, "LJ runs need a sphere radius per run");
+    z.sys = SYS_LJ;
+    int G = p->lanes > 0 ? p->lanes : auto_lanes(n);
+    if (G < 2 || G > 64 || (G & (G - 1))) return fail(c, SADMC_E_ARG, "lanes must be a power of two in 2..64");
+    z.G = G;
+    z.K = (n + G - 1) / G;
+    if (z.K > 16) return fail(c, SADMC_E_ARG, "too few lanes for this many atoms (max 16 atoms per lane)");
+    z.KMAX = z.K <= 1 ? 1 : z.K <= 2 ? 2 : z.K <= 4 ? 4 : z.K <= 8 ? 8 : 16;
+    z.npad = G * z.K;
+  } else {
+    return fail(c, SADMC_E_ARG, "energy-ceiling replicas are implemented for Ising and LJ systems");
+  }
+  if ((int64_t)RM * z.G > 512) return fail(c, SADMC_E_ARG, "max_replicas * lanes must be <= 512");
+  if (zmc_lds_bytes(RM, z.npad) > 65536) return fail(c, SADMC_E_ARG, "a run's systems do not fit 64 KiB of LDS");
+  z.threads = ((RM * z.G + 63) / 64) * 64;
+  const int steps = z.sys == SYS_ISING ? n * n : n;
+  const double dim = z.sys == SYS_ISING ? (double)n * n : 3.0 * n;
+  const int64_t SR = (int64_t)S * RM;
+  std::vector<ZSim> sim(S);
+  std::vector<sadmc_zreplica> rep(SR);
+  std::vector<double> slot(3 * SR, 0.0), med((size_t)S * 4096, 0.0);
+  std::vector<int32_t> slot_of(SR);
+  std::vector<double> pos(z.sys == SYS_ISING ? 0 : (size_t)SR * 3 * z.npad, 0.0);
+  std::vector<uint64_t> bits(z.sys == SYS_ISING ? SR : 0, 0);
+  std::memset(rep.data(), 0, sizeof(sadmc_zreplica) * SR);
+  const int MAX_INIT = 1 << 15;
+  std::vector<double> energies(MAX_INIT);
+  for (int s = 0; s < S; s++) {
+    // MC::from_params (energy_replicas.rs:344-395)
+    HostSys sys;
+    sys.sys = z.sys;
+    sys.n = n;
+    sys.G = z.G;
+    sys.radius = geom ? geom[s] : 0.0;
+    if (z.sys == SYS_ISING) {
+      sys.s.resize((size_t)n * n);
+      if (spins)
+        std::memcpy(sys.s.data(), spins + (size_t)s * n * n, (size_t)n * n);
+      else
+        sadmc_init_spins(n, p->ising_init, seeds[s], sys.s.data());
+    } else {
+      sys.c.resize((size_t)3 * n);
+      if (coords)
+        std::memcpy(sys.c.data(), coords + (size_t)s * 3 * n, sizeof(double) * 3 * n);
+      else if (sadmc_init_coords(SADMC_LJ, n, sys.radius, seeds[s], sys.c.data()))
+        return fail(c, SADMC_E_INIT, "initial configuration failed");
+    }
+    sys.E = sys.energy();
+    uint64_t s0, s1;
+    rng_seed(seeds[s], s0, s1);
+    for (int k = 0; k < MAX_INIT; k++) energies[k] = sys.randomize(s0, s1);
+    HostSys high = sys;
+    high.randomize(s0, s1);
+    while (sys.E > energies[MAX_INIT / 2]) sys.randomize(s0, s1);
+    std::sort(energies.begin(), energies.end());
+    const double e2 = energies[MAX_INIT / 2], e4 = energies[MAX_INIT / 4];
+    const double max_size = z.sys == SYS_ISING ? 0.5 : sys.radius;
+    for (int r = 0; r < 2; r++) {  // Replica::new (energy_replicas.rs:145-168)
+      sadmc_zreplica &q = rep[(size_t)s * RM + r];
+      q.max_energy = r == 0 ? INFINITY : e2;
+      q.cutoff_energy = r == 0 ? e2 : e4;
+      q.lowest_max_energy = q.max_energy;
+      q.unique_visitors = 1;
+      q.collecting_data = 1;
+      q.rng[0] = s0;
+      q.rng[1] = s1;
+      q.translation_scale = max_size;
+    }
+    rng_jump(s0, s1);
+    sim[s].rng[0] = s0;
+    sim[s].rng[1] = s1;
+    sim[s].moves = 0;
+    sim[s].R = 2;
+    sim[s].status = 0;
+    sim[s].med_len = 1;
+    med[(size_t)s * 4096] = e4;
+    for (int r = 0; r < RM; r++) slot_of[(size_t)s * RM + r] = r;
+    const HostSys *systems[2] = {&high, &sys};
+    for (int r = 0; r < 2; r++) {
+      const size_t w = (size_t)s * RM + r;
+      slot[3 * w] = systems[r]->E;
+      slot[3 * w + 1] = 0.0;
+      slot[3 * w + 2] = rep[w].lowest_max_energy;
+      if (z.sys == SYS_ISING) {
+        for (int k = 0; k < n * n; k++)
+          if (systems[r]->s[k] > 0) bits[w] |= 1ULL << k;
+      } else {
+        for (int d = 0; d < 3; d++)
+          for (int j = 0; j < n; j++) pos[(w * 3 + d) * z.npad + j] = systems[r]->c[(size_t)d * n + j];
+      }
+    }
+  }
+  HIPCHECK(c, hipSetDevice(c->device));
+  HIPCHECK(c, hipMalloc(&z.d_sim, sizeof(ZSim) * S));
+  HIPCHECK(c, hipMalloc(&z.d_rep, sizeof(sadmc_zreplica) * SR));
+  HIPCHECK(c, hipMalloc(&z.d_slot, sizeof(double) * 3 * SR));
+  HIPCHECK(c, hipMalloc(&z.d_med, sizeof(double) * med.size()));
+  HIPCHECK(c, hipMalloc(&z.d_slot_of, sizeof(int32_t) * SR));
+  HIPCHECK(c, hipMemcpyAsync(z.d_sim, sim.data(), sizeof(ZSim) * S, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(z.d_rep, rep.data(), sizeof(sadmc_zreplica) * SR, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(z.d_slot, slot.data(), sizeof(double) * 3 * SR, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(z.d_med, med.data(), sizeof(double) * med.size(), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(z.d_slot_of, slot_of.data(), sizeof(int32_t) * SR, hipMemcpyHostToDevice, c->stream));
+  if (z.sys == SYS_ISING) {
+    HIPCHECK(c, hipMalloc(&z.d_spins, sizeof(uint64_t) * SR));
+    HIPCHECK(c, hipMemcpyAsync(z.d_spins, bits.data(), sizeof(uint64_t) * SR, hipMemcpyHostToDevice, c->stream));
+  } else {
+    HIPCHECK(c, hipMalloc(&z.d_pos, sizeof(double) * pos.size()));
+    HIPCHECK(c, hipMemcpyAsync(z.d_pos, pos.data(), sizeof(double) * pos.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMalloc(&z.d_geom, sizeof(double) * S));
+    HIPCHECK(c, hipMemcpyAsync(z.d_geom, geom, sizeof(double) * S, hipMemcpyHostToDevice, c->stream));
+  }
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  SetArgs &a = z.args;
+  std::memset(&a, 0, sizeof a);
+  a.pos = z.d_pos;
+  a.spins = z.d_spins;
+  a.W = (int32_t)SR;
+  a.n = n;
+  a.npad = z.npad;
+  a.G = z.G;
+  a.K = z.K;
+  a.method = SADMC_CANONICAL;
+  a.precision = SADMC_FP64;
+  a.move_kind = SADMC_MOVE_CUBE;
+  a.width = 1.0;
+  a.inv_width = 1.0;
+  const uint64_t nn = (uint64_t)n;
+  a.zone_n = ~0ULL - ((~0ULL - nn + 1) % nn);
+  a.zone_site = (nn << __builtin_clzll(nn)) - 1;
+  ZArgs &za = z.za;
+  za.sim = z.d_sim;
+  za.rep = z.d_rep;
+  za.slot = z.d_slot;
+  za.slot_of = z.d_slot_of;
+  za.med = z.d_med;
+  za.geom = z.d_geom;
+  za.RMAX = RM;
+  za.steps = steps;
+  za.isbnb = isbnb > 0 ? (uint64_t)isbnb : 64;
+  za.min_T = min_T;
+  za.shrink = std::pow(0.5, 1.0 / dim);
+  c->zmcs.push_back(z);
+  if (zmc_id) *zmc_id = (int32_t)c->zmcs.size() - 1;
+  return SADMC_OK;
+}
+
+static ZmcHost *zmc_get(sadmc_ctx *c, int32_t id) {
+  if (!c || id < 0 || id >= (int32_t)c->zmcs.size()) return nullptr;
+  return &c->zmcs[id];
+}
+
+int sadmc_zmc_run(sadmc_ctx *c, int32_t id, uint64_t rounds) {
+  ZmcHost *z = zmc_get(c, id);
+  if (!z) return fail(c, SADMC_E_ARG, "no such replica set");
+  HIPCHECK(c, hipSetDevice(c->device));
+  if (rounds) HIPCHECK(c, launch_zmc(z->sys, z->KMAX, z->za, z->args, z->S, z->threads, rounds, c->stream));
+  std::vector<ZSim> sim(z->S);
+  HIPCHECK(c, hipMemcpyAsync(sim.data(), z->d_sim, sizeof(ZSim) * z->S, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  for (int s = 0; s < z->S; s++)
+    if (sim[s].status) {
+      char buf[160];
+      std::snprintf(buf, sizeof buf, "replica run %d stopped: its lowest replica needs to split beyond %d replicas",
+                    s, z->RM);
+      return fail(c, SADMC_E_CAPACITY, buf);
+    }
+  return SADMC_OK;
+}
+
+int sadmc_zmc_replicas(sadmc_ctx *c, int32_t id, int32_t s, int32_t max_out, sadmc_zreplica *out, int32_t *count) {
+  ZmcHost *z = zmc_get(c, id);
+  if (!z || s < 0 || s >= z->S) return fail(c, SADMC_E_ARG, "no such replica run");
+  ZSim sim;
+  HIPCHECK(c, hipMemcpyAsync(&sim, z->d_sim + s, sizeof sim, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  if (count) *count = sim.R;
+  if (out) {
+    const int k = std::min(max_out, sim.R);
+    if (k > 0)
+      HIPCHECK(c, hipMemcpyAsync(out, z->d_rep + (int64_t)s * z->RM, sizeof(sadmc_zreplica) * k,
+                                 hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+  }
+  return SADMC_OK;
+}
+
+static int zmc_slot(sadmc_ctx *c, ZmcHost *z, int32_t s, int32_t r, int64_t *w) {
+  if (s < 0 || s >= z->S || r < 0 || r >= z->RM) return fail(c, SADMC_E_ARG, "run or replica out of range");
+  int32_t slot = 0;
+  HIPCHECK(c, hipMemcpyAsync(&slot, z->d_slot_of + (int64_t)s * z->RM + r, sizeof slot, hipMemcpyDeviceToHost,
+                             c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  *w = (int64_t)s * z->RM + slot;
+  return SADMC_OK;
+}
+
+int sadmc_zmc_coords(sadmc_ctx *c, int32_t id, int32_t s, int32_t r, double *coords) {
+  ZmcHost *z = zmc_get(c, id);
+  if (!z || !coords) return fail(c, SADMC_E_ARG, "no such replica set or null output");
+  if (z->sys == SYS_ISING) return fail(c, SADMC_E_ARG, "Ising replicas have spins, not coordinates");
+  int64_t w;
+  int rc = zmc_slot(c, z, s, r, &w);
+  if (rc) return rc;
+  for (int d = 0; d < 3; d++)
+    HIPCHECK(c, hipMemcpyAsync(coords + (size_t)d * z->n, z->d_pos + (w * 3 + d) * z->npad, sizeof(double) * z->n,
+                               hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  return SADMC_OK;
+}
+
+int sadmc_zmc_spins(sadmc_ctx *c, int32_t id, int32_t s, int32_t r, int8_t *spins) {
+  ZmcHost *z = zmc_get(c, id);
+  if (!z || !spins) return fail(c, SADMC_E_ARG, "no such replica set or null output");
+  if (z->sys != SYS_ISING) return fail(c, SADMC_E_ARG, "only Ising replicas have spins");
+  int64_t w;
+  int rc = zmc_slot(c, z, s, r, &w);
+  if (rc) return rc;
+  uint64_t b = 0;
+  HIPCHECK(c, hipMemcpyAsync(&b, z->d_spins + w, sizeof b, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  for (int k = 0; k < z->n * z->n; k++) spins[k] = ((b >> k) & 1ULL) ? 1 : -1;
+  return SADMC_OK;
+}
+
+int sadmc_zmc_state(sadmc_ctx *c, int32_t id, int32_t s, uint64_t *rng, uint64_t *moves, double *median,
+                    int32_t *len) {
+  ZmcHost *z = zmc_get(c, id);
+  if (!z || s < 0 || s >= z->S) return fail(c, SADMC_E_ARG, "no such replica run");
+  ZSim sim;
+  HIPCHECK(c, hipMemcpyAsync(&sim, z->d_sim + s, sizeof sim, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  if (rng) {
+    rng[0] = sim.rng[0];
+    rng[1] = sim.rng[1];
+  }
+  if (moves) *moves = sim.moves;
+  if (len) *len = sim.med_len;
+  if (median && sim.med_len > 0) {
+    HIPCHECK(c, hipMemcpyAsync(median, z->d_med + (int64_t)s * 4096, sizeof(double) * sim.med_len,
+                               hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+  }
   return SADMC_OK;
 }
 

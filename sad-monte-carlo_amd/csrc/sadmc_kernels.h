@@ -67,6 +67,29 @@ struct PtArgs {
   int32_t R, steps;
   double d, sc;        // cube half-width and its Uniform scale
 };
+// Energy-ceiling replica runs (k_zmc.hip): one workgroup per run.  Slots
+// hold systems (slot s of run i is walker i * RMAX + s of the SetArgs); the
+// replica -> slot table follows the reference's exchanges of systems.
+struct ZSim {
+  uint64_t rng[2];  // the run's own generator
+  uint64_t moves;
+  int32_t R, status, med_len, pad;
+};
+struct ZArgs {
+  ZSim *sim;              // [S]
+  sadmc_zreplica *rep;    // [S][RMAX] by replica
+  double *slot;           // [S][RMAX][3]: E, error, lowest_max_energy of each slot's system
+  int32_t *slot_of;       // [S][RMAX]
+  double *med;            // [S][4096] median estimator
+  const double *geom;     // [S] sphere radius (LJ)
+  int32_t RMAX, steps;
+  uint64_t isbnb;
+  double min_T, shrink;   // shrink: 0.5^(1/dimensionality)
+};
+size_t zmc_lds_bytes(int RMAX, int npad);
+hipError_t launch_zmc(int sys, int kmax, const ZArgs &z, const SetArgs &a, int sims, int threads, uint64_t rounds,
+                      hipStream_t s);
+
 // Walker sets of the binning path (k_binning.hip); the system part uses
 // the SetArgs of the set (positions, spins, lanes), the rest is here.
 struct BinArgs {

@@ -75,6 +75,15 @@ REPLICA_DTYPE = np.dtype([
     ("accepted_swaps", "<u8"), ("rejected_swaps", "<u8"), ("geom", "<f8"), ("status", "<i4"), ("reserved0", "<i4"),
 ])
 
+# sadmc_zreplica: one replica of an energy-ceiling run.
+ZREPLICA_DTYPE = np.dtype([
+    ("max_energy", "<f8"), ("cutoff_energy", "<f8"), ("rejected_count", "<u8"), ("accepted_count", "<u8"),
+    ("above_count", "<u8"), ("below_count", "<u8"), ("upwelling_count", "<u8"), ("above_total", "<f8"),
+    ("below_total", "<f8"), ("above_total_squared", "<f8"), ("below_total_squared", "<f8"),
+    ("lowest_max_energy", "<f8"), ("unique_visitors", "<u8"), ("collecting_data", "<i4"), ("reserved", "<i4"),
+    ("rng", "<u8", 2), ("translation_scale", "<f8"), ("E", "<f8"), ("error", "<f8"),
+])
+
 # sadmc_bcounts / sadmc_bwalker_state / sadmc_bbin: walkers of the binning path.
 BCOUNTS_DTYPE = np.dtype([
     ("min_total", "<f8"), ("max_total", "<f8"), ("e_max_total", "<f8"), ("e_max_count", "<f8"),
@@ -114,6 +123,7 @@ EXPORTS = [
     "sadmc_prop_train", "sadmc_mean_entropy",
     "sadmc_pt_add", "sadmc_pt_run", "sadmc_pt_replicas", "sadmc_pt_coords", "sadmc_pt_spins", "sadmc_pt_ladder_rng",
     "sadmc_bin_add_set", "sadmc_bin_run", "sadmc_bin_state", "sadmc_bin_bins", "sadmc_bin_coords", "sadmc_bin_spins",
+    "sadmc_zmc_add", "sadmc_zmc_run", "sadmc_zmc_replicas", "sadmc_zmc_coords", "sadmc_zmc_spins", "sadmc_zmc_state",
 ]
 
 PROP_NP, PROP_REC, PROP_NS = 1377, 24, 16
@@ -176,6 +186,12 @@ def lib():
     L.sadmc_bin_bins.argtypes = [vp, i32, i32, i64, vp]
     L.sadmc_bin_coords.argtypes = [vp, i32, i32, vp]
     L.sadmc_bin_spins.argtypes = [vp, i32, i32, vp]
+    L.sadmc_zmc_add.argtypes = [vp, C.POINTER(SetParams), i32, vp, vp, vp, vp, d, i64, i32, C.POINTER(i32)]
+    L.sadmc_zmc_run.argtypes = [vp, i32, u64]
+    L.sadmc_zmc_replicas.argtypes = [vp, i32, i32, i32, vp, C.POINTER(i32)]
+    L.sadmc_zmc_coords.argtypes = [vp, i32, i32, i32, vp]
+    L.sadmc_zmc_spins.argtypes = [vp, i32, i32, i32, vp]
+    L.sadmc_zmc_state.argtypes = [vp, i32, i32, vp, C.POINTER(u64), vp, C.POINTER(i32)]
     for name in EXPORTS:
         if name not in ("sadmc_last_error", "sadmc_destroy", "sadmc_abi_version"):
             getattr(L, name).restype = C.c_int
@@ -335,6 +351,50 @@ class Context:
         a = np.zeros(n * n, np.int8)
         self._check(lib().sadmc_bin_spins(self.h, s, w, _p(a)), "bin_spins")
         return a
+
+    # ---- energy-ceiling replicas (energy_replicas.rs)
+    def zmc_add(self, params, seeds, geom=None, coords=None, spins=None, min_T=0.2, isbnb=64, max_replicas=32):
+        seeds = np.ascontiguousarray(seeds, dtype=np.uint64)
+        g = None if geom is None else np.ascontiguousarray(np.broadcast_to(geom, seeds.shape), dtype=np.float64)
+        c = None if coords is None else np.ascontiguousarray(coords, dtype=np.float64)
+        sp = None if spins is None else np.ascontiguousarray(spins, dtype=np.int8)
+        zid = C.c_int32()
+        self._check(lib().sadmc_zmc_add(self.h, C.byref(params), len(seeds), _p(seeds), _p(g), _p(c), _p(sp), min_T,
+                                        isbnb, max_replicas, C.byref(zid)), "zmc_add")
+        if not hasattr(self, "zmcs"):
+            self.zmcs = []
+        self.zmcs.append((params, len(seeds), max_replicas))
+        return zid.value
+
+    def zmc_run(self, z, rounds):
+        self._check(lib().sadmc_zmc_run(self.h, z, rounds), "zmc_run")
+
+    def zmc_replicas(self, z, s):
+        rm = self.zmcs[z][2]
+        out = np.zeros(rm, ZREPLICA_DTYPE)
+        cnt = C.c_int32()
+        self._check(lib().sadmc_zmc_replicas(self.h, z, s, rm, _p(out), C.byref(cnt)), "zmc_replicas")
+        return out[:cnt.value]
+
+    def zmc_coords(self, z, s, r):
+        c = np.zeros(3 * self.zmcs[z][0].n)
+        self._check(lib().sadmc_zmc_coords(self.h, z, s, r, _p(c)), "zmc_coords")
+        return c
+
+    def zmc_spins(self, z, s, r):
+        n = self.zmcs[z][0].n
+        a = np.zeros(n * n, np.int8)
+        self._check(lib().sadmc_zmc_spins(self.h, z, s, r, _p(a)), "zmc_spins")
+        return a
+
+    def zmc_state(self, z, s):
+        """(generator, moves, median estimator energies) of run s."""
+        rng = np.zeros(2, np.uint64)
+        mv = C.c_uint64()
+        med = np.zeros(4096)
+        ln = C.c_int32()
+        self._check(lib().sadmc_zmc_state(self.h, z, s, _p(rng), C.byref(mv), _p(med), C.byref(ln)), "zmc_state")
+        return rng, mv.value, med[:ln.value]
 
     # ---- parallel tempering
     def pt_add(self, params, temps, seeds, geom=None, coords=None, spins=None, canonical_steps=1):
