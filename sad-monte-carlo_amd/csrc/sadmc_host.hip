@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "sadmc_kernels.h"
@@ -1703,9 +1704,11 @@ int sadmc_zmc_add(sadmc_ctx *c, const sadmc_set_params *p, int32_t S, const uint
   std::vector<uint64_t> bits(z.sys == SYS_ISING ? SR : 0, 0);
   std::memset(rep.data(), 0, sizeof(sadmc_zreplica) * SR);
   const int MAX_INIT = 1 << 15;
-  std::vector<double> energies(MAX_INIT);
-  for (int s = 0; s < S; s++) {
-    // MC::from_params (energy_replicas.rs:344-395)
+  std::vector<int> init_failed(S, 0);
+  // MC::from_params (energy_replicas.rs:344-395) for run s; runs are
+  // independent, so they are spread over host threads
+  auto init_run = [&](int s) {
+    std::vector<double> energies(MAX_INIT);
     HostSys sys;
     sys.sys = z.sys;
     sys.n = n;
@@ -1721,8 +1724,10 @@ int sadmc_zmc_add(sadmc_ctx *c, const sadmc_set_params *p, int32_t S, const uint
       sys.c.resize((size_t)3 * n);
       if (coords)
         std::memcpy(sys.c.data(), coords + (size_t)s * 3 * n, sizeof(double) * 3 * n);
-      else if (sadmc_init_coords(SADMC_LJ, n, sys.radius, seeds[s], sys.c.data()))
-        return fail(c, SADMC_E_INIT, "initial configuration failed");
+      else if (sadmc_init_coords(SADMC_LJ, n, sys.radius, seeds[s], sys.c.data())) {
+        init_failed[s] = 1;
+        return;
+      }
     }
     sys.E = sys.energy();
     uint64_t s0, s1;
@@ -1768,7 +1773,18 @@ int sadmc_zmc_add(sadmc_ctx *c, const sadmc_set_params *p, int32_t S, const uint
           for (int j = 0; j < n; j++) pos[(w * 3 + d) * z.npad + j] = systems[r]->c[(size_t)d * n + j];
       }
     }
+  };
+  {
+    const int nt = std::max(1, std::min<int>(S, std::min(16u, std::max(1u, std::thread::hardware_concurrency()))));
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; t++)
+      pool.emplace_back([&, t]() {
+        for (int s = t; s < S; s += nt) init_run(s);
+      });
+    for (auto &th : pool) th.join();
   }
+  for (int s = 0; s < S; s++)
+    if (init_failed[s]) return fail(c, SADMC_E_INIT, "initial configuration failed");
   HIPCHECK(c, hipSetDevice(c->device));
   HIPCHECK(c, hipMalloc(&z.d_sim, sizeof(ZSim) * S));
   HIPCHECK(c, hipMalloc(&z.d_rep, sizeof(sadmc_zreplica) * SR));
