@@ -488,6 +488,49 @@ def zmc(rounds, runs, lanes, cpu_seconds):
     return out
 
 
+def entropy(moves, walkers, lanes, repeats):
+    """The mean S(E) of a walker set reduced on the device (the LJ31 SAD
+    configuration after `moves` moves per walker), timed against the same
+    reduction on the host from downloaded bins (analysis.mean_entropy, the
+    restatement of parse-binning.py:150-171 over walkers).  Algorithmic
+    bytes: each walker's 16-byte hot records over its bins (read once)."""
+    sys.path.insert(0, os.path.join(ROOT, "sad-monte-carlo_amd"))
+    import analysis
+    ctx = S.Context(0)
+    sid = ctx.add_set(lj_params(S.FP32_PAIRS, lanes), np.arange(walkers, dtype=np.uint64), geom=RADIUS)
+    ctx.init()
+    ctx.run(moves)
+    E, Sd, cnt = ctx.mean_entropy(sid)
+    times = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        ctx.mean_entropy(sid)
+        times.append(time.perf_counter() - t0)
+    dev = float(np.median(times))
+    st, rows = ctx.download_raw(sid)
+    t0 = time.perf_counter()
+    pairs = []
+    for w in range(walkers):
+        b = S.logical_bins(st[w], rows[w])
+        pairs.append((st[w], b))
+    Eh, Sh, ch = analysis.mean_entropy(pairs)
+    host = time.perf_counter() - t0
+    ctx.close()
+    nbins = int(np.sum(st["len"]))
+    gbs = 16.0 * nbins / dev / 1e9
+    same = len(E) == len(Eh) and np.allclose(E, Eh, atol=1e-9) and np.array_equal(cnt, ch)
+    out = {"device_s": dev, "host_s": host, "walkers": walkers, "moves": moves, "bins_total": nbins,
+           "grid_bins": int(len(E)), "same_grid_as_host": bool(same),
+           "max_abs_diff_vs_host": float(np.max(np.abs(Sd - Sh))) if same else None,
+           "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": gbs / HBM_PEAK_GBS, "bytes_per_bin": 16},
+           "cpu_baseline": {"value": host, "unit": "s", "cores": 1, "kind": "port",
+                            "sample": "analysis.mean_entropy over the same walkers' downloaded bins"},
+           "gpu_over_cpu": host / dev}
+    log(f"[entropy] device {dev * 1e3:.2f} ms, host {host:.2f} s over {walkers} walkers, {nbins} bins")
+    return out
+
+
 def copy_probe(nbytes):
     """A plain device-to-device copy of known size, for scaling the HBM byte
     counters (FETCH_SIZE, WRITE_SIZE) of a profiler run."""
@@ -523,6 +566,8 @@ def main():
     ap.add_argument("--binning-linear", action="store_true", help="with --binning: the linear interpolation")
     ap.add_argument("--zmc", type=int, default=0,
                     help="only run LJ31 energy-ceiling replica runs for this many rounds")
+    ap.add_argument("--entropy", type=int, default=0,
+                    help="only time the walker-set S(E) reduction after this many moves per walker")
     ap.add_argument("--tempering", type=int, default=0,
                     help="only run LJ31 parallel tempering ladders for this many rounds")
     a = ap.parse_args()
@@ -548,6 +593,11 @@ def main():
         res = zmc(a.zmc, 1024, a.lanes, 0.0 if a.no_cpu else 15.0)
         print(json.dumps({"metric": "LJ31 energy-ceiling replica moves/s on 1 GPU", "unit": "moves/s",
                           "higher_is_better": True, "value": res["moves_per_s"], **res}))
+        return
+    if a.entropy > 0:
+        res = entropy(a.entropy, a.walkers, a.lanes, 5)
+        print(json.dumps({"metric": "LJ31 walker-set S(E) reduction time on 1 GPU", "unit": "s",
+                          "higher_is_better": False, "value": res["device_s"], **res}))
         return
     if a.tempering > 0:
         res = tempering(a.tempering, 512, 16, a.lanes, 0.0 if a.no_cpu else 15.0)

@@ -106,13 +106,18 @@ __global__ __launch_bounds__(64) void k_entropy_walker(const sadmc_walker_state 
   }
 }
 
+// One lane per grid bin and one block row per chunk of walkers: partial
+// sums over the chunk's walkers, so that the grid fills the device.
 __global__ __launch_bounds__(256) void k_entropy_bins(const EntropyWalker *ew, int W, const HotBin *hot, int64_t cap,
-                                                      double width, int64_t ng, double *S, int64_t *count) {
+                                                      double width, int64_t ng, int chunks, double *psum,
+                                                      int64_t *pcnt) {
   const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (g >= ng) return;
+  const int c = blockIdx.y;
+  const int w0 = (int)((int64_t)W * c / chunks), w1 = (int)((int64_t)W * (c + 1) / chunks);
   double sum = 0.0;
   int64_t n = 0;
-  for (int w = 0; w < W; w++) {
+  for (int w = w0; w < w1; w++) {
     const EntropyWalker e = ew[w];
     const int64_t i = g - e.off;
     if (!e.ok || i < 0 || i >= e.len) continue;
@@ -131,19 +136,47 @@ __global__ __launch_bounds__(256) void k_entropy_bins(const EntropyWalker *ew, i
     sum += v - e.c;
     n += 1;
   }
+  psum[(int64_t)c * ng + g] = sum;
+  pcnt[(int64_t)c * ng + g] = n;
+}
+
+// The chunks' partial sums, combined in chunk order.
+__global__ __launch_bounds__(256) void k_entropy_combine(int64_t ng, int chunks, const double *psum,
+                                                         const int64_t *pcnt, double *S, int64_t *count) {
+  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (g >= ng) return;
+  double sum = 0.0;
+  int64_t n = 0;
+  for (int c = 0; c < chunks; c++) {
+    sum += psum[(int64_t)c * ng + g];
+    n += pcnt[(int64_t)c * ng + g];
+  }
   S[g] = n ? sum / (double)n : 0.0;
   count[g] = n;
 }
 
+int entropy_chunks(int W, int64_t ng) {
+  const int64_t blocks = (ng + 255) / 256;
+  int64_t c = (2048 + blocks - 1) / blocks;  // about 8 blocks per CU
+  if (c > 64) c = 64;
+  if (c > W) c = W;
+  return c < 1 ? 1 : (int)c;
+}
+
 hipError_t launch_entropy(const sadmc_walker_state *ws, const HotBin *hot, int W, int64_t cap, double width,
-                          double gmin, int64_t ga, int64_t gb, int64_t ng, void *ew, double *S, int64_t *count,
-                          hipStream_t s) {
+                          double gmin, int64_t ga, int64_t gb, int64_t ng, void *ew, int chunks, double *psum,
+                          int64_t *pcnt, double *S, int64_t *count, hipStream_t s) {
   hipLaunchKernelGGL(k_entropy_walker, dim3(W), dim3(64), 0, s, ws, hot, cap, width, gmin, ga, gb,
                      (EntropyWalker *)ew);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_entropy_bins, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, s, (const EntropyWalker *)ew,
-                     W, hot, cap, width, ng, S, count);
+  const unsigned bx = (unsigned)((ng + 255) / 256);
+  hipLaunchKernelGGL(k_entropy_bins, dim3(bx, (unsigned)chunks), dim3(256), 0, s, (const EntropyWalker *)ew, W, hot,
+                     cap, width, ng, chunks, psum, pcnt);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_entropy_combine, dim3(bx), dim3(256), 0, s, ng, chunks, (const double *)psum,
+                     (const int64_t *)pcnt, S, count);
   return hipGetLastError();
 }
 

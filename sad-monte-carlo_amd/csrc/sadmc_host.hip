@@ -1019,14 +1019,17 @@ int sadmc_mean_entropy(sadmc_ctx *c, int32_t set, double e_a, double e_b, int64_
   if (nbins) *nbins = ng;
   if (ng > max_bins) return fail(c, SADMC_E_CAPACITY, "output holds fewer bins than the set covers");
   void *ew = nullptr;
-  double *dS = nullptr;
-  int64_t *dC = nullptr;
+  double *dS = nullptr, *dPS = nullptr;
+  int64_t *dC = nullptr, *dPC = nullptr;
+  const int chunks = entropy_chunks(s->W, ng);
   int rc = SADMC_OK;
   if (hipMalloc(&ew, entropy_walker_bytes() * s->W) != hipSuccess || hipMalloc(&dS, sizeof(double) * ng) != hipSuccess ||
-      hipMalloc(&dC, sizeof(int64_t) * ng) != hipSuccess) {
+      hipMalloc(&dC, sizeof(int64_t) * ng) != hipSuccess ||
+      hipMalloc(&dPS, sizeof(double) * ng * chunks) != hipSuccess ||
+      hipMalloc(&dPC, sizeof(int64_t) * ng * chunks) != hipSuccess) {
     rc = fail(c, SADMC_E_HIP, "allocation failed");
-  } else if (launch_entropy(s->d_ws, s->d_hot, s->W, s->cap, w, gmin, ga, gb, ng, ew, dS, dC, c->stream) !=
-                 hipSuccess ||
+  } else if (launch_entropy(s->d_ws, s->d_hot, s->W, s->cap, w, gmin, ga, gb, ng, ew, chunks, dPS, dPC, dS, dC,
+                            c->stream) != hipSuccess ||
              (S && hipMemcpyAsync(S, dS, sizeof(double) * ng, hipMemcpyDeviceToHost, c->stream) != hipSuccess) ||
              (count && hipMemcpyAsync(count, dC, sizeof(int64_t) * ng, hipMemcpyDeviceToHost, c->stream) != hipSuccess) ||
              hipStreamSynchronize(c->stream) != hipSuccess) {
@@ -1035,6 +1038,8 @@ int sadmc_mean_entropy(sadmc_ctx *c, int32_t set, double e_a, double e_b, int64_
   (void)hipFree(ew);
   (void)hipFree(dS);
   (void)hipFree(dC);
+  (void)hipFree(dPS);
+  (void)hipFree(dPC);
   return rc;
 }
 

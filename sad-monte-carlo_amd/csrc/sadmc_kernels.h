@@ -46,9 +46,12 @@ hipError_t launch_prop_grad(const float *rec, int64_t nrec, int64_t first, int c
                             float *partial, hipStream_t s);
 // On-device mean entropy of a walker set (k_entropy.hip); `ew` holds
 // entropy_walker_bytes() per walker.
+// The bin pass splits the walkers into `chunks` (entropy_chunks) whose
+// partial sums (psum, pcnt: chunks * ng each) are combined in chunk order.
 hipError_t launch_entropy(const sadmc_walker_state *ws, const HotBin *hot, int W, int64_t cap, double width,
-                          double gmin, int64_t ga, int64_t gb, int64_t ng, void *ew, double *S, int64_t *count,
-                          hipStream_t s);
+                          double gmin, int64_t ga, int64_t gb, int64_t ng, void *ew, int chunks, double *psum,
+                          int64_t *pcnt, double *S, int64_t *count, hipStream_t s);
+int entropy_chunks(int W, int64_t ng);
 size_t entropy_walker_bytes();
 hipError_t launch_prop_update(const float *partial, int nblk, int count, float *master, float *m, float *v,
                               uint16_t *wbf, float lr, int step, float *grad_out, float *loss_out, hipStream_t s);
