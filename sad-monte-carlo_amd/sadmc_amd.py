@@ -20,6 +20,8 @@ ISING, LJ, WCA = 0, 1, 2
 SAD, SAMC, WL, INV_T_WL, CANONICAL = 0, 1, 2, 3, 4
 TRANSLATION_SCALE, ACCEPTANCE_RATE = 0, 1
 BIN_HISTOGRAM, BIN_LINEAR = 0, 1
+# walker status codes (sadmc_walker_state.status, sadmc_bwalker_state.status)
+WS_OK, WS_INDEX, WS_VERIFY, WS_INIT, WS_CAPACITY = 0, 1, 2, 3, 4
 FP64, FP32_PAIRS = 0, 1
 MOVE_CUBE, MOVE_LEARNED = 0, 1
 

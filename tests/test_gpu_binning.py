@@ -146,3 +146,21 @@ def test_linear_wca32_with_pressure(gpu_ctx_factory):
     sp = S.make_params(system=S.WCA, n=n, lanes=8, method=S.SAD, method_param=0.1, energy_bin=0.1, move_param=0.1,
                        has_max_e=1, max_e=2.5 * n, collect_pressure=1, binning=S.BIN_LINEAR)
     _run(gpu_ctx_factory, sp, np.arange(4, dtype=np.uint64) + 9, L, [3100], "linear wca32")
+
+
+@pytest.mark.parametrize("binning", [S.BIN_HISTOGRAM, S.BIN_LINEAR])
+def test_window_overflow_is_reported(gpu_ctx_factory, binning):
+    # a 9-bin window (e_lo_bound / e_hi_bound) cannot hold a 4x4 Ising walk:
+    # every walker stops with a capacity status and its bins stay inside the window
+    print(f"case binning window overflow binning={binning}", flush=True)
+    ctx = gpu_ctx_factory()
+    sp = S.make_params(system=S.ISING, n=4, ising_init=1, method=S.SAD, method_param=1.0, energy_bin=1.0,
+                       e_lo_bound=-10.0, e_hi_bound=-9.0, binning=binning)
+    s = ctx.bin_add_set(sp, np.arange(8, dtype=np.uint64) + 1)
+    with pytest.raises(S.SadmcError, match="status 4"):
+        ctx.bin_run(s, 5000)
+    for w in range(8):
+        st = ctx.bin_state(s, w)
+        assert int(st["status"]) == S.WS_CAPACITY
+        assert 0 <= int(st["base"]) and int(st["base"]) + int(st["len"]) <= int(st["cap"]) == 9
+        assert float(st["E"]) % 4 == 0 and -32 <= float(st["E"]) <= 32
