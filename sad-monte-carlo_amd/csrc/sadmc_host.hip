@@ -153,6 +153,24 @@ static int fail(sadmc_ctx *c, int code, const std::string &msg) {
     if (_e != hipSuccess) return fail(ctx, SADMC_E_HIP, std::string(#call ": ") + hipGetErrorString(_e)); \
   } while (0)
 
+// Frees the device buffers of a set under construction when an error
+// returns early, unless release() is called once the set is complete.
+struct AllocGuard {
+  std::vector<void *> ptrs;
+  bool armed = true;
+  template <typename T>
+  hipError_t alloc(T **p, size_t bytes) {
+    hipError_t e = hipMalloc((void **)p, bytes);
+    if (e == hipSuccess) ptrs.push_back((void *)*p);
+    return e;
+  }
+  void release() { armed = false; }
+  ~AllocGuard() {
+    if (armed)
+      for (void *p : ptrs) (void)hipFree(p);
+  }
+};
+
 // ---------------------------------------------------------------- init rules
 
 // LJ initial positions (uniform in the sphere, all pairs >= 0.8 sigma):
@@ -1184,6 +1202,7 @@ int sadmc_upload_raw(sadmc_ctx *c, int32_t set, const sadmc_walker_state *states
 int sadmc_pt_add(sadmc_ctx *c, const sadmc_set_params *p, int32_t L, int32_t R, const double *temps,
                  const uint64_t *seeds, const double *geom, const double *coords, const int8_t *spins,
                  int32_t canonical_steps, int32_t *pt_id) {
+  AllocGuard guard;
   if (!c || !p || !temps || !seeds || L <= 0 || R <= 0) return fail(c, SADMC_E_ARG, "null argument or empty ladder");
   if (canonical_steps <= 0) return fail(c, SADMC_E_ARG, "canonical_steps must be > 0");
   if (p->precision != SADMC_FP64 || p->refresh != 0 || p->move_kind != SADMC_MOVE_CUBE)
@@ -1267,20 +1286,20 @@ int sadmc_pt_add(sadmc_ctx *c, const sadmc_set_params *p, int32_t L, int32_t R, 
     lrng[2 * l + 1] = s1;
   }
   HIPCHECK(c, hipSetDevice(c->device));
-  HIPCHECK(c, hipMalloc(&t.d_rep, sizeof(sadmc_replica) * S));
-  HIPCHECK(c, hipMalloc(&t.d_slot_e, sizeof(double) * 2 * S));
-  HIPCHECK(c, hipMalloc(&t.d_slot_of, sizeof(int32_t) * S));
-  HIPCHECK(c, hipMalloc(&t.d_lrng, sizeof(uint64_t) * 2 * L));
+  HIPCHECK(c, guard.alloc(&t.d_rep, sizeof(sadmc_replica) * S));
+  HIPCHECK(c, guard.alloc(&t.d_slot_e, sizeof(double) * 2 * S));
+  HIPCHECK(c, guard.alloc(&t.d_slot_of, sizeof(int32_t) * S));
+  HIPCHECK(c, guard.alloc(&t.d_lrng, sizeof(uint64_t) * 2 * L));
   HIPCHECK(c, hipMemcpyAsync(t.d_rep, rep.data(), sizeof(sadmc_replica) * S, hipMemcpyHostToDevice, c->stream));
   HIPCHECK(c, hipMemcpyAsync(t.d_slot_of, slot_of.data(), sizeof(int32_t) * S, hipMemcpyHostToDevice, c->stream));
   HIPCHECK(c, hipMemcpyAsync(t.d_lrng, lrng.data(), sizeof(uint64_t) * 2 * L, hipMemcpyHostToDevice, c->stream));
   if (t.sys == SYS_ISING) {
-    HIPCHECK(c, hipMalloc(&t.d_spins, sizeof(uint64_t) * S));
+    HIPCHECK(c, guard.alloc(&t.d_spins, sizeof(uint64_t) * S));
     HIPCHECK(c, hipMemcpyAsync(t.d_spins, bits.data(), sizeof(uint64_t) * S, hipMemcpyHostToDevice, c->stream));
   } else {
-    HIPCHECK(c, hipMalloc(&t.d_pos, sizeof(double) * pos.size()));
+    HIPCHECK(c, guard.alloc(&t.d_pos, sizeof(double) * pos.size()));
     HIPCHECK(c, hipMemcpyAsync(t.d_pos, pos.data(), sizeof(double) * pos.size(), hipMemcpyHostToDevice, c->stream));
-    HIPCHECK(c, hipMalloc(&t.d_geom, sizeof(double) * L));
+    HIPCHECK(c, guard.alloc(&t.d_geom, sizeof(double) * L));
     HIPCHECK(c, hipMemcpyAsync(t.d_geom, geom, sizeof(double) * L, hipMemcpyHostToDevice, c->stream));
   }
   SetArgs &a = t.args;
@@ -1315,6 +1334,7 @@ int sadmc_pt_add(sadmc_ctx *c, const sadmc_set_params *p, int32_t L, int32_t R, 
   pa.sc = t.sys == SYS_ISING ? 0.0 : uniform_scale(-p->move_param, p->move_param);
   HIPCHECK(c, launch_pt(t.sys, t.KMAX, a, pa, L, t.threads, 0, 1, c->stream));
   HIPCHECK(c, hipStreamSynchronize(c->stream));
+  guard.release();
   c->pts.push_back(t);
   if (pt_id) *pt_id = (int32_t)c->pts.size() - 1;
   return SADMC_OK;
@@ -1393,6 +1413,7 @@ int sadmc_pt_ladder_rng(sadmc_ctx *c, int32_t pt, uint64_t *out) {
 
 int sadmc_bin_add_set(sadmc_ctx *c, const sadmc_set_params *p, int32_t W, const uint64_t *seeds, const double *geom,
                       const double *coords, const int8_t *spins, int32_t *set_id) {
+  AllocGuard guard;
   if (!c || !p || W <= 0 || !seeds) return fail(c, SADMC_E_ARG, "null argument or no walkers");
   if (p->method != SADMC_SAD && p->method != SADMC_SAMC && p->method != SADMC_WL && p->method != SADMC_INV_T_WL)
     return fail(c, SADMC_E_ARG, "the binning walker runs SAD, SAMC, WL or 1/t-WL");
@@ -1472,17 +1493,17 @@ int sadmc_bin_add_set(sadmc_ctx *c, const sadmc_set_params *p, int32_t W, const 
     s.gamma = 1.0;
   }
   HIPCHECK(c, hipSetDevice(c->device));
-  HIPCHECK(c, hipMalloc(&b.d_ws, sizeof(sadmc_bwalker_state) * W));
-  HIPCHECK(c, hipMalloc(&b.d_bins, sizeof(sadmc_bbin) * (size_t)W * b.cap));
-  HIPCHECK(c, hipMalloc(&b.d_flag, sizeof(int32_t)));
+  HIPCHECK(c, guard.alloc(&b.d_ws, sizeof(sadmc_bwalker_state) * W));
+  HIPCHECK(c, guard.alloc(&b.d_bins, sizeof(sadmc_bbin) * (size_t)W * b.cap));
+  HIPCHECK(c, guard.alloc(&b.d_flag, sizeof(int32_t)));
   HIPCHECK(c, hipMemsetAsync(b.d_bins, 0, sizeof(sadmc_bbin) * (size_t)W * b.cap, c->stream));
   HIPCHECK(c, hipMemsetAsync(b.d_flag, 0, sizeof(int32_t), c->stream));
   HIPCHECK(c, hipMemcpyAsync(b.d_ws, ws.data(), sizeof(sadmc_bwalker_state) * W, hipMemcpyHostToDevice, c->stream));
   if (b.sys == SYS_ISING) {
-    HIPCHECK(c, hipMalloc(&b.d_spins, sizeof(uint64_t) * W));
+    HIPCHECK(c, guard.alloc(&b.d_spins, sizeof(uint64_t) * W));
     HIPCHECK(c, hipMemcpyAsync(b.d_spins, bits.data(), sizeof(uint64_t) * W, hipMemcpyHostToDevice, c->stream));
   } else {
-    HIPCHECK(c, hipMalloc(&b.d_pos, sizeof(double) * pos.size()));
+    HIPCHECK(c, guard.alloc(&b.d_pos, sizeof(double) * pos.size()));
     HIPCHECK(c, hipMemcpyAsync(b.d_pos, pos.data(), sizeof(double) * pos.size(), hipMemcpyHostToDevice, c->stream));
   }
   SetArgs &a = b.args;
@@ -1521,6 +1542,7 @@ int sadmc_bin_add_set(sadmc_ctx *c, const sadmc_set_params *p, int32_t W, const 
   ba.linear = p->binning == SADMC_BIN_LINEAR;
   HIPCHECK(c, launch_binning(b.sys, b.KMAX, ba, a, 0, 1, c->stream));
   HIPCHECK(c, hipStreamSynchronize(c->stream));
+  guard.release();
   c->bsets.push_back(b);
   if (set_id) *set_id = (int32_t)c->bsets.size() - 1;
   return SADMC_OK;
@@ -1669,6 +1691,7 @@ struct HostSys {
 int sadmc_zmc_add(sadmc_ctx *c, const sadmc_set_params *p, int32_t S, const uint64_t *seeds, const double *geom,
                   const double *coords, const int8_t *spins, double min_T, int64_t isbnb, int32_t RM,
                   int32_t *zmc_id) {
+  AllocGuard guard;
   if (!c || !p || S <= 0 || !seeds) return fail(c, SADMC_E_ARG, "null argument or no runs");
   if (RM < 2) return fail(c, SADMC_E_ARG, "max_replicas must be >= 2");
   if (p->precision != SADMC_FP64 || p->refresh != 0 || p->move_kind != SADMC_MOVE_CUBE)
@@ -1791,23 +1814,23 @@ int sadmc_zmc_add(sadmc_ctx *c, const sadmc_set_params *p, int32_t S, const uint
   for (int s = 0; s < S; s++)
     if (init_failed[s]) return fail(c, SADMC_E_INIT, "initial configuration failed");
   HIPCHECK(c, hipSetDevice(c->device));
-  HIPCHECK(c, hipMalloc(&z.d_sim, sizeof(ZSim) * S));
-  HIPCHECK(c, hipMalloc(&z.d_rep, sizeof(sadmc_zreplica) * SR));
-  HIPCHECK(c, hipMalloc(&z.d_slot, sizeof(double) * 3 * SR));
-  HIPCHECK(c, hipMalloc(&z.d_med, sizeof(double) * med.size()));
-  HIPCHECK(c, hipMalloc(&z.d_slot_of, sizeof(int32_t) * SR));
+  HIPCHECK(c, guard.alloc(&z.d_sim, sizeof(ZSim) * S));
+  HIPCHECK(c, guard.alloc(&z.d_rep, sizeof(sadmc_zreplica) * SR));
+  HIPCHECK(c, guard.alloc(&z.d_slot, sizeof(double) * 3 * SR));
+  HIPCHECK(c, guard.alloc(&z.d_med, sizeof(double) * med.size()));
+  HIPCHECK(c, guard.alloc(&z.d_slot_of, sizeof(int32_t) * SR));
   HIPCHECK(c, hipMemcpyAsync(z.d_sim, sim.data(), sizeof(ZSim) * S, hipMemcpyHostToDevice, c->stream));
   HIPCHECK(c, hipMemcpyAsync(z.d_rep, rep.data(), sizeof(sadmc_zreplica) * SR, hipMemcpyHostToDevice, c->stream));
   HIPCHECK(c, hipMemcpyAsync(z.d_slot, slot.data(), sizeof(double) * 3 * SR, hipMemcpyHostToDevice, c->stream));
   HIPCHECK(c, hipMemcpyAsync(z.d_med, med.data(), sizeof(double) * med.size(), hipMemcpyHostToDevice, c->stream));
   HIPCHECK(c, hipMemcpyAsync(z.d_slot_of, slot_of.data(), sizeof(int32_t) * SR, hipMemcpyHostToDevice, c->stream));
   if (z.sys == SYS_ISING) {
-    HIPCHECK(c, hipMalloc(&z.d_spins, sizeof(uint64_t) * SR));
+    HIPCHECK(c, guard.alloc(&z.d_spins, sizeof(uint64_t) * SR));
     HIPCHECK(c, hipMemcpyAsync(z.d_spins, bits.data(), sizeof(uint64_t) * SR, hipMemcpyHostToDevice, c->stream));
   } else {
-    HIPCHECK(c, hipMalloc(&z.d_pos, sizeof(double) * pos.size()));
+    HIPCHECK(c, guard.alloc(&z.d_pos, sizeof(double) * pos.size()));
     HIPCHECK(c, hipMemcpyAsync(z.d_pos, pos.data(), sizeof(double) * pos.size(), hipMemcpyHostToDevice, c->stream));
-    HIPCHECK(c, hipMalloc(&z.d_geom, sizeof(double) * S));
+    HIPCHECK(c, guard.alloc(&z.d_geom, sizeof(double) * S));
     HIPCHECK(c, hipMemcpyAsync(z.d_geom, geom, sizeof(double) * S, hipMemcpyHostToDevice, c->stream));
   }
   HIPCHECK(c, hipStreamSynchronize(c->stream));
@@ -1840,6 +1863,7 @@ int sadmc_zmc_add(sadmc_ctx *c, const sadmc_set_params *p, int32_t S, const uint
   za.isbnb = isbnb > 0 ? (uint64_t)isbnb : 64;
   za.min_T = min_T;
   za.shrink = std::pow(0.5, 1.0 / dim);
+  guard.release();
   c->zmcs.push_back(z);
   if (zmc_id) *zmc_id = (int32_t)c->zmcs.size() - 1;
   return SADMC_OK;
