@@ -297,16 +297,20 @@ int sadmc_bin_coords(sadmc_ctx *ctx, int32_t set, int32_t w, double *coords);
 int sadmc_bin_spins(sadmc_ctx *ctx, int32_t set, int32_t w, int8_t *spins);
 
 /* ---- energy-ceiling replicas (src/mc/energy_replicas.rs, "ZMC") ---------
- * `sims` independent runs of Ising or LJ systems (the replicas binary),
+ * `sims` independent runs of Ising, LJ or WCA systems (the replicas binary;
+ * geom: the LJ sphere radius or the WCA box length of each run),
  * each on one workgroup with up to max_replicas replicas (replicas * lanes
  * <= 512).  from_params (energy_replicas.rs:344-395) runs on the host when
  * the set is added: 2^15 randomizations with seed_from_u64(seeds[s]), the
  * replicas [inf, E(1/2)] and [E(1/2), E(1/4)], the generator jumped, the
  * median estimator at E(1/4).  A round is MC::run_once
- * (energy_replicas.rs:504-641) without plugins: N (LJ) or N^2 (Ising)
+ * (energy_replicas.rs:504-641) without plugins: N (LJ, WCA) or N^2 (Ising)
  * moves per bounded replica, accepted iff E' < max_energy, cube moves of
- * half-width translation_scale (max_size at first: the radius, or 0.5), a
- * randomization for the unbounded replica; neighbour swaps when the lower
+ * half-width translation_scale (max_size at first: the radius, the box
+ * diagonal, or 0.5), a randomization for the unbounded replica (WCA's adds
+ * the atoms back one by one through its error estimate, wca.rs:251-267);
+ * WCA replicas collect the pressure every N^2 moves while above their cutoff
+ * (the "pressure" above_extra, wca.rs:202-218); neighbour swaps when the lower
  * system fits under the upper ceiling; the median estimator; a new replica
  * after `isbnb` (<= 0: 64) unique visitors when the mean below the cutoff is
  * min_T under it; occasional translation-scale updates.  The lowest replica
@@ -322,6 +326,8 @@ typedef struct {
   uint64_t rng[2];
   double translation_scale;
   double E, error; /* the replica's system */
+  double pressure_total;   /* above_extra["pressure"]: total and count (WCA) */
+  uint64_t pressure_count;
 } sadmc_zreplica;
 
 int sadmc_zmc_add(sadmc_ctx *ctx, const sadmc_set_params *p, int32_t sims, const uint64_t *seeds,

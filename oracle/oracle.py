@@ -517,7 +517,8 @@ class ZRep(C.Structure):
                 ("above_total_squared", C.c_double), ("below_total_squared", C.c_double),
                 ("lowest_max_energy", C.c_double), ("unique_visitors", C.c_uint64), ("collecting_data", C.c_int32),
                 ("reserved", C.c_int32), ("rng", C.c_uint64 * 2), ("translation_scale", C.c_double),
-                ("E", C.c_double), ("error", C.c_double)]
+                ("E", C.c_double), ("error", C.c_double), ("pressure_total", C.c_double),
+                ("pressure_count", C.c_uint64)]
 
 
 class Zmc:

@@ -2377,14 +2377,66 @@ static or_mc *mc_clone(const or_mc *m) {
   return c;
 }
 
+/* The WCA energy of the first m atoms of SoA arrays, in lane-group order. */
+static double or_wca_energy_first(const double *x, const double *y, const double *z, int m, double L, double rc2,
+                                  int G) {
+  double h = 0.5 * L, acc[64];
+  for (int l = 0; l < G; l++) {
+    acc[l] = 0.0;
+    for (int i = l; i < m; i += G)
+      for (int j = 0; j < i; j++)
+        acc[l] += or_wca_pair(norm2(min_image(x[i] - x[j], L, h), min_image(y[i] - y[j], L, h),
+                                    min_image(z[i] - z[j], L, h)),
+                              rc2);
+  }
+  return butterfly(acc, G);
+}
+
 /* System::randomize: ising.rs:79-85 (the low bit of each draw, in site
  * order), lj.rs:262-279 (each atom uniform in the sphere by rejection from
- * Uniform(-1, 1)^3, scaled by the radius).  The energy is recomputed; the
- * error estimate is left as it was. */
+ * Uniform(-1, 1)^3, scaled by the radius), wca.rs:251-267 (below).  The
+ * energy is recomputed; Ising and LJ leave the error estimate as it was. */
 static double sys_randomize(or_mc *m, uint64_t rng[2]) {
   if (m->sys == OR_ISING) {
     for (int64_t k = 0; k < (int64_t)m->n * m->n; k++) m->S[k] = (int8_t)(((int8_t)or_rng_next(rng) & 1) * 2 - 1);
     m->E = or_ising_energy(m->n, m->S);
+  } else if (m->sys == OR_WCA) {
+    /* wca.rs:251-267: every atom removed, then added back one at a time at
+     * put_in_cell(Uniform(0, L)^3); each addition is a Change::Add whose
+     * confirm goes through set_energy (wca.rs:105-116, 164-180) with the
+     * atom count after the addition, so the error estimate (and, when it
+     * trips, an energy over the atoms added so far) evolves from the old
+     * system's E; the energy is recomputed at the end.  dabse, the new atom's
+     * pair terms with the atoms already there, is summed in lane-group order
+     * (the reference sums it in cell-list order). */
+    const double L = m->L, h = m->halfL;
+    const int n = m->n, G = m->G > 0 ? m->G : 1;
+    for (int i = 0; i < n; i++) {
+      double x = put_in_cell1(or_uniform_f64(rng, 0.0, L), L);
+      double y = put_in_cell1(or_uniform_f64(rng, 0.0, L), L);
+      double z = put_in_cell1(or_uniform_f64(rng, 0.0, L), L);
+      double acc[64];
+      for (int l = 0; l < G; l++) acc[l] = 0.0;
+      for (int j = 0; j < i; j++)
+        acc[j % G] += or_wca_pair(norm2(min_image(m->x[j] - x, L, h), min_image(m->y[j] - y, L, h),
+                                        min_image(m->z[j] - z, L, h)),
+                                  m->rc2);
+      double dabse = butterfly(acc, G);
+      m->x[i] = x;
+      m->y[i] = y;
+      m->z[i] = z;
+      double new_e = m->E + dabse;
+      double cnt = (double)(i + 1);
+      double single = dabse > fabs(new_e) ? 1e-14 * dabse * cnt : 1e-14 * fabs(new_e) * cnt;
+      m->error += single * cnt;
+      if (m->error > fabs(new_e) * 1e-13 * cnt * cnt) {
+        m->E = or_wca_energy_first(m->x, m->y, m->z, i + 1, L, m->rc2, G);
+        m->error = 1e-15 * m->E * cnt;
+      } else {
+        m->E = new_e;
+      }
+    }
+    m->E = sys_compute_energy(m, m->G);
   } else {
     double R = m->p.radius;
     for (int i = 0; i < m->n; i++) {
@@ -2416,7 +2468,12 @@ struct or_zmc {
   int med_len;
 };
 
-static double zmc_max_size(const or_mc *m) { return m->sys == OR_ISING ? 0.5 : m->p.radius; }
+/* max_size: ising.rs:120-122, lj.rs:375-377, wca.rs:354-356. */
+static double zmc_max_size(const or_mc *m) {
+  if (m->sys == OR_ISING) return 0.5;
+  if (m->sys == OR_WCA) return sqrt(norm2(m->L, m->L, m->L));
+  return m->p.radius;
+}
 static uint64_t zmc_dim(const or_mc *m) { return m->sys == OR_ISING ? sys_moves_to_randomize(m) : 3 * sys_moves_to_randomize(m); }
 
 /* Replica::new (energy_replicas.rs:145-168). */
@@ -2477,7 +2534,26 @@ or_zmc *or_zmc_new(const or_params *p, uint64_t seed, double min_T, uint64_t isb
 
 /* Replica::run_once (energy_replicas.rs:200-245); LJ and Ising collect no
  * extra data. */
-static void zrep_run_once(or_zrep *r, or_mc *m, double very_lowest_max) {
+/* wca.rs:202-218: data_to_collect(iter) is the pressure every N^2 moves
+ * (all ordered pairs, lane-group order). */
+static int zmc_pressure(const or_mc *m, uint64_t iter, double *val) {
+  if (m->sys != OR_WCA) return 0;
+  if (iter % ((uint64_t)m->n * (uint64_t)m->n) != 0) return 0;
+  double acc[64], L = m->L, h = m->halfL;
+  int n = m->n, G = m->G > 0 ? m->G : 1;
+  for (int l = 0; l < G; l++) {
+    acc[l] = 0.0;
+    for (int i = l; i < n; i += G)
+      for (int j = 0; j < n; j++)
+        if (j != i)
+          acc[l] += wca_pair_pressure(norm2(min_image(m->x[i] - m->x[j], L, h), min_image(m->y[i] - m->y[j], L, h),
+                                            min_image(m->z[i] - m->z[j], L, h)), m->rc2);
+  }
+  *val = butterfly(acc, G) / (3.0 * (L * L * L));
+  return 1;
+}
+
+static void zrep_run_once(or_zrep *r, or_mc *m, double very_lowest_max, uint64_t iter) {
   m->rng[0] = r->rng[0];
   m->rng[1] = r->rng[1];
   if (isfinite(r->max_energy)) {
@@ -2504,6 +2580,11 @@ static void zrep_run_once(or_zrep *r, or_mc *m, double very_lowest_max) {
       r->above_count += 1;
       r->above_total += e;
       r->above_total_squared += e * e;
+      double pv;
+      if (zmc_pressure(m, iter, &pv)) {  /* above_extra["pressure"] */
+        r->pressure_total += pv;
+        r->pressure_count += 1;
+      }
     } else {
       r->below_count += 1;
       r->below_total += e;
@@ -2547,6 +2628,10 @@ static void zrep_decimate(or_zrep *r) {
     r->below_total_squared /= (double)r->below_count;
     r->below_count = 1;
   }
+  if (r->pressure_count > 1) {
+    r->pressure_total /= (double)r->pressure_count;
+    r->pressure_count = 1;
+  }
   r->lowest_max_energy = r->max_energy;
   r->accepted_count = 1;
   r->rejected_count = 1;
@@ -2578,9 +2663,9 @@ void or_zmc_run(or_zmc *z, uint64_t rounds) {
     for (int k = 0; k < z->R; k++) {
       these += steps;
       if (isfinite(z->r[k].max_energy)) {
-        for (uint64_t i = 0; i < steps; i++) zrep_run_once(&z->r[k], z->sys[k], lowest);
+        for (uint64_t i = 0; i < steps; i++) zrep_run_once(&z->r[k], z->sys[k], lowest, z->moves + i);
       } else {
-        zrep_run_once(&z->r[k], z->sys[k], lowest);
+        zrep_run_once(&z->r[k], z->sys[k], lowest, z->moves);
       }
     }
     int first = (or_rng_next(z->rng) >> 63) ? 0 : 1;

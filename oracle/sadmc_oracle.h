@@ -256,7 +256,7 @@ void or_bmc_spins(const or_bmc *b, int8_t *spins);
 void or_bmc_free(or_bmc *b);
 
 /* ---- energy-ceiling replicas (src/mc/energy_replicas.rs, "ZMC") --------
- * Ising or LJ systems.  from_params (energy_replicas.rs:344-395): 2^15
+ * Ising, LJ or WCA systems.  from_params (energy_replicas.rs:344-395): 2^15
  * randomizations of the system with the generator seed_from_u64(seed), a
  * second randomized system for the unbounded replica, the starting system
  * randomized until its energy is below the 2^14-th sampled energy (taken
@@ -279,6 +279,8 @@ typedef struct {
   uint64_t rng[2];
   double translation_scale;
   double E, error;  /* the replica's system */
+  double pressure_total;   /* above_extra["pressure"] (WCA): total and count */
+  uint64_t pressure_count;
 } or_zrep;
 typedef struct or_zmc or_zmc;
 or_zmc *or_zmc_new(const or_params *p, uint64_t seed, double min_T, uint64_t isbnb, const double *coords,

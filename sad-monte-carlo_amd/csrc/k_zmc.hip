@@ -2,10 +2,12 @@
 // run, one lane group per system slot.
 //
 // A round (MC::run_once, energy_replicas.rs:504-641, without plugins):
-// - every replica's group makes its moves at once: N (LJ) or N^2 (Ising)
-//   moves accepted iff E' < max_energy for a bounded replica
+// - every replica's group makes its moves at once: N (LJ, WCA) or N^2
+//   (Ising) moves accepted iff E' < max_energy for a bounded replica
 //   (Replica::run_once, energy_replicas.rs:200-245), one randomization for
-//   the unbounded one, with the replica's generator and counters;
+//   the unbounded one, with the replica's generator and counters; a WCA
+//   replica above its cutoff also collects the pressure on iterations that
+//   are multiples of N^2 (System::data_to_collect, wca.rs:202-218);
 // - lane 0 then walks the neighbour exchanges, feeds the median estimator
 //   and decides whether the lowest replica splits; the split's median is a
 //   selection over the estimator's energies by the whole workgroup; lane 0
@@ -27,6 +29,8 @@ struct ZMeta {
   double max_e, cutoff, ts;
   uint64_t rej, acc, above_c, below_c, upw, uniq;
   double above_t, below_t, above_sq, below_sq;
+  double p_t;    // above_extra["pressure"]: total and count
+  uint64_t p_c;
   int32_t collecting, pad;
 };
 
@@ -43,6 +47,19 @@ __device__ __forceinline__ void z_collect(ZMeta &m, double E, double low, double
     m.below_sq += E * E;
   }
   if (low == very_lowest) m.upw += 1;
+}
+
+// The WCA energy of atoms [0, m) in lane-group order: lane l sums the pairs
+// (i, j < i) of its atoms i = l, l + G, ... < m.
+template <int SYS, int KMAX>
+__device__ double wca_energy_first(const Engine<SYS, KMAX, 0, M_CAN> &e, int m, double rc2) {
+  double acc = 0.0;
+  for (int i = e.lane; i < m; i += e.G)
+    for (int j = 0; j < i; j++)
+      acc += wca_pair(norm2(min_image(e.lx[i] - e.lx[j], e.L, e.halfL), min_image(e.ly[i] - e.ly[j], e.L, e.halfL),
+                            min_image(e.lz[i] - e.lz[j], e.L, e.halfL)),
+                      rc2);
+  return group_sum(acc, e.G);
 }
 
 }  // namespace
@@ -69,6 +86,7 @@ __global__ __launch_bounds__(512) void k_zmc(ZArgs z, SetArgs a, uint64_t rounds
   int32_t *rep_of = slot_of + RM;
   double *red = (double *)(rep_of + RM);  // 40 doubles: selection results and per-wave partials
   int32_t *ctl = (int32_t *)(red + 40);   // R, split, source slot, status
+  uint64_t *smoves = (uint64_t *)(ctl + 4);  // the run's move count at the round's start
   const int64_t S0 = (int64_t)sim * RM;
   double *M = z.med + (int64_t)sim * 4096;
 
@@ -90,6 +108,8 @@ __global__ __launch_bounds__(512) void k_zmc(ZArgs z, SetArgs a, uint64_t rounds
     m.below_t = q.below_total;
     m.above_sq = q.above_total_squared;
     m.below_sq = q.below_total_squared;
+    m.p_t = q.pressure_total;
+    m.p_c = q.pressure_count;
     m.collecting = q.collecting_data;
     m.pad = 0;
     meta[r] = m;
@@ -103,6 +123,7 @@ __global__ __launch_bounds__(512) void k_zmc(ZArgs z, SetArgs a, uint64_t rounds
     ctl[0] = z.sim[sim].R;
     ctl[1] = 0;
     ctl[3] = z.sim[sim].status;
+    *smoves = z.sim[sim].moves;
   }
   __syncthreads();
   for (int r = tid; r < RM; r += nthr) rep_of[slot_of[r]] = r;
@@ -114,6 +135,9 @@ __global__ __launch_bounds__(512) void k_zmc(ZArgs z, SetArgs a, uint64_t rounds
     if (SYS == SYS_LJ) {
       const double g = z.geom[sim];
       e.R2 = g * g;
+    } else if (SYS == SYS_WCA) {
+      e.L = z.geom[sim];
+      e.halfL = 0.5 * e.L;
     }
     e.status = 0;
   }
@@ -128,10 +152,12 @@ __global__ __launch_bounds__(512) void k_zmc(ZArgs z, SetArgs a, uint64_t rounds
   __syncthreads();
 
   const double sc1 = uniform_scale(-1.0, 1.0);
+  const uint64_t nn2 = (uint64_t)a.n * (uint64_t)a.n;
   for (uint64_t round = 0; round < rounds; round++) {
     const int R = ctl[0];
     if (ctl[3]) break;
     const double very_lowest = meta[R - 1].max_e;
+    const uint64_t mv0 = *smoves;
     // ---- every replica's moves
     if (member && slot < R) {
       e.E = sE[slot];
@@ -168,6 +194,10 @@ __global__ __launch_bounds__(512) void k_zmc(ZArgs z, SetArgs a, uint64_t rounds
             m.rej += 1;
           }
           z_collect(m, e.E, low, very_lowest);
+          if (SYS == SYS_WCA && m.collecting && e.E > m.cutoff && (mv0 + (uint64_t)i) % nn2 == 0) {
+            m.p_t += e.pressure();
+            m.p_c += 1;
+          }
         }
       } else {
         // System::randomize: ising.rs:79-85, lj.rs:262-279
@@ -177,6 +207,40 @@ __global__ __launch_bounds__(512) void k_zmc(ZArgs z, SetArgs a, uint64_t rounds
           for (int k = 0; k < nn; k++)
             if (rng_next(e.s0, e.s1) & 1ULL) bits |= 1ULL << k;
           e.spins = bits;
+        } else if (SYS == SYS_WCA) {
+          // wca.rs:251-267: the atoms are added back one at a time at
+          // put_in_cell(Uniform(0, L)^3); each addition goes through
+          // set_energy (wca.rs:164-180) with the atom count after it, so
+          // the error estimate runs on from the old system's; dabse sums
+          // the new atom's pairs with the atoms already placed, lane-group
+          // order
+          const double Lb = e.L, scL = uniform_scale(0.0, Lb);
+          for (int i = 0; i < e.n; i++) {
+            const double x = put_in_cell(uniform_sample(e.s0, e.s1, 0.0, scL), Lb);
+            const double y = put_in_cell(uniform_sample(e.s0, e.s1, 0.0, scL), Lb);
+            const double zz = put_in_cell(uniform_sample(e.s0, e.s1, 0.0, scL), Lb);
+            double acc = 0.0;
+            for (int j = lane; j < i; j += G)
+              acc += wca_pair(norm2(min_image(e.lx[j] - x, Lb, e.halfL), min_image(e.ly[j] - y, Lb, e.halfL),
+                                    min_image(e.lz[j] - zz, Lb, e.halfL)),
+                              a.rc2);
+            const double dabse = group_sum(acc, G);
+            if ((i & (G - 1)) == lane) {
+              e.lx[i] = x;
+              e.ly[i] = y;
+              e.lz[i] = zz;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            const double new_e = e.E + dabse, cnt = (double)(i + 1);
+            const double single = dabse > fabs(new_e) ? 1e-14 * dabse * cnt : 1e-14 * fabs(new_e) * cnt;
+            e.err += single * cnt;
+            if (e.err > fabs(new_e) * 1e-13 * cnt * cnt) {
+              e.E = wca_energy_first(e, i + 1, a.rc2);
+              e.err = 1e-15 * e.E * cnt;
+            } else {
+              e.E = new_e;
+            }
+          }
         } else {
           const double Rr = z.geom[sim];
           for (int i = 0; i < e.n; i++) {
@@ -208,6 +272,10 @@ __global__ __launch_bounds__(512) void k_zmc(ZArgs z, SetArgs a, uint64_t rounds
         e.E = e.full_energy();
         low = m.max_e;
         z_collect(m, e.E, low, very_lowest);
+        if (SYS == SYS_WCA && m.collecting && e.E > m.cutoff && mv0 % nn2 == 0) {
+          m.p_t += e.pressure();
+          m.p_c += 1;
+        }
       }
       if (lane == 0) {
         m.rng0 = e.s0;
@@ -326,6 +394,10 @@ __global__ __launch_bounds__(512) void k_zmc(ZArgs z, SetArgs a, uint64_t rounds
           nr.below_sq /= (double)nr.below_c;
           nr.below_c = 1;
         }
+        if (nr.p_c > 1) {
+          nr.p_t /= (double)nr.p_c;
+          nr.p_c = 1;
+        }
         nr.acc = 1;
         nr.rej = 1;
         nr.uniq = 1;
@@ -365,6 +437,7 @@ __global__ __launch_bounds__(512) void k_zmc(ZArgs z, SetArgs a, uint64_t rounds
         }
       }
       moves += (uint64_t)R * (uint64_t)z.steps;
+      *smoves = moves;
       ctl[1] = 0;
     }
     __syncthreads();
@@ -396,6 +469,8 @@ __global__ __launch_bounds__(512) void k_zmc(ZArgs z, SetArgs a, uint64_t rounds
     q.translation_scale = m.ts;
     q.E = sE[s];
     q.error = sErr[s];
+    q.pressure_total = m.p_t;
+    q.pressure_count = m.p_c;
     z.slot_of[S0 + r] = s;
     z.slot[3 * (S0 + r)] = sE[r];
     z.slot[3 * (S0 + r) + 1] = sErr[r];
@@ -425,6 +500,14 @@ static hipError_t launch_zmc_t(const ZArgs &z, const SetArgs &a, int sims, int t
 hipError_t launch_zmc(int sys, int kmax, const ZArgs &z, const SetArgs &a, int sims, int threads, uint64_t rounds,
                       hipStream_t s) {
   if (sys == SYS_ISING) return launch_zmc_t<SYS_ISING, 1>(z, a, sims, threads, rounds, s);
+  if (sys == SYS_WCA) switch (kmax) {
+      case 1: return launch_zmc_t<SYS_WCA, 1>(z, a, sims, threads, rounds, s);
+      case 2: return launch_zmc_t<SYS_WCA, 2>(z, a, sims, threads, rounds, s);
+      case 4: return launch_zmc_t<SYS_WCA, 4>(z, a, sims, threads, rounds, s);
+      case 8: return launch_zmc_t<SYS_WCA, 8>(z, a, sims, threads, rounds, s);
+      case 16: return launch_zmc_t<SYS_WCA, 16>(z, a, sims, threads, rounds, s);
+      default: return hipErrorInvalidValue;
+    }
   if (sys != SYS_LJ) return hipErrorInvalidValue;
   switch (kmax) {
     case 1: return launch_zmc_t<SYS_LJ, 1>(z, a, sims, threads, rounds, s);

@@ -1629,10 +1629,35 @@ namespace {
 // energy summed in the device's lane-group order.
 struct HostSys {
   int sys = 0, n = 0, G = 1;
-  double radius = 0.0;
+  double radius = 0.0;     // LJ sphere radius, or the WCA box length
+  double rc2 = 0.0;        // WCA cutoff squared
   std::vector<double> c;   // 3n
   std::vector<int8_t> s;   // n*n
-  double E = 0.0;
+  double E = 0.0, err = 0.0;
+
+  // lane l sums the pairs (i, j < i) of its atoms i = l, l + G, ... < m; the
+  // lanes meet in an xor butterfly
+  double pair_energy(int m) const {
+    const double *x = c.data(), *y = x + n, *z = x + 2 * n;
+    const double L = radius, h = 0.5 * radius;
+    double acc[64];
+    for (int l = 0; l < G; l++) acc[l] = 0.0;
+    for (int i = 0; i < m; i++)
+      for (int j = 0; j < i; j++)
+        acc[i % G] += sys == SYS_LJ ? lj_pair(norm2(x[i] - x[j], y[i] - y[j], z[i] - z[j]))
+                                    : wca_pair(norm2(min_image(x[i] - x[j], L, h), min_image(y[i] - y[j], L, h),
+                                                     min_image(z[i] - z[j], L, h)),
+                                               rc2);
+    return butterfly(acc);
+  }
+  double butterfly(double *acc) const {
+    for (int d = 1; d < G; d <<= 1) {
+      double t[64];
+      for (int l = 0; l < G; l++) t[l] = acc[l] + acc[l ^ d];
+      for (int l = 0; l < G; l++) acc[l] = t[l];
+    }
+    return acc[0];
+  }
 
   double energy() const {
     if (sys == SYS_ISING) {
@@ -1647,24 +1672,41 @@ struct HostSys {
         }
       return e;
     }
-    // lane l sums the pairs (i, j < i) of its atoms i = l, l + G, ...; the
-    // lanes meet in an xor butterfly
-    const double *x = c.data(), *y = x + n, *z = x + 2 * n;
-    double acc[64];
-    for (int l = 0; l < G; l++) acc[l] = 0.0;
-    for (int i = 0; i < n; i++)
-      for (int j = 0; j < i; j++) acc[i % G] += lj_pair(norm2(x[i] - x[j], y[i] - y[j], z[i] - z[j]));
-    for (int d = 1; d < G; d <<= 1) {
-      double t[64];
-      for (int l = 0; l < G; l++) t[l] = acc[l] + acc[l ^ d];
-      for (int l = 0; l < G; l++) acc[l] = t[l];
-    }
-    return acc[0];
+    return pair_energy(n);
   }
-  // System::randomize (ising.rs:79-85, lj.rs:262-279)
+  // System::randomize (ising.rs:79-85, lj.rs:262-279, wca.rs:251-267)
   double randomize(uint64_t &s0, uint64_t &s1) {
     if (sys == SYS_ISING) {
       for (int k = 0; k < n * n; k++) s[k] = (int8_t)(((rng_next(s0, s1) & 1ULL) ? 1 : 0) * 2 - 1);
+    } else if (sys == SYS_WCA) {
+      // the atoms are added back one at a time, each through set_energy
+      // (wca.rs:105-116, 164-180) with the atom count after the addition
+      const double L = radius, h = 0.5 * radius, scL = uniform_scale(0.0, L);
+      double *x = c.data(), *y = x + n, *z = x + 2 * n;
+      for (int i = 0; i < n; i++) {
+        const double px = put_in_cell(uniform_sample(s0, s1, 0.0, scL), L);
+        const double py = put_in_cell(uniform_sample(s0, s1, 0.0, scL), L);
+        const double pz = put_in_cell(uniform_sample(s0, s1, 0.0, scL), L);
+        double acc[64];
+        for (int l = 0; l < G; l++) acc[l] = 0.0;
+        for (int j = 0; j < i; j++)
+          acc[j % G] += wca_pair(norm2(min_image(x[j] - px, L, h), min_image(y[j] - py, L, h),
+                                       min_image(z[j] - pz, L, h)),
+                                 rc2);
+        const double dabse = butterfly(acc);
+        x[i] = px;
+        y[i] = py;
+        z[i] = pz;
+        const double new_e = E + dabse, cnt = (double)(i + 1);
+        const double single = dabse > std::fabs(new_e) ? 1e-14 * dabse * cnt : 1e-14 * std::fabs(new_e) * cnt;
+        err += single * cnt;
+        if (err > std::fabs(new_e) * 1e-13 * cnt * cnt) {
+          E = pair_energy(i + 1);
+          err = 1e-15 * E * cnt;
+        } else {
+          E = new_e;
+        }
+      }
     } else {
       const double sc = uniform_scale(-1.0, 1.0);
       double *x = c.data(), *y = x + n, *z = x + 2 * n;
@@ -1704,10 +1746,14 @@ int sadmc_zmc_add(sadmc_ctx *c, const sadmc_set_params *p, int32_t S, const uint
   if (p->system == SADMC_ISING) {
     if (n < 2 || n > 8) return fail(c, SADMC_E_ARG, "Ising lattice width must be 2..8");
     z.sys = SYS_ISING;
-  } else if (p->system == SADMC_LJ) {
+  } else if (p->system == SADMC_LJ || p->system == SADMC_WCA) {
     if (n < 2 || n > 1024) return fail(c, SADMC_E_ARG, "atom count must be 2..1024");
-    if (!geom) return fail(c, SADMC_E_ARG, "LJ runs need a sphere radius per run");
-    z.sys = SYS_LJ;
+    if (!geom) return fail(c, SADMC_E_ARG, "LJ/WCA runs need a sphere radius or box length per run");
+    z.sys = p->system == SADMC_LJ ? SYS_LJ : SYS_WCA;
+    if (z.sys == SYS_WCA)
+      for (int s = 0; s < S; s++)
+        if (!(geom[s] > 0.0) || geom[s] * geom[s] < p->rc2)
+          return fail(c, SADMC_E_ARG, "each WCA box must be at least the cutoff wide");
     int G = p->lanes > 0 ? p->lanes : auto_lanes(n);
     if (G < 2 || G > 64 || (G & (G - 1))) return fail(c, SADMC_E_ARG, "lanes must be a power of two in 2..64");
     z.G = G;
@@ -1716,7 +1762,7 @@ int sadmc_zmc_add(sadmc_ctx *c, const sadmc_set_params *p, int32_t S, const uint
     z.KMAX = z.K <= 1 ? 1 : z.K <= 2 ? 2 : z.K <= 4 ? 4 : z.K <= 8 ? 8 : 16;
     z.npad = G * z.K;
   } else {
-    return fail(c, SADMC_E_ARG, "energy-ceiling replicas are implemented for Ising and LJ systems");
+    return fail(c, SADMC_E_ARG, "unknown system");
   }
   if ((int64_t)RM * z.G > 512) return fail(c, SADMC_E_ARG, "max_replicas * lanes must be <= 512");
   if (zmc_lds_bytes(RM, z.npad) > 65536) return fail(c, SADMC_E_ARG, "a run's systems do not fit 64 KiB of LDS");
@@ -1742,6 +1788,7 @@ int sadmc_zmc_add(sadmc_ctx *c, const sadmc_set_params *p, int32_t S, const uint
     sys.n = n;
     sys.G = z.G;
     sys.radius = geom ? geom[s] : 0.0;
+    sys.rc2 = p->rc2;
     if (z.sys == SYS_ISING) {
       sys.s.resize((size_t)n * n);
       if (spins)
@@ -1752,7 +1799,7 @@ int sadmc_zmc_add(sadmc_ctx *c, const sadmc_set_params *p, int32_t S, const uint
       sys.c.resize((size_t)3 * n);
       if (coords)
         std::memcpy(sys.c.data(), coords + (size_t)s * 3 * n, sizeof(double) * 3 * n);
-      else if (sadmc_init_coords(SADMC_LJ, n, sys.radius, seeds[s], sys.c.data())) {
+      else if (sadmc_init_coords(p->system, n, sys.radius, seeds[s], sys.c.data())) {
         init_failed[s] = 1;
         return;
       }
@@ -1766,7 +1813,10 @@ int sadmc_zmc_add(sadmc_ctx *c, const sadmc_set_params *p, int32_t S, const uint
     while (sys.E > energies[MAX_INIT / 2]) sys.randomize(s0, s1);
     std::sort(energies.begin(), energies.end());
     const double e2 = energies[MAX_INIT / 2], e4 = energies[MAX_INIT / 4];
-    const double max_size = z.sys == SYS_ISING ? 0.5 : sys.radius;
+    // max_size: ising.rs:120-122, lj.rs:375-377, wca.rs:354-356
+    const double max_size = z.sys == SYS_ISING ? 0.5
+                            : z.sys == SYS_LJ  ? sys.radius
+                                               : std::sqrt(norm2(sys.radius, sys.radius, sys.radius));
     for (int r = 0; r < 2; r++) {  // Replica::new (energy_replicas.rs:145-168)
       sadmc_zreplica &q = rep[(size_t)s * RM + r];
       q.max_energy = r == 0 ? INFINITY : e2;
@@ -1791,7 +1841,7 @@ int sadmc_zmc_add(sadmc_ctx *c, const sadmc_set_params *p, int32_t S, const uint
     for (int r = 0; r < 2; r++) {
       const size_t w = (size_t)s * RM + r;
       slot[3 * w] = systems[r]->E;
-      slot[3 * w + 1] = 0.0;
+      slot[3 * w + 1] = systems[r]->err;
       slot[3 * w + 2] = rep[w].lowest_max_energy;
       if (z.sys == SYS_ISING) {
         for (int k = 0; k < n * n; k++)
@@ -1846,6 +1896,7 @@ int sadmc_zmc_add(sadmc_ctx *c, const sadmc_set_params *p, int32_t S, const uint
   a.method = SADMC_CANONICAL;
   a.precision = SADMC_FP64;
   a.move_kind = SADMC_MOVE_CUBE;
+  a.rc2 = p->rc2;
   a.width = 1.0;
   a.inv_width = 1.0;
   const uint64_t nn = (uint64_t)n;

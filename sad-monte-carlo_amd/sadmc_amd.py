@@ -83,7 +83,8 @@ ZREPLICA_DTYPE = np.dtype([
     ("above_count", "<u8"), ("below_count", "<u8"), ("upwelling_count", "<u8"), ("above_total", "<f8"),
     ("below_total", "<f8"), ("above_total_squared", "<f8"), ("below_total_squared", "<f8"),
     ("lowest_max_energy", "<f8"), ("unique_visitors", "<u8"), ("collecting_data", "<i4"), ("reserved", "<i4"),
-    ("rng", "<u8", 2), ("translation_scale", "<f8"), ("E", "<f8"), ("error", "<f8"),
+    ("rng", "<u8", 2), ("translation_scale", "<f8"), ("E", "<f8"), ("error", "<f8"), ("pressure_total", "<f8"),
+    ("pressure_count", "<u8"),
 ])
 
 # sadmc_bcounts / sadmc_bwalker_state / sadmc_bbin: walkers of the binning path.

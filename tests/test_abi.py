@@ -43,6 +43,7 @@ int main(void) {
          offsetof(sadmc_walker_state, cap), offsetof(sadmc_set_params, move_kind));
   printf("%zu %zu %zu %zu %zu %zu\n", sizeof(sadmc_replica), sizeof(sadmc_bwalker_state), sizeof(sadmc_bbin),
          sizeof(sadmc_bcounts), offsetof(sadmc_bwalker_state, pressure), offsetof(sadmc_bwalker_state, min_gamma));
+  printf("%zu %zu\n", sizeof(sadmc_zreplica), offsetof(sadmc_zreplica, pressure_count));
   return 0;
 }
 """
@@ -53,7 +54,10 @@ int main(void) {
         subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", exe, c], check=True)
         out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()
     ws, b, sp, cap_off, mk_off = map(int, out[:5])
-    rep, bws, bb, bc, pr_off, mg_off = map(int, out[5:])
+    rep, bws, bb, bc, pr_off, mg_off = map(int, out[5:11])
+    zr, zpc_off = map(int, out[11:])
+    assert zr == S.ZREPLICA_DTYPE.itemsize
+    assert zpc_off == S.ZREPLICA_DTYPE.fields["pressure_count"][1]
     assert rep == S.REPLICA_DTYPE.itemsize
     assert bws == S.BWALKER_STATE_DTYPE.itemsize and bb == S.BBIN_DTYPE.itemsize == 80
     assert bc == S.BCOUNTS_DTYPE.itemsize == 64

@@ -14,12 +14,15 @@ pytestmark = pytest.mark.gpu
 
 FIELDS = ["max_energy", "cutoff_energy", "rejected_count", "accepted_count", "above_count", "below_count",
           "upwelling_count", "above_total", "below_total", "above_total_squared", "below_total_squared",
-          "lowest_max_energy", "unique_visitors", "collecting_data", "translation_scale", "E", "error"]
+          "lowest_max_energy", "unique_visitors", "collecting_data", "translation_scale", "E", "error",
+          "pressure_total", "pressure_count"]
 
 
 def _oracle(sp, seed, geom, min_T, isbnb, coords=None, spins=None):
     import oracle as O
-    op = O.make_params(system=sp.system, n=sp.n, move_param=sp.move_param, radius=geom or 0.0,
+    op = O.make_params(system=sp.system, n=sp.n, move_param=sp.move_param,
+                       radius=(geom or 0.0) if sp.system == S.LJ else 0.0,
+                       box=(geom or 0.0) if sp.system == S.WCA else 0.0, rc2=sp.rc2,
                        order_g=sp.lanes if sp.system != S.ISING else 0, ising_init=sp.ising_init)
     return O.Zmc(op, seed=int(seed), min_T=min_T, isbnb=isbnb, coords=coords, spins=spins)
 
@@ -77,11 +80,30 @@ def test_lj31_eight_lanes(gpu_ctx_factory):
     _compare(ctx, zid, sp, [9], 3.0, 3000, [], 0.05, 16, 4)
 
 
+def test_wca8_runs_bit_exact(gpu_ctx_factory):
+    # WCA's randomization (atoms added back through the error estimate) and
+    # the pressure collected above each cutoff every N^2 moves
+    print("case zmc wca8 S=2 G=4", flush=True)
+    ctx = gpu_ctx_factory()
+    sp = S.make_params(system=S.WCA, n=8, lanes=4, move_param=0.1)
+    seeds = [7, 8]
+    zid = ctx.zmc_add(sp, seeds, geom=3.0, min_T=0.05, isbnb=64, max_replicas=32)
+    _compare(ctx, zid, sp, seeds, 3.0, 6000, [2000], 0.05, 64, 5)
+
+
+def test_wca32_eight_lanes(gpu_ctx_factory):
+    print("case zmc wca32 S=1 G=8", flush=True)
+    ctx = gpu_ctx_factory()
+    sp = S.make_params(system=S.WCA, n=32, lanes=8, move_param=0.1)
+    zid = ctx.zmc_add(sp, [11], geom=4.5, min_T=0.05, isbnb=16, max_replicas=40)
+    _compare(ctx, zid, sp, [11], 4.5, 1500, [], 0.05, 16, 4)
+
+
 def test_zmc_capacity_and_arguments(gpu_ctx_factory):
     print("case zmc capacity and arguments", flush=True)
     ctx = gpu_ctx_factory()
-    with pytest.raises(S.SadmcError, match="Ising and LJ"):
-        ctx.zmc_add(S.make_params(system=S.WCA, n=32), [1], geom=4.0)
+    with pytest.raises(S.SadmcError, match="cutoff wide"):
+        ctx.zmc_add(S.make_params(system=S.WCA, n=32), [1], geom=1.0)
     with pytest.raises(S.SadmcError, match="512"):
         ctx.zmc_add(S.make_params(system=S.LJ, n=13, lanes=16), [1], geom=2.5, max_replicas=64)
     zid = ctx.zmc_add(S.make_params(system=S.ISING, n=4, ising_init=1), [5], max_replicas=4)
