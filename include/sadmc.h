@@ -89,7 +89,9 @@ typedef struct {
   double e_hi_bound;     /* bin window: highest energy a walker may reach */
   int32_t move_kind;     /* sadmc_move_kind */
   int32_t binning;       /* binning-path walkers: sadmc_binning (0 histogram) */
-  double reserved[4];
+  double high_resolution_de; /* binning-path walkers: > 0 keeps the high_resolution side
+                                histogram of this width (energy_binning.rs:63, 125) */
+  double reserved[3];
 } sadmc_set_params;
 
 /* Device-resident per-walker state; `sadmc_download_raw` returns an array of
@@ -295,6 +297,26 @@ int sadmc_bin_state(sadmc_ctx *ctx, int32_t set, int32_t w, sadmc_bwalker_state 
 int sadmc_bin_bins(sadmc_ctx *ctx, int32_t set, int32_t w, int64_t max_bins, sadmc_bbin *out);
 int sadmc_bin_coords(sadmc_ctx *ctx, int32_t set, int32_t w, double *coords);
 int sadmc_bin_spins(sadmc_ctx *ctx, int32_t set, int32_t w, int8_t *spins);
+
+/* EnergyMC::high_resolution (energy_binning.rs:125, 328-330, 568-570): a
+ * histogram::Bins of width high_resolution_de made by Bins::new at the
+ * walker's starting energy, and incremented with gamma 0 beside every ln w
+ * update.  Its ln w totals stay 0 (so min_total = max_total = 0 and
+ * e_max_total = -inf throughout); its counts and count statistics are kept
+ * here.  Bins span the same energy window as the walker's own. */
+typedef struct {
+  double min, width;    /* histogram::Bins min and width */
+  double min_e, max_e;  /* lowest and highest energy seen */
+  int64_t len;          /* logical number of bins */
+  int64_t base;         /* physical slot of logical bin 0 */
+  uint64_t total_count, min_count, max_count;
+  double e_max_count;
+} sadmc_hires;
+
+/* Walker w's side histogram: its statistics, and (counts non-null) the
+ * counts of logical bins [0, len).  SADMC_E_ARG when the set was added
+ * without high_resolution_de; SADMC_E_CAPACITY if max_bins < len. */
+int sadmc_bin_hires(sadmc_ctx *ctx, int32_t set, int32_t w, sadmc_hires *out, int64_t max_bins, uint64_t *counts);
 
 /* ---- energy-ceiling replicas (src/mc/energy_replicas.rs, "ZMC") ---------
  * `sims` independent runs of Ising, LJ or WCA systems (the replicas binary;

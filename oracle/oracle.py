@@ -27,7 +27,7 @@ class Params(C.Structure):
         ("refresh", C.c_int32), ("order_g", C.c_int32), ("ising_init", C.c_int32),
         ("collect_pressure", C.c_int32), ("seed", C.c_uint64),
         ("move_kind", C.c_int32), ("drop_hastings", C.c_int32), ("mlp", C.c_void_p),
-        ("binning", C.c_int32),
+        ("binning", C.c_int32), ("high_resolution_de", C.c_double),
     ]
 
 
@@ -144,6 +144,7 @@ def lib():
     L.or_bmc_coords.argtypes = [vp, vp]
     L.or_bmc_spins.argtypes = [vp, vp]
     L.or_bmc_free.argtypes = [vp]
+    L.or_bmc_hires.argtypes = [vp, vp, vp]; L.or_bmc_hires.restype = C.c_int64
     L.or_pt_free.argtypes = [vp]
     L.or_prop_features.argtypes = [C.c_int32, vp, C.c_int32, C.c_double, C.c_double, C.c_double, vp]
     L.or_prop_mlp.argtypes = [vp, vp]; L.or_prop_mlp.restype = C.c_float
@@ -308,6 +309,21 @@ class BWalker:
 
     def count_at(self, e):
         return lib().or_bmc_count_at(self.h, e)
+
+    def hires(self):
+        """The high_resolution side histogram: (statistics dict, counts), or None."""
+        st = np.zeros(10)
+        n = lib().or_bmc_hires(self.h, _ptr(st), None)
+        if n < 0:
+            return None
+        counts = np.zeros(max(n, 1), np.uint64)
+        lib().or_bmc_hires(self.h, _ptr(st), _ptr(counts))
+        keys = ("min", "width", "min_e", "max_e", "len", "total_count", "min_count", "max_count", "e_max_count",
+                "min_total")
+        d = dict(zip(keys, st.tolist()))
+        for k in ("len", "total_count", "min_count", "max_count"):
+            d[k] = int(d[k])
+        return d, counts[:n]
 
     def coords(self):
         c = np.zeros(3 * self.p.n)

@@ -1724,6 +1724,15 @@ struct or_bmc {
   double min_gamma;
   int status;
   int linear;  /* binning/linear.rs instead of binning/histogram.rs */
+  /* EnergyMC::high_resolution (energy_binning.rs:125): a histogram::Bins
+   * of its own width whose ln w BinCounts is incremented with gamma 0 */
+  int has_hr;
+  double hr_min, hr_width, hr_min_e, hr_max_e;
+  int64_t hr_len, hr_cap;
+  double *hr_total;
+  uint64_t *hr_count;
+  double hr_min_total, hr_max_total, hr_e_max_total, hr_e_max_count;
+  uint64_t hr_min_count, hr_max_count, hr_total_count;
 };
 
 /* f64::max / f64::min as folded from NaN (binning/histogram.rs:366-372):
@@ -2107,11 +2116,76 @@ static int bmc_reject(or_bmc *b, double e1, double e2) {
 }
 
 /* energy_binning.rs:320-500. */
+/* The side histogram's Bins::increment_count(e, 0.0): histogram.rs:182-192
+ * (min_e / max_e, prep_for_e at :147-167 with insert_zero / push_zero at
+ * :47-58, energy_to_index at :129-140) and BinCounts::increment_count at
+ * :59-81 with value 0, its rescans included. */
+static int hr_increment(or_bmc *b, double e) {
+  if (e > b->hr_max_e) b->hr_max_e = e;
+  if (e < b->hr_min_e) b->hr_min_e = e;
+  if (b->hr_len == 0) b->hr_min = floor(e / b->hr_width) * b->hr_width;
+  while (e < b->hr_min || e >= b->hr_min + b->hr_width * (double)b->hr_len) {
+    if (b->hr_len + 1 > b->hr_cap) {
+      b->hr_cap = b->hr_cap ? 2 * b->hr_cap : 16;
+      b->hr_total = (double *)realloc(b->hr_total, sizeof(double) * b->hr_cap);
+      b->hr_count = (uint64_t *)realloc(b->hr_count, sizeof(uint64_t) * b->hr_cap);
+    }
+    if (e < b->hr_min) { /* insert_zero */
+      memmove(b->hr_total + 1, b->hr_total, sizeof(double) * (size_t)b->hr_len);
+      memmove(b->hr_count + 1, b->hr_count, sizeof(uint64_t) * (size_t)b->hr_len);
+      b->hr_total[0] = 0.0;
+      b->hr_count[0] = 0;
+      b->hr_min -= b->hr_width;
+    } else { /* push_zero */
+      b->hr_total[b->hr_len] = 0.0;
+      b->hr_count[b->hr_len] = 0;
+    }
+    b->hr_len += 1;
+    b->hr_min_total = 0.0;
+    b->hr_min_count = 0;
+  }
+  double fi = (e - b->hr_min) / b->hr_width;
+  int64_t idx = fi == (double)b->hr_len ? (int64_t)fi - 1 : (int64_t)fi;
+  if (!(e >= b->hr_min) || idx < 0 || idx >= b->hr_len) { /* an index panic in the reference */
+    b->status = ST_INDEX;
+    return 0;
+  }
+  b->hr_total_count += 1;
+  uint64_t old_count = b->hr_count[idx];
+  b->hr_count[idx] += 1;
+  double old_total = b->hr_total[idx];
+  b->hr_total[idx] += 0.0;
+  if (b->hr_total[idx] > b->hr_max_total) {
+    double mx = NAN;
+    for (int64_t i = 0; i < b->hr_len; i++) mx = fold_max(mx, b->hr_total[i]);
+    if (mx == b->hr_total[idx]) {
+      b->hr_max_total = b->hr_total[idx];
+      b->hr_e_max_total = e;
+    }
+  }
+  if (old_total == b->hr_min_total) {
+    double mn = NAN;
+    for (int64_t i = 0; i < b->hr_len; i++) mn = fold_min(mn, b->hr_total[i]);
+    b->hr_min_total = mn;
+  }
+  if (old_count == b->hr_min_count) {
+    uint64_t mn = UINT64_MAX;
+    for (int64_t i = 0; i < b->hr_len; i++) mn = b->hr_count[i] < mn ? b->hr_count[i] : mn;
+    b->hr_min_count = mn;
+  }
+  if (b->hr_count[idx] > b->hr_max_count) {
+    b->hr_max_count = b->hr_count[idx];
+    b->hr_e_max_count = e;
+  }
+  return 1;
+}
+
 static void bmc_update(or_bmc *b, double energy) {
   double gamma = bmc_gamma(b);
   double old_highest = (b->linear ? b->lnw.fmax_count : (double)b->lnw.max_count) / b->width;
   double old_here = bm_get_count(b, energy);
   if (!bmc_increment(b, energy, gamma)) return;
+  if (b->has_hr && !hr_increment(b, energy)) return;
   if (b->method == OR_SAD) {
     double here = bm_get_count(b, energy);
     if (old_here == 0.0 && !bmc_accumulate(b, &b->tf, energy, (double)b->moves)) return;
@@ -2241,6 +2315,15 @@ static or_bmc *bmc_from_system(const or_params *bp, or_mc *m) {
   b->min_e = E;
   b->max_e = E;
   b->len = 0;
+  if (bp->high_resolution_de > 0) { /* Bins::new(system.energy(), de), energy_binning.rs:568-570 */
+    b->has_hr = 1;
+    b->hr_width = bp->high_resolution_de;
+    b->hr_min = (round(E / b->hr_width) - 0.5) * b->hr_width;
+    b->hr_min_e = E;
+    b->hr_max_e = E;
+    b->hr_e_max_total = -INFINITY;
+    b->hr_e_max_count = -INFINITY;
+  }
   bmc_grow(b, 16);
   bc_new(&b->lnw, 0, b->cap);
   b->acc_rate = 0.5;
@@ -2347,8 +2430,28 @@ void or_bmc_free(or_bmc *b) {
   bcounts *all[5] = {&b->lnw, &b->en, &b->tf, &b->hist, &b->pr};
   for (int k = 0; k < 5; k++)
     if (all[k]->exists) bc_free(all[k]);
+  free(b->hr_total);
+  free(b->hr_count);
   or_mc_free(b->m);
   free(b);
+}
+
+int64_t or_bmc_hires(const or_bmc *b, double *stats, uint64_t *counts) {
+  if (!b->has_hr) return -1;
+  if (stats) {
+    stats[0] = b->hr_min;
+    stats[1] = b->hr_width;
+    stats[2] = b->hr_min_e;
+    stats[3] = b->hr_max_e;
+    stats[4] = (double)b->hr_len;
+    stats[5] = (double)b->hr_total_count;
+    stats[6] = (double)b->hr_min_count;
+    stats[7] = (double)b->hr_max_count;
+    stats[8] = b->hr_e_max_count;
+    stats[9] = b->hr_min_total;
+  }
+  if (counts && b->hr_len > 0) memcpy(counts, b->hr_count, sizeof(uint64_t) * (size_t)b->hr_len);
+  return b->hr_len;
 }
 
 /* ===================================================================== */

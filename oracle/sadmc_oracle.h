@@ -80,6 +80,7 @@ typedef struct {
   int32_t drop_hastings;    /* test hook: omit ln H (a planted mistake) */
   const float *mlp;         /* learned move: OR_PROP_NP weights (BF16 values as float) */
   int32_t binning;          /* binning walker: 0 histogram (histogram.rs), 1 linear (linear.rs) */
+  double high_resolution_de; /* binning walker: > 0 keeps the high_resolution side histogram */
 } or_params;
 
 typedef struct {
@@ -254,6 +255,11 @@ double or_bmc_count_at(const or_bmc *b, double e);
 void or_bmc_coords(const or_bmc *b, double *coords);
 void or_bmc_spins(const or_bmc *b, int8_t *spins);
 void or_bmc_free(or_bmc *b);
+/* The high_resolution side histogram (energy_binning.rs:125, 328-330):
+ * stats[10] = min, width, min_e, max_e, len, total_count, min_count,
+ * max_count, e_max_count, min_total (as doubles); counts[len] when non-null.
+ * Returns len, or -1 when the walker keeps none. */
+int64_t or_bmc_hires(const or_bmc *b, double *stats, uint64_t *counts);
 
 /* ---- energy-ceiling replicas (src/mc/energy_replicas.rs, "ZMC") --------
  * Ising, LJ or WCA systems.  from_params (energy_replicas.rs:344-395): 2^15

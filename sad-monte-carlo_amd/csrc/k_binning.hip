@@ -88,6 +88,11 @@ struct BinWalker {
   // (a rescan between the two bins of a linear update would count the
   // second bin's new value twice)
   bool stale_t[5], stale_c[5];
+  // the high_resolution side histogram (energy_binning.rs:125, 328-330):
+  // counts in HR[hr_base + i], and the smallest count's multiplicity
+  uint64_t *HR;
+  sadmc_hires hs;
+  uint64_t hr_ncmin;
 
   SADMC_DF const BinArgs &args() const { return *Bp; }
 
@@ -168,6 +173,12 @@ struct BinWalker {
       stale_c[k] = false;
       if (bc[k].exists) rescan_all(k);
     }
+    HR = nullptr;
+    if (ba.hr) {
+      HR = ba.hr + (int64_t)walker * ba.hr_cap;
+      hs = ba.hrs[walker];
+      hr_rescan();
+    }
   }
 
   SADMC_DF void store(int walker) {
@@ -214,6 +225,7 @@ struct BinWalker {
     bc_store(bc[BC_TF], w.t_found);
     bc_store(bc[BC_HIST], w.hist);
     bc_store(bc[BC_PR], w.pressure);
+    if (HR) args().hrs[walker] = hs;
     if (status) *args().flag = 1;
   }
 
@@ -434,6 +446,101 @@ struct BinWalker {
       ncmin[k] += 1;
     }
     if (k == BC_LNW && !(lmax >= 0.0)) lmax = 0.0;
+  }
+
+  // ---------------------------------------- the high_resolution histogram
+  // The smallest count and its multiplicity (a count never exceeds the
+  // number of moves, so ~0 stands for an empty histogram).
+  SADMC_DF void hr_rescan() {
+    uint64_t m = ~0ULL, c = 0;
+    for (int64_t j0 = lane; j0 < hs.len; j0 += 4 * G) {
+      uint64_t v[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int64_t j = j0 + u * G;
+        v[u] = j < hs.len ? HR[hs.base + j] : ~0ULL;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        if (j0 + u * G >= hs.len) continue;
+        if (v[u] < m) {
+          m = v[u];
+          c = 1;
+        } else if (v[u] == m) {
+          c += 1;
+        }
+      }
+    }
+    for (int d = 1; d < G; d <<= 1) {
+      const uint64_t om = (uint64_t)__shfl_xor((long long)m, d, 64);
+      const uint64_t oc = (uint64_t)__shfl_xor((long long)c, d, 64);
+      if (om < m) {
+        m = om;
+        c = oc;
+      } else if (om == m) {
+        c += oc;
+      }
+    }
+    if (hs.len > 0) hs.min_count = m;
+    hr_ncmin = c;
+  }
+  // insert_zero / push_zero (histogram.rs:47-58) on the side histogram
+  SADMC_DF void hr_new_zero(int64_t p) {
+    HR[p] = 0;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (hs.min_count == 0) {
+      hr_ncmin += 1;
+    } else {
+      hs.min_count = 0;
+      hr_ncmin = 1;
+    }
+    hs.len += 1;
+  }
+  // Bins::increment_count(e, 0.0) (histogram.rs:182-192, 147-167, 59-81):
+  // min_e / max_e, the window grown to hold e, then the bin's count and the
+  // count statistics (the totals stay 0)
+  SADMC_DF bool hr_increment(double en) {
+    if (en > hs.max_e) hs.max_e = en;
+    if (en < hs.min_e) hs.min_e = en;
+    if (hs.len == 0) {
+      hs.min = floor(en / hs.width) * hs.width;
+      const int64_t b0 = (int64_t)floor((hs.min - args().e_lo) / hs.width) + 4;
+      hs.base = b0 < 0 ? 0 : (b0 >= args().hr_cap ? args().hr_cap - 1 : b0);
+    }
+    while (en < hs.min) {
+      if (hs.base == 0) {
+        status = SADMC_WS_CAPACITY;
+        return false;
+      }
+      hs.base -= 1;
+      hr_new_zero(hs.base);
+      hs.min -= hs.width;
+    }
+    while (en >= hs.min + hs.width * (double)hs.len) {
+      if (hs.base + hs.len >= args().hr_cap) {
+        status = SADMC_WS_CAPACITY;
+        return false;
+      }
+      hr_new_zero(hs.base + hs.len);
+    }
+    // histogram.rs:129-140
+    const double fi = (en - hs.min) / hs.width;
+    const int64_t idx = fi == (double)hs.len ? (int64_t)fi - 1 : (int64_t)fi;
+    if (!(en >= hs.min) || idx < 0 || idx >= hs.len) {  // an index panic in the reference
+      status = SADMC_WS_INDEX;
+      return false;
+    }
+    uint64_t &r = HR[hs.base + idx];
+    const uint64_t old_count = r, new_count = old_count + 1;
+    r = new_count;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    hs.total_count += 1;
+    if (old_count == hs.min_count && --hr_ncmin == 0) hr_rescan();
+    if (new_count > hs.max_count) {
+      hs.max_count = new_count;
+      hs.e_max_count = en;
+    }
+    return true;
   }
 
   // ------------------------------------------------------------ growth
@@ -758,6 +865,7 @@ struct BinWalker {
       }
       lnw_increment(en, idx, g);
     }
+    if (HR && !hr_increment(en)) return false;
     if (method == SADMC_SAD) {
       const double here = get_count(en);
       if (old_here == 0.0 && !accumulate(BC_TF, en, (double)moves)) return false;
@@ -887,6 +995,19 @@ struct BinWalker {
     bc[BC_LNW].exists = 1;
     bc[BC_LNW].e_max_total = -__builtin_inf();
     bc[BC_LNW].e_max_count = -__builtin_inf();
+    if (HR) {  // Bins::new(system.energy(), de) (energy_binning.rs:568-570)
+      hs.width = args().hr_de;
+      hs.min = (round(E / hs.width) - 0.5) * hs.width;
+      hs.min_e = E;
+      hs.max_e = E;
+      hs.len = 0;
+      hs.base = 0;
+      hs.total_count = 0;
+      hs.min_count = 0;
+      hs.max_count = 0;
+      hs.e_max_count = -__builtin_inf();
+      hr_ncmin = 0;
+    }
     status = e.status;
   }
 };

@@ -106,6 +106,9 @@ struct BinSetHost {
   double *d_pos = nullptr;
   uint64_t *d_spins = nullptr;
   int32_t *d_flag = nullptr;
+  int64_t hr_cap = 0;            // side histogram bins per walker (0: none)
+  sadmc_hires *d_hrs = nullptr;
+  uint64_t *d_hr = nullptr;
 };
 
 }  // namespace
@@ -388,7 +391,8 @@ void sadmc_destroy(sadmc_ctx *c) {
                     (void *)z.d_geom, (void *)z.d_slot_of, (void *)z.d_spins})
       (void)hipFree(p);
   for (auto &b : c->bsets)
-    for (void *p : {(void *)b.d_ws, (void *)b.d_bins, (void *)b.d_pos, (void *)b.d_spins, (void *)b.d_flag})
+    for (void *p : {(void *)b.d_ws, (void *)b.d_bins, (void *)b.d_pos, (void *)b.d_spins, (void *)b.d_flag,
+                    (void *)b.d_hrs, (void *)b.d_hr})
       (void)hipFree(p);
   for (auto &t : c->pts)
     for (void *p : {(void *)t.d_rep, (void *)t.d_slot_e, (void *)t.d_pos, (void *)t.d_geom, (void *)t.d_slot_of,
@@ -1450,6 +1454,13 @@ int sadmc_bin_add_set(sadmc_ctx *c, const sadmc_set_params *p, int32_t W, const 
   q.energy_bin = width;
   double e_lo, e_hi;
   sadmc_bin_window(&q, geom ? geom[0] : 0.0, &e_lo, &e_hi, &b.cap);
+  if (p->high_resolution_de < 0 || std::isnan(p->high_resolution_de))
+    return fail(c, SADMC_E_ARG, "high_resolution_de must be >= 0");
+  if (p->high_resolution_de > 0) {
+    const double nb = std::ceil((e_hi - e_lo) / p->high_resolution_de) + 8;
+    if (!(nb < 2147483648.0)) return fail(c, SADMC_E_ARG, "high_resolution_de is too fine for the energy window");
+    b.hr_cap = (int64_t)nb;
+  }
   std::vector<uint64_t> bits;
   std::vector<double> pos;
   if (b.sys == SYS_ISING) {
@@ -1499,6 +1510,12 @@ int sadmc_bin_add_set(sadmc_ctx *c, const sadmc_set_params *p, int32_t W, const 
   HIPCHECK(c, hipMemsetAsync(b.d_bins, 0, sizeof(sadmc_bbin) * (size_t)W * b.cap, c->stream));
   HIPCHECK(c, hipMemsetAsync(b.d_flag, 0, sizeof(int32_t), c->stream));
   HIPCHECK(c, hipMemcpyAsync(b.d_ws, ws.data(), sizeof(sadmc_bwalker_state) * W, hipMemcpyHostToDevice, c->stream));
+  if (b.hr_cap) {
+    HIPCHECK(c, guard.alloc(&b.d_hrs, sizeof(sadmc_hires) * W));
+    HIPCHECK(c, guard.alloc(&b.d_hr, sizeof(uint64_t) * (size_t)W * b.hr_cap));
+    HIPCHECK(c, hipMemsetAsync(b.d_hrs, 0, sizeof(sadmc_hires) * W, c->stream));
+    HIPCHECK(c, hipMemsetAsync(b.d_hr, 0, sizeof(uint64_t) * (size_t)W * b.hr_cap, c->stream));
+  }
   if (b.sys == SYS_ISING) {
     HIPCHECK(c, guard.alloc(&b.d_spins, sizeof(uint64_t) * W));
     HIPCHECK(c, hipMemcpyAsync(b.d_spins, bits.data(), sizeof(uint64_t) * W, hipMemcpyHostToDevice, c->stream));
@@ -1540,6 +1557,10 @@ int sadmc_bin_add_set(sadmc_ctx *c, const sadmc_set_params *p, int32_t W, const 
   ba.min_e = p->min_e;
   ba.max_e = p->max_e;
   ba.linear = p->binning == SADMC_BIN_LINEAR;
+  ba.hrs = b.d_hrs;
+  ba.hr = b.d_hr;
+  ba.hr_cap = b.hr_cap;
+  ba.hr_de = p->high_resolution_de;
   HIPCHECK(c, launch_binning(b.sys, b.KMAX, ba, a, 0, 1, c->stream));
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   guard.release();
@@ -1592,6 +1613,21 @@ int sadmc_bin_bins(sadmc_ctx *c, int32_t set, int32_t w, int64_t max_bins, sadmc
   BinSetHost *b = bset_get(c, set);
   if (st.len > 0)
     HIPCHECK(c, hipMemcpyAsync(out, b->d_bins + (int64_t)w * b->cap + st.base, sizeof(sadmc_bbin) * st.len,
+                               hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  return SADMC_OK;
+}
+
+int sadmc_bin_hires(sadmc_ctx *c, int32_t set, int32_t w, sadmc_hires *out, int64_t max_bins, uint64_t *counts) {
+  BinSetHost *b = bset_get(c, set);
+  if (!b || !out || w < 0 || w >= b->W) return fail(c, SADMC_E_ARG, "no such binning walker or null output");
+  if (!b->hr_cap) return fail(c, SADMC_E_ARG, "the set keeps no high_resolution histogram");
+  HIPCHECK(c, hipMemcpyAsync(out, b->d_hrs + w, sizeof *out, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  if (!counts) return SADMC_OK;
+  if (out->len > max_bins) return fail(c, SADMC_E_CAPACITY, "output holds fewer bins than the side histogram has");
+  if (out->len > 0)
+    HIPCHECK(c, hipMemcpyAsync(counts, b->d_hr + (int64_t)w * b->hr_cap + out->base, sizeof(uint64_t) * out->len,
                                hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   return SADMC_OK;

@@ -104,6 +104,12 @@ struct BinArgs {
   int32_t move_plan, has_min_e, has_max_e, collect_pressure;
   double move_param, min_e, max_e;
   int32_t linear;           // binning/linear.rs instead of binning/histogram.rs
+  // the high_resolution side histogram (null when off): per-walker
+  // statistics and [W][hr_cap] counts, bins of width hr_de
+  sadmc_hires *hrs;
+  uint64_t *hr;
+  int64_t hr_cap;
+  double hr_de;
 };
 // `init` runs from_params (energy, descent, method state) instead of moves.
 hipError_t launch_binning(int sys, int kmax, const BinArgs &ba, const SetArgs &sa, uint64_t moves, int init,

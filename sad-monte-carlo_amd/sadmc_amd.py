@@ -48,7 +48,8 @@ class SetParams(C.Structure):
         ("rc2", C.c_double), ("refresh", C.c_int32), ("ising_init", C.c_int32),
         ("collect_pressure", C.c_int32), ("lanes", C.c_int32),
         ("e_lo_bound", C.c_double), ("e_hi_bound", C.c_double),
-        ("move_kind", C.c_int32), ("binning", C.c_int32), ("reserved", C.c_double * 4),
+        ("move_kind", C.c_int32), ("binning", C.c_int32), ("high_resolution_de", C.c_double),
+        ("reserved", C.c_double * 3),
     ]
 
 
@@ -108,6 +109,11 @@ BBIN_DTYPE = np.dtype([
     ("t_found_total", "<f8"), ("t_found_count", "<u8"), ("hist_count", "<u8"), ("pressure_total", "<f8"),
     ("pressure_count", "<u8"), ("reserved", "<u8"),
 ])
+# sadmc_hires: the high_resolution side histogram's statistics.
+HIRES_DTYPE = np.dtype([
+    ("min", "<f8"), ("width", "<f8"), ("min_e", "<f8"), ("max_e", "<f8"), ("len", "<i8"), ("base", "<i8"),
+    ("total_count", "<u8"), ("min_count", "<u8"), ("max_count", "<u8"), ("e_max_count", "<f8"),
+])
 
 class WalkerStateC(C.Structure):
     _fields_ = [(name, C.c_uint8 * WALKER_STATE_DTYPE.itemsize) for name in ("raw",)]
@@ -126,6 +132,7 @@ EXPORTS = [
     "sadmc_prop_train", "sadmc_mean_entropy",
     "sadmc_pt_add", "sadmc_pt_run", "sadmc_pt_replicas", "sadmc_pt_coords", "sadmc_pt_spins", "sadmc_pt_ladder_rng",
     "sadmc_bin_add_set", "sadmc_bin_run", "sadmc_bin_state", "sadmc_bin_bins", "sadmc_bin_coords", "sadmc_bin_spins",
+    "sadmc_bin_hires",
     "sadmc_zmc_add", "sadmc_zmc_run", "sadmc_zmc_replicas", "sadmc_zmc_coords", "sadmc_zmc_spins", "sadmc_zmc_state",
 ]
 
@@ -189,6 +196,7 @@ def lib():
     L.sadmc_bin_bins.argtypes = [vp, i32, i32, i64, vp]
     L.sadmc_bin_coords.argtypes = [vp, i32, i32, vp]
     L.sadmc_bin_spins.argtypes = [vp, i32, i32, vp]
+    L.sadmc_bin_hires.argtypes = [vp, i32, i32, vp, i64, vp]
     L.sadmc_zmc_add.argtypes = [vp, C.POINTER(SetParams), i32, vp, vp, vp, vp, d, i64, i32, C.POINTER(i32)]
     L.sadmc_zmc_run.argtypes = [vp, i32, u64]
     L.sadmc_zmc_replicas.argtypes = [vp, i32, i32, i32, vp, C.POINTER(i32)]
@@ -354,6 +362,15 @@ class Context:
         a = np.zeros(n * n, np.int8)
         self._check(lib().sadmc_bin_spins(self.h, s, w, _p(a)), "bin_spins")
         return a
+
+    def bin_hires(self, s, w):
+        """The high_resolution side histogram of walker w: (statistics, counts)."""
+        st = np.zeros((), HIRES_DTYPE)
+        self._check(lib().sadmc_bin_hires(self.h, s, w, _p(st), 0, None), "bin_hires")
+        n = int(st["len"])
+        counts = np.zeros(max(n, 1), np.uint64)
+        self._check(lib().sadmc_bin_hires(self.h, s, w, _p(st), len(counts), _p(counts)), "bin_hires")
+        return st, counts[:n]
 
     # ---- energy-ceiling replicas (energy_replicas.rs)
     def zmc_add(self, params, seeds, geom=None, coords=None, spins=None, min_T=0.2, isbnb=64, max_replicas=32):

@@ -27,7 +27,8 @@ def oracle_params(sp, seed, geom=0.0, order_g=None):
         energy_bin=sp.energy_bin, has_min_e=sp.has_min_e, has_max_e=sp.has_max_e, min_e=sp.min_e,
         max_e=sp.max_e, radius=geom if sp.system == 1 else 0.0, box=geom if sp.system == 2 else 0.0,
         rc2=sp.rc2, refresh=sp.refresh, order_g=g if sp.system != 0 else 0, ising_init=sp.ising_init,
-        collect_pressure=sp.collect_pressure, binning=sp.binning, seed=int(seed))
+        collect_pressure=sp.collect_pressure, binning=sp.binning, high_resolution_de=sp.high_resolution_de,
+        seed=int(seed))
 
 
 SCALAR_KEYS = [

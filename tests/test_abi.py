@@ -43,7 +43,8 @@ int main(void) {
          offsetof(sadmc_walker_state, cap), offsetof(sadmc_set_params, move_kind));
   printf("%zu %zu %zu %zu %zu %zu\n", sizeof(sadmc_replica), sizeof(sadmc_bwalker_state), sizeof(sadmc_bbin),
          sizeof(sadmc_bcounts), offsetof(sadmc_bwalker_state, pressure), offsetof(sadmc_bwalker_state, min_gamma));
-  printf("%zu %zu\n", sizeof(sadmc_zreplica), offsetof(sadmc_zreplica, pressure_count));
+  printf("%zu %zu %zu %zu\n", sizeof(sadmc_zreplica), offsetof(sadmc_zreplica, pressure_count),
+         sizeof(sadmc_hires), offsetof(sadmc_set_params, high_resolution_de));
   return 0;
 }
 """
@@ -55,7 +56,9 @@ int main(void) {
         out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()
     ws, b, sp, cap_off, mk_off = map(int, out[:5])
     rep, bws, bb, bc, pr_off, mg_off = map(int, out[5:11])
-    zr, zpc_off = map(int, out[11:])
+    zr, zpc_off, hr, hr_off = map(int, out[11:])
+    assert hr == S.HIRES_DTYPE.itemsize == 80
+    assert hr_off == S.SetParams.high_resolution_de.offset
     assert zr == S.ZREPLICA_DTYPE.itemsize
     assert zpc_off == S.ZREPLICA_DTYPE.fields["pressure_count"][1]
     assert rep == S.REPLICA_DTYPE.itemsize

@@ -164,3 +164,65 @@ def test_window_overflow_is_reported(gpu_ctx_factory, binning):
         assert int(st["status"]) == S.WS_CAPACITY
         assert 0 <= int(st["base"]) and int(st["base"]) + int(st["len"]) <= int(st["cap"]) == 9
         assert float(st["E"]) % 4 == 0 and -32 <= float(st["E"]) <= 32
+
+
+# ---- the high_resolution side histogram (energy_binning.rs:125, 328-330)
+
+def _compare_hires(ctx, s, w, orc, label):
+    gs, gc = ctx.bin_hires(s, w)
+    os_, oc = orc.hires()
+    for k in ("min", "width", "min_e", "max_e", "e_max_count"):
+        assert np.float64(gs[k]).tobytes() == np.float64(os_[k]).tobytes(), f"{label} {k}: {gs[k]!r} {os_[k]!r}"
+    for k in ("len", "total_count", "min_count", "max_count"):
+        assert int(gs[k]) == os_[k], f"{label} {k}: {gs[k]!r} {os_[k]!r}"
+    assert os_["min_total"] == 0.0
+    np.testing.assert_array_equal(gc, oc, err_msg=label)
+
+
+HIRES_CASES = [
+    ("ising_sad", dict(system=S.ISING, n=4, ising_init=1, method=S.SAD, method_param=1.0, energy_bin=1.0,
+                       high_resolution_de=2.0), None, [6000, 4000]),
+    ("lj13_samc", dict(system=S.LJ, n=13, lanes=4, method=S.SAMC, method_param=500.0, energy_bin=0.1,
+                       move_param=0.1, has_max_e=1, max_e=0.0, high_resolution_de=0.01), 2.5, [8000, 4000]),
+    ("lj13_wl_linear", dict(system=S.LJ, n=13, lanes=4, method=S.WL, energy_bin=0.1, move_param=0.1,
+                            has_min_e=1, min_e=-40.0, has_max_e=1, max_e=0.0, binning=S.BIN_LINEAR,
+                            high_resolution_de=0.025), 2.5, [10000]),
+    ("wca32_sad_pressure", dict(system=S.WCA, n=32, lanes=8, method=S.SAD, method_param=0.1, energy_bin=0.1,
+                                move_param=0.1, has_max_e=1, max_e=80.0, collect_pressure=1,
+                                high_resolution_de=0.05), (32 / 0.6) ** (1.0 / 3.0), [2100, 2000]),
+]
+
+
+@pytest.mark.parametrize("name,kw,geom,chunks", HIRES_CASES, ids=[c[0] for c in HIRES_CASES])
+def test_high_resolution_histogram_bit_exact(gpu_ctx_factory, name, kw, geom, chunks):
+    import oracle as O
+    print(f"case binning high_resolution {name}", flush=True)
+    sp = S.make_params(**kw)
+    seeds = np.arange(4, dtype=np.uint64) + 21
+    ctx, s = _run(gpu_ctx_factory, sp, seeds, geom, chunks, name)
+    for w, seed in enumerate(seeds):
+        orc = O.BWalker(oracle_params(sp, seed, 0.0 if geom is None else geom))
+        assert orc.run(sum(chunks)) == 0
+        _compare_hires(ctx, s, w, orc, f"{name} walker {w}")
+        st, counts = ctx.bin_hires(s, w)
+        assert int(st["total_count"]) == sum(chunks) == int(counts.sum())
+
+
+def test_high_resolution_window_overflow_is_reported(gpu_ctx_factory):
+    # the side histogram's window (bins of 1 over [-32, -20] plus margins)
+    # overflows while the walker's own (bins of 16) does not
+    print("case binning high_resolution overflow", flush=True)
+    ctx = gpu_ctx_factory()
+    sp = S.make_params(system=S.ISING, n=4, ising_init=1, method=S.SAD, method_param=1.0, energy_bin=16.0,
+                       e_lo_bound=-32.0, e_hi_bound=-20.0, high_resolution_de=1.0)
+    s = ctx.bin_add_set(sp, np.arange(8, dtype=np.uint64) + 1)
+    with pytest.raises(S.SadmcError, match="status 4"):
+        ctx.bin_run(s, 20000)
+    stopped = 0
+    for w in range(8):
+        st, counts = ctx.bin_hires(s, w)
+        assert 0 <= int(st["base"]) and int(st["base"]) + int(st["len"]) <= 20
+        stopped += int(ctx.bin_state(s, w)["status"]) == S.WS_CAPACITY
+    assert stopped >= 1
+    with pytest.raises(S.SadmcError, match="no high_resolution"):
+        ctx.bin_hires(ctx.bin_add_set(S.make_params(system=S.ISING, n=4, method=S.SAD, method_param=1.0), [1]), 0)

@@ -211,3 +211,60 @@ def test_linear_energy_extra_weights_sum_to_moves():
     assert int(st["energy"]["total_count"]) == 40000 == int(st["lnw"]["total_count"])
     w = bins["energy_count"].view(np.float64)
     assert abs(w.sum() - 40000) < 1e-6  # each move adds weights 1 - offset and offset
+
+
+# ---- the high_resolution side histogram (energy_binning.rs:125, 328-330)
+
+@pytest.mark.parametrize("method,kw", [(O.SAMC, dict(method_param=300.0)),
+                                       (O.WL, dict(has_min_e=1, min_e=-32.0, has_max_e=1, max_e=32.0))],
+                         ids=["samc", "wl"])
+def test_high_resolution_at_own_width_repeats_ln_w_counts(method, kw):
+    # SAMC and WL never reset ln w counts, and the side histogram is
+    # incremented with the same energies: at the walker's own width it holds
+    # the same window, counts and count statistics (a wrong index, edge or
+    # window growth in either histogram breaks the equality)
+    p = O.make_params(system=O.ISING, n=4, ising_init=1, method=method, energy_bin=1.0, high_resolution_de=1.0,
+                      seed=3, **kw)
+    b = O.BWalker(p)
+    assert b.run(20000) == 0
+    h, counts = b.hires()
+    s, bins = b.state(), b.bins()
+    np.testing.assert_array_equal(counts, bins["lnw_count"])
+    assert (h["min"], h["len"], h["min_e"], h["max_e"]) == (s["min"], s["len"], s["min_e"], s["max_e"])
+    for k in ("min_count", "max_count", "total_count", "e_max_count"):
+        assert h[k] == s["lnw"][k], k
+    assert h["total_count"] == 20000
+
+
+def test_high_resolution_fine_bins_nest_in_coarse_ones():
+    # LJ13 under SAMC with ln w bins of 1 and side bins of 1/4: every side
+    # bin lies inside one ln w bin (all edges are exact multiples of 1/4), so
+    # the side counts summed per ln w bin are the ln w counts; the statistics
+    # are those of the counts themselves
+    p = O.make_params(system=O.LJ, n=13, radius=2.5, order_g=4, method=O.SAMC, method_param=500.0, energy_bin=1.0,
+                      move_param=0.1, has_max_e=1, max_e=0.0, high_resolution_de=0.25, seed=9)
+    b = O.BWalker(p)
+    assert b.run(30000) == 0
+    h, fine = b.hires()
+    s, coarse = b.state(), b.bins()["lnw_count"]
+    assert h["len"] > 4 * 3
+    edges = h["min"] + 0.25 * np.arange(h["len"])
+    ci = np.floor((edges - s["min"]) / 1.0).astype(int)
+    assert ci.min() >= 0 and ci.max() < len(coarse)
+    summed = np.zeros(len(coarse), np.uint64)
+    np.add.at(summed, ci, fine)
+    np.testing.assert_array_equal(summed, coarse)
+    assert h["total_count"] == int(fine.sum()) == 30000
+    assert h["min_count"] == int(fine.min()) and h["max_count"] == int(fine.max())
+    # e_max_count is the energy that first took a bin past the old maximum:
+    # its bin holds the maximum
+    k = int(np.floor((h["e_max_count"] - h["min"]) / 0.25))
+    assert fine[k] == h["max_count"]
+    assert h["min_e"] == s["min_e"] and h["max_e"] == s["max_e"]
+    assert h["min"] <= h["min_e"] and h["max_e"] < h["min"] + 0.25 * h["len"]
+
+
+def test_no_high_resolution_by_default():
+    b = O.BWalker(O.make_params(system=O.ISING, n=4, method=O.SAD, method_param=1.0, energy_bin=1.0, seed=1))
+    b.run(100)
+    assert b.hires() is None
