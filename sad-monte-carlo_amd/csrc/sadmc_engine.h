@@ -190,6 +190,10 @@ struct Engine {
   double gpre, spec_u;
   bool gdirty;
   uint64_t spec0, spec1;
+  // sqrt(1 / moves) for the group's next G moves: lane l holds the value of
+  // move rs_first + l; rs_next counts the moves served from it (G = empty)
+  double rs_lane;
+  int32_t rs_next;
 
   // ---------------------------------------------------------------- setup
   SADMC_DF void load_system(const SetArgs &a, int walker, int lane_in_group, double *lds_group) {
@@ -239,6 +243,7 @@ struct Engine {
     err = w.error;
     moves = w.moves;
     tm = (double)moves;
+    rs_next = 1 << 30;
     accepted = w.accepted_moves;
     acc_rate = w.acceptance_rate;
     tscale = w.translation_scale;
@@ -853,10 +858,27 @@ struct Engine {
     spec_u = gen_f64(spec0, spec1);
   }
   // `need && u > exp(x)`, consuming the uniform only when `need`.
+  // The comparison is settled from a single-precision 2^(x log2 e) whenever
+  // u lies outside that estimate's error band; only inside the band is the
+  // double exp computed.  Band: t = x log2 e (double, relative error 2^-52)
+  // rounds to float with |dt| <= |t| 2^-24, v_exp_f32 is within a few ulp
+  // (2^-21), so the estimate is within |t| 2^-24 ln 2 + 2^-21 < 1e-7 |t| +
+  // 1e-6 of exp(x), and of the correctly rounded double exp (2^-53 more);
+  // the band is ten times that.  Below t = -125 exp(x) < 2^-125 lies under
+  // every nonzero u (multiples of 2^-53), so the test is u > 0.  NaN and
+  // x >= 0 take the double path.
   SADMC_DF bool mh_test(bool need, double x) {
     if (!need) return false;
     s0 = spec0;
     s1 = spec1;
+    const double t = x * 1.4426950408889634;
+    if (t < -125.0) return spec_u > 0.0;
+    if (t <= 0.0) {
+      const double est = (double)__builtin_amdgcn_exp2f((float)t);
+      const double band = est * (1e-5 - 1e-6 * t);
+      if (spec_u > est + band) return true;
+      if (spec_u < est - band) return false;
+    }
     return spec_u > exp(x);
   }
   // Metropolis-Hastings test on the log acceptance ratio.
@@ -1163,7 +1185,18 @@ struct Engine {
     }
     const double e1 = E;
     int32_t i1 = cur_i;
-    const double rs = sqrt(1.0 / tm);
+    double rs;
+    if (SYS == SYS_ISING) {
+      rs = sqrt(1.0 / tm);
+    } else {
+      // each lane of the group computes one of the next G values at once
+      if (rs_next >= G) {
+        rs_lane = sqrt(1.0 / (tm + (double)lane));
+        rs_next = 0;
+      }
+      rs = __shfl(rs_lane, rs_next, G);
+      rs_next += 1;
+    }
     acc_rate *= 1.0 - rs;
     double e2;
     const bool some = plan_move(tscale, usc, e2);
