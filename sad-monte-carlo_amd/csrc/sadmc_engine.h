@@ -599,32 +599,39 @@ struct Engine {
       ty = put_in_cell(fy + uy, L);
       tz = put_in_cell(fz + uz, L);
     }
+    // Every slot computes its pair terms; the moved atom's own slot and the
+    // padding slots (whose terms may be inf or NaN) are dropped by a select
+    // of the accumulator, which keeps the sums those of the skipped form
+    // without a divergent branch per atom.
     double acc = 0.0, accabs = 0.0;
 #pragma unroll
     for (int k = 0; k < KMAX; k++) {
       if (k < K) {
-        int j = lane + G * k;
-        if (j < n && j != w) {
-          if (SYS == SYS_LJ) {
-            if (PREC == 1) {
-              float an = (float)(xk(k) - tx), bn = (float)(yk(k) - ty), cn = (float)(zk(k) - tz);
-              float ao = (float)(xk(k) - fx), bo = (float)(yk(k) - fy), co = (float)(zk(k) - fz);
-              float r2n = an * an + bn * bn + cn * cn;
-              float r2o = ao * ao + bo * bo + co * co;
-              acc += (double)lj_pair_f32(r2n) - (double)lj_pair_f32(r2o);
-            } else {
-              acc += lj_pair(norm2(xk(k) - tx, yk(k) - ty, zk(k) - tz)) - lj_pair(norm2(xk(k) - fx, yk(k) - fy, zk(k) - fz));
-            }
+        const int j = lane + G * k;
+        const bool use = j < n && j != w;
+        if (SYS == SYS_LJ) {
+          double t;
+          if (PREC == 1) {
+            float an = (float)(xk(k) - tx), bn = (float)(yk(k) - ty), cn = (float)(zk(k) - tz);
+            float ao = (float)(xk(k) - fx), bo = (float)(yk(k) - fy), co = (float)(zk(k) - fz);
+            float r2n = an * an + bn * bn + cn * cn;
+            float r2o = ao * ao + bo * bo + co * co;
+            t = (double)lj_pair_f32(r2n) - (double)lj_pair_f32(r2o);
           } else {
-            double vn = wca_pair(norm2(min_image(xk(k) - tx, L, halfL), min_image(yk(k) - ty, L, halfL),
-                                       min_image(zk(k) - tz, L, halfL)),
-                                 args().rc2);
-            double vo = wca_pair(norm2(min_image(xk(k) - fx, L, halfL), min_image(yk(k) - fy, L, halfL),
-                                       min_image(zk(k) - fz, L, halfL)),
-                                 args().rc2);
-            acc += vn - vo;
-            accabs += vn + vo;
+            t = lj_pair(norm2(xk(k) - tx, yk(k) - ty, zk(k) - tz)) - lj_pair(norm2(xk(k) - fx, yk(k) - fy, zk(k) - fz));
           }
+          const double na = acc + t;
+          acc = use ? na : acc;
+        } else {
+          double vn = wca_pair(norm2(min_image(xk(k) - tx, L, halfL), min_image(yk(k) - ty, L, halfL),
+                                     min_image(zk(k) - tz, L, halfL)),
+                               args().rc2);
+          double vo = wca_pair(norm2(min_image(xk(k) - fx, L, halfL), min_image(yk(k) - fy, L, halfL),
+                                     min_image(zk(k) - fz, L, halfL)),
+                               args().rc2);
+          const double na = acc + (vn - vo), nb = accabs + (vn + vo);
+          acc = use ? na : acc;
+          accabs = use ? nb : accabs;
         }
       }
     }
