@@ -89,9 +89,11 @@ struct BinWalker {
   // second bin's new value twice)
   bool stale_t[5], stale_c[5];
   // the high_resolution side histogram (energy_binning.rs:125, 328-330):
-  // counts in HR[hr_base + i], and the smallest count's multiplicity
+  // counts in HR[hs.base + i], its statistics in *HS (read and written
+  // back by each increment, so they hold no registers across the move loop
+  // when the histogram is off), and the smallest count's multiplicity
   uint64_t *HR;
-  sadmc_hires hs;
+  sadmc_hires *HS;
   uint64_t hr_ncmin;
 
   SADMC_DF const BinArgs &args() const { return *Bp; }
@@ -174,10 +176,13 @@ struct BinWalker {
       if (bc[k].exists) rescan_all(k);
     }
     HR = nullptr;
+    HS = nullptr;
     if (ba.hr) {
       HR = ba.hr + (int64_t)walker * ba.hr_cap;
-      hs = ba.hrs[walker];
-      hr_rescan();
+      HS = ba.hrs + walker;
+      sadmc_hires hs = *HS;
+      hr_rescan(hs);
+      *HS = hs;
     }
   }
 
@@ -225,7 +230,6 @@ struct BinWalker {
     bc_store(bc[BC_TF], w.t_found);
     bc_store(bc[BC_HIST], w.hist);
     bc_store(bc[BC_PR], w.pressure);
-    if (HR) args().hrs[walker] = hs;
     if (status) *args().flag = 1;
   }
 
@@ -451,7 +455,7 @@ struct BinWalker {
   // ---------------------------------------- the high_resolution histogram
   // The smallest count and its multiplicity (a count never exceeds the
   // number of moves, so ~0 stands for an empty histogram).
-  SADMC_DF void hr_rescan() {
+  SADMC_DF void hr_rescan(sadmc_hires &hs) {
     uint64_t m = ~0ULL, c = 0;
     for (int64_t j0 = lane; j0 < hs.len; j0 += 4 * G) {
       uint64_t v[4];
@@ -485,7 +489,7 @@ struct BinWalker {
     hr_ncmin = c;
   }
   // insert_zero / push_zero (histogram.rs:47-58) on the side histogram
-  SADMC_DF void hr_new_zero(int64_t p) {
+  SADMC_DF void hr_new_zero(sadmc_hires &hs, int64_t p) {
     HR[p] = 0;
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     if (hs.min_count == 0) {
@@ -500,6 +504,12 @@ struct BinWalker {
   // min_e / max_e, the window grown to hold e, then the bin's count and the
   // count statistics (the totals stay 0)
   SADMC_DF bool hr_increment(double en) {
+    sadmc_hires hs = *HS;
+    const bool ok = hr_increment(hs, en);
+    *HS = hs;
+    return ok;
+  }
+  SADMC_DF bool hr_increment(sadmc_hires &hs, double en) {
     if (en > hs.max_e) hs.max_e = en;
     if (en < hs.min_e) hs.min_e = en;
     if (hs.len == 0) {
@@ -513,7 +523,7 @@ struct BinWalker {
         return false;
       }
       hs.base -= 1;
-      hr_new_zero(hs.base);
+      hr_new_zero(hs, hs.base);
       hs.min -= hs.width;
     }
     while (en >= hs.min + hs.width * (double)hs.len) {
@@ -521,7 +531,7 @@ struct BinWalker {
         status = SADMC_WS_CAPACITY;
         return false;
       }
-      hr_new_zero(hs.base + hs.len);
+      hr_new_zero(hs, hs.base + hs.len);
     }
     // histogram.rs:129-140
     const double fi = (en - hs.min) / hs.width;
@@ -535,7 +545,7 @@ struct BinWalker {
     r = new_count;
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     hs.total_count += 1;
-    if (old_count == hs.min_count && --hr_ncmin == 0) hr_rescan();
+    if (old_count == hs.min_count && --hr_ncmin == 0) hr_rescan(hs);
     if (new_count > hs.max_count) {
       hs.max_count = new_count;
       hs.e_max_count = en;
@@ -996,6 +1006,7 @@ struct BinWalker {
     bc[BC_LNW].e_max_total = -__builtin_inf();
     bc[BC_LNW].e_max_count = -__builtin_inf();
     if (HR) {  // Bins::new(system.energy(), de) (energy_binning.rs:568-570)
+      sadmc_hires hs;
       hs.width = args().hr_de;
       hs.min = (round(E / hs.width) - 0.5) * hs.width;
       hs.min_e = E;
@@ -1006,6 +1017,7 @@ struct BinWalker {
       hs.min_count = 0;
       hs.max_count = 0;
       hs.e_max_count = -__builtin_inf();
+      *HS = hs;
       hr_ncmin = 0;
     }
     status = e.status;
