@@ -420,7 +420,7 @@ def binning_path(moves, walkers, lanes, cpu_seconds, linear=False):
 
 def zmc(rounds, runs, lanes, cpu_seconds):
     """Energy-ceiling replicas (energy_replicas.rs) of LJ31 clusters in a
-    sphere of radius 3.0: `runs` independent runs of up to 48 replicas, 8
+    sphere of radius 3.0: `runs` independent runs of up to 16 replicas, 8
     lanes each, min_T = 0.05, a new replica after 64 unique visitors; one
     untimed launch of 200 rounds (replicas split there), then one timed
     launch of `rounds` rounds.  Value: replica moves/s (N = 31 moves per
@@ -566,6 +566,7 @@ def main():
     ap.add_argument("--binning-linear", action="store_true", help="with --binning: the linear interpolation")
     ap.add_argument("--zmc", type=int, default=0,
                     help="only run LJ31 energy-ceiling replica runs for this many rounds")
+    ap.add_argument("--zmc-runs", type=int, default=1024, help="with --zmc: independent runs")
     ap.add_argument("--entropy", type=int, default=0,
                     help="only time the walker-set S(E) reduction after this many moves per walker")
     ap.add_argument("--tempering", type=int, default=0,
@@ -590,7 +591,7 @@ def main():
                           "higher_is_better": True, "value": res["moves_per_s"], **res}))
         return
     if a.zmc > 0:
-        res = zmc(a.zmc, 1024, a.lanes, 0.0 if a.no_cpu else 15.0)
+        res = zmc(a.zmc, a.zmc_runs, a.lanes, 0.0 if a.no_cpu else 15.0)
         print(json.dumps({"metric": "LJ31 energy-ceiling replica moves/s on 1 GPU", "unit": "moves/s",
                           "higher_is_better": True, "value": res["moves_per_s"], **res}))
         return
