@@ -566,7 +566,8 @@ def main():
     ap.add_argument("--binning-linear", action="store_true", help="with --binning: the linear interpolation")
     ap.add_argument("--zmc", type=int, default=0,
                     help="only run LJ31 energy-ceiling replica runs for this many rounds")
-    ap.add_argument("--zmc-runs", type=int, default=1024, help="with --zmc: independent runs")
+    ap.add_argument("--zmc-runs", type=int, default=2048,
+                    help="with --zmc: independent runs (2048 = 8 runs per CU, all resident at once)")
     ap.add_argument("--entropy", type=int, default=0,
                     help="only time the walker-set S(E) reduction after this many moves per walker")
     ap.add_argument("--tempering", type=int, default=0,

@@ -321,8 +321,8 @@ int sadmc_bin_hires(sadmc_ctx *ctx, int32_t set, int32_t w, sadmc_hires *out, in
 /* ---- energy-ceiling replicas (src/mc/energy_replicas.rs, "ZMC") ---------
  * `sims` independent runs of Ising, LJ or WCA systems (the replicas binary;
  * geom: the LJ sphere radius or the WCA box length of each run),
- * each on one workgroup with up to max_replicas replicas (replicas * lanes
- * <= 512).  from_params (energy_replicas.rs:344-395) runs on the host when
+ * each on one wave with up to max_replicas replicas (replicas * lanes
+ * <= 512; the wave's lane groups take the replicas in turn).  from_params (energy_replicas.rs:344-395) runs on the host when
  * the set is added: 2^15 randomizations with seed_from_u64(seeds[s]), the
  * replicas [inf, E(1/2)] and [E(1/2), E(1/4)], the generator jumped, the
  * median estimator at E(1/4).  A round is MC::run_once

@@ -1,8 +1,10 @@
-// Energy-ceiling replicas (src/mc/energy_replicas.rs): one workgroup per
-// run, one lane group per system slot.
+// Energy-ceiling replicas (src/mc/energy_replicas.rs): one wave per run;
+// its lane groups of G lanes take the system slots in turn (group g runs
+// slots g, g + 64/G, ... that hold replicas), so slots without a replica
+// cost nothing.
 //
 // A round (MC::run_once, energy_replicas.rs:504-641, without plugins):
-// - every replica's group makes its moves at once: N (LJ, WCA) or N^2
+// - every replica's group makes its moves: N (LJ, WCA) or N^2
 //   (Ising) moves accepted iff E' < max_energy for a bounded replica
 //   (Replica::run_once, energy_replicas.rs:200-245), one randomization for
 //   the unbounded one, with the replica's generator and counters; a WCA
@@ -70,11 +72,13 @@ size_t zmc_lds_bytes(int RMAX, int npad) {
 }
 
 template <int SYS, int KMAX>
-__global__ __launch_bounds__(512) void k_zmc(ZArgs z, SetArgs a, uint64_t rounds) {
+__global__ __launch_bounds__(64) void k_zmc(ZArgs z, SetArgs a, uint64_t rounds) {
   extern __shared__ double lds[];
   const int RM = z.RMAX, G = a.G, sim = blockIdx.x;
   const int tid = threadIdx.x, nthr = blockDim.x;
-  const int slot = tid / G, lane = tid & (G - 1);
+  // one wave per run: its groups take slots group, group + per, ... in
+  // turn, so only slots holding replicas cost anything
+  const int per = nthr / G, group = tid / G, lane = tid & (G - 1);
   const int npad = SYS == SYS_ISING ? 0 : a.npad;
   double *pos = lds;
   uint64_t *spn = (uint64_t *)(pos + 3 * RM * npad);
@@ -129,9 +133,8 @@ __global__ __launch_bounds__(512) void k_zmc(ZArgs z, SetArgs a, uint64_t rounds
   for (int r = tid; r < RM; r += nthr) rep_of[slot_of[r]] = r;
 
   Engine<SYS, KMAX, 0, M_CAN> e;
-  const bool member = slot < RM;
-  if (member) {
-    e.load_system(a, (int)(S0 + slot), lane, pos + (int64_t)slot * 3 * npad);
+  for (int slot = group; slot < RM; slot += per) e.load_system(a, (int)(S0 + slot), lane, pos + (int64_t)slot * 3 * npad);
+  {
     if (SYS == SYS_LJ) {
       const double g = z.geom[sim];
       e.R2 = g * g;
@@ -159,7 +162,12 @@ __global__ __launch_bounds__(512) void k_zmc(ZArgs z, SetArgs a, uint64_t rounds
     const double very_lowest = meta[R - 1].max_e;
     const uint64_t mv0 = *smoves;
     // ---- every replica's moves
-    if (member && slot < R) {
+    for (int slot = group; slot < R; slot += per) {
+      if (SYS != SYS_ISING) {
+        e.lx = pos + (int64_t)slot * 3 * npad;
+        e.ly = e.lx + npad;
+        e.lz = e.ly + npad;
+      }
       e.E = sE[slot];
       e.err = sErr[slot];
       if (SYS == SYS_ISING) {

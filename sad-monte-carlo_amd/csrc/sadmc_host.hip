@@ -1802,7 +1802,7 @@ int sadmc_zmc_add(sadmc_ctx *c, const sadmc_set_params *p, int32_t S, const uint
   }
   if ((int64_t)RM * z.G > 512) return fail(c, SADMC_E_ARG, "max_replicas * lanes must be <= 512");
   if (zmc_lds_bytes(RM, z.npad) > 65536) return fail(c, SADMC_E_ARG, "a run's systems do not fit 64 KiB of LDS");
-  z.threads = ((RM * z.G + 63) / 64) * 64;
+  z.threads = 64;  // one wave per run; its lane groups take the slots in turn
   const int steps = z.sys == SYS_ISING ? n * n : n;
   const double dim = z.sys == SYS_ISING ? (double)n * n : 3.0 * n;
   const int64_t SR = (int64_t)S * RM;

@@ -70,7 +70,7 @@ struct PtArgs {
   int32_t R, steps;
   double d, sc;        // cube half-width and its Uniform scale
 };
-// Energy-ceiling replica runs (k_zmc.hip): one workgroup per run.  Slots
+// Energy-ceiling replica runs (k_zmc.hip): one wave per run.  Slots
 // hold systems (slot s of run i is walker i * RMAX + s of the SetArgs); the
 // replica -> slot table follows the reference's exchanges of systems.
 struct ZSim {
