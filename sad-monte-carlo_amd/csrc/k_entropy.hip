@@ -59,8 +59,16 @@ __global__ __launch_bounds__(64) void k_entropy_walker(const sadmc_walker_state 
     return best;
   };
   const int64_t ilo = nearest(s.too_lo), ihi = nearest(s.too_hi);
+  // four loads in flight per lane; the additions keep the i order
   double hsum = 0.0;
-  for (int64_t i = ilo + threadIdx.x; i <= ihi; i += 64) hsum += (double)H[i].hist;
+  for (int64_t i0 = ilo + threadIdx.x; i0 <= ihi; i0 += 256) {
+    uint64_t h[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) h[u] = i0 + 64 * u <= ihi ? H[i0 + 64 * u].hist : 0;
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (i0 + 64 * u <= ihi) hsum += (double)h[u];
+  }
   for (int d = 32; d >= 1; d >>= 1) hsum += __shfl_xor(hsum, d, 64);
   const double mean_hist = hsum / (double)(ihi - ilo + 1);
   const double lnw_lo = H[ilo].lnw, lnw_hi = H[ihi].lnw;
@@ -117,24 +125,41 @@ __global__ __launch_bounds__(256) void k_entropy_bins(const EntropyWalker *ew, i
   const int w0 = (int)((int64_t)W * c / chunks), w1 = (int)((int64_t)W * (c + 1) / chunks);
   double sum = 0.0;
   int64_t n = 0;
-  for (int w = w0; w < w1; w++) {
-    const EntropyWalker e = ew[w];
-    const int64_t i = g - e.off;
-    if (!e.ok || i < 0 || i >= e.len) continue;
-    const HotBin b = hot[(int64_t)w * cap + e.base + i];
-    const double E = e.bmin + ((double)i + 0.5) * width;
-    double v;
-    if (E < e.too_lo) {
-      if (b.hist == 0) continue;
-      v = e.lnw_lo + (E - e.too_lo) / e.min_T + log((double)b.hist / e.mean_hist);
-    } else if (E > e.too_hi) {
-      if (b.hist == 0) continue;
-      v = e.lnw_hi + log((double)b.hist / e.mean_hist);
-    } else {
-      v = b.lnw;
+  for (int w4 = w0; w4 < w1; w4 += 4) {
+    // the records of up to four walkers in flight, used in walker order
+    HotBin bq[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int w = w4 + u;
+      bq[u].lnw = 0.0;
+      bq[u].hist = 0;
+      if (w < w1) {
+        const int64_t i = g - ew[w].off;
+        if (ew[w].ok && i >= 0 && i < ew[w].len) bq[u] = hot[(int64_t)w * cap + ew[w].base + i];
+      }
     }
-    sum += v - e.c;
-    n += 1;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int w = w4 + u;
+      if (w >= w1) continue;
+      const EntropyWalker e = ew[w];
+      const int64_t i = g - e.off;
+      if (!e.ok || i < 0 || i >= e.len) continue;
+      const HotBin b = bq[u];
+      const double E = e.bmin + ((double)i + 0.5) * width;
+      double v;
+      if (E < e.too_lo) {
+        if (b.hist == 0) continue;
+        v = e.lnw_lo + (E - e.too_lo) / e.min_T + log((double)b.hist / e.mean_hist);
+      } else if (E > e.too_hi) {
+        if (b.hist == 0) continue;
+        v = e.lnw_hi + log((double)b.hist / e.mean_hist);
+      } else {
+        v = b.lnw;
+      }
+      sum += v - e.c;
+      n += 1;
+    }
   }
   psum[(int64_t)c * ng + g] = sum;
   pcnt[(int64_t)c * ng + g] = n;
@@ -153,6 +178,74 @@ __global__ __launch_bounds__(256) void k_entropy_combine(int64_t ng, int chunks,
   }
   S[g] = n ? sum / (double)n : 0.0;
   count[g] = n;
+}
+
+// The set's common grid from the walker states, on the device (one block):
+// gmin = the smallest bins_min, lo / hi = the largest too_lo / smallest
+// too_hi over the walkers still running with bins, then ng = the largest
+// round((bins_min - gmin) / width) + len.  out: gmin, lo, hi, ng (as
+// int64 bits), any.  Minima and maxima do not depend on the order.
+__global__ __launch_bounds__(1024) void k_entropy_range(const sadmc_walker_state *ws, int W, double width,
+                                                        EntropyRange *out) {
+  __shared__ double sg[1024], sl[1024], sh[1024];
+  __shared__ int64_t sn[1024];
+  __shared__ int sa[1024];
+  const int t = threadIdx.x;
+  double g = __builtin_inf(), lo = -__builtin_inf(), hi = __builtin_inf();
+  int any = 0;
+  for (int w = t; w < W; w += 1024) {
+    const sadmc_walker_state &x = ws[w];
+    if (x.status || x.len <= 0) continue;
+    g = x.bins_min < g ? x.bins_min : g;
+    lo = x.too_lo > lo ? x.too_lo : lo;
+    hi = x.too_hi < hi ? x.too_hi : hi;
+    any = 1;
+  }
+  sg[t] = g;
+  sl[t] = lo;
+  sh[t] = hi;
+  sa[t] = any;
+  __syncthreads();
+  for (int d = 512; d > 0; d >>= 1) {
+    if (t < d) {
+      sg[t] = sg[t + d] < sg[t] ? sg[t + d] : sg[t];
+      sl[t] = sl[t + d] > sl[t] ? sl[t + d] : sl[t];
+      sh[t] = sh[t + d] < sh[t] ? sh[t + d] : sh[t];
+      sa[t] = sa[t] | sa[t + d];
+    }
+    __syncthreads();
+  }
+  const double gmin = sg[0];
+  int64_t ng = 0;
+  for (int w = t; w < W; w += 1024) {
+    const sadmc_walker_state &x = ws[w];
+    if (x.status || x.len <= 0) continue;
+    const int64_t e = (int64_t)llround((x.bins_min - gmin) / width) + x.len;
+    ng = e > ng ? e : ng;
+  }
+  __syncthreads();
+  sn[t] = ng;
+  __syncthreads();
+  for (int d = 512; d > 0; d >>= 1) {
+    if (t < d) sn[t] = sn[t + d] > sn[t] ? sn[t + d] : sn[t];
+    __syncthreads();
+  }
+  if (t == 0) {
+    EntropyRange r;
+    r.gmin = gmin;
+    r.lo = sl[0];
+    r.hi = sh[0];
+    r.ng = sn[0];
+    r.any = sa[0];
+    r.pad = 0;
+    *out = r;
+  }
+}
+
+hipError_t launch_entropy_range(const sadmc_walker_state *ws, int W, double width, EntropyRange *out,
+                                hipStream_t s) {
+  hipLaunchKernelGGL(k_entropy_range, dim3(1), dim3(1024), 0, s, ws, W, width, out);
+  return hipGetLastError();
 }
 
 int entropy_chunks(int W, int64_t ng) {

@@ -143,6 +143,9 @@ struct sadmc_ctx {
   std::vector<PtHost> pts;
   std::vector<BinSetHost> bsets;
   std::vector<ZmcHost> zmcs;
+  // entropy reduction scratch, grown on demand and kept for later calls
+  void *d_ent = nullptr;
+  size_t ent_bytes = 0;
 };
 
 static int fail(sadmc_ctx *c, int code, const std::string &msg) {
@@ -386,6 +389,7 @@ void sadmc_destroy(sadmc_ctx *c) {
   for (void *p : {(void *)c->d_pw, (void *)c->d_pm, (void *)c->d_pv, (void *)c->d_pbf, (void *)c->d_rec,
                   (void *)c->d_partial, (void *)c->d_grad, (void *)c->d_loss})
     (void)hipFree(p);
+  (void)hipFree(c->d_ent);
   for (auto &z : c->zmcs)
     for (void *p : {(void *)z.d_sim, (void *)z.d_rep, (void *)z.d_slot, (void *)z.d_med, (void *)z.d_pos,
                     (void *)z.d_geom, (void *)z.d_slot_of, (void *)z.d_spins})
@@ -1013,55 +1017,57 @@ int sadmc_mean_entropy(sadmc_ctx *c, int32_t set, double e_a, double e_b, int64_
   if (!s || !c->inited) return fail(c, SADMC_E_ARG, "bad set or not initialised");
   if (s->p.method != SADMC_SAD) return fail(c, SADMC_E_ARG, "the entropy reduction reads SAD walkers");
   HIPCHECK(c, hipSetDevice(c->device));
-  std::vector<sadmc_walker_state> ws(s->W);
-  HIPCHECK(c, hipMemcpyAsync(ws.data(), s->d_ws, sizeof(sadmc_walker_state) * s->W, hipMemcpyDeviceToHost,
-                             c->stream));
-  HIPCHECK(c, hipStreamSynchronize(c->stream));
   const double w = s->width;
-  double gmin = 0.0, lo = -HUGE_VAL, hi = HUGE_VAL;
-  bool any = false;
-  for (const auto &x : ws) {
-    if (x.status || x.len <= 0) continue;
-    gmin = any ? std::min(gmin, x.bins_min) : x.bins_min;
-    lo = std::max(lo, x.too_lo);
-    hi = std::min(hi, x.too_hi);
-    any = true;
+  // scratch: the grid summary, the per-walker constants, S and counts, the
+  // chunks' partial sums; one allocation reused by later calls
+  const size_t al = 256;
+  auto up = [&](size_t b) { return (b + al - 1) / al * al; };
+  const size_t ew_off = up(sizeof(EntropyRange));
+  const size_t fixed = ew_off + up(entropy_walker_bytes() * s->W);
+  if (c->ent_bytes < fixed) {
+    (void)hipFree(c->d_ent);
+    c->d_ent = nullptr;
+    c->ent_bytes = 0;
+    HIPCHECK(c, hipMalloc(&c->d_ent, fixed));
+    c->ent_bytes = fixed;
   }
-  if (!any) return fail(c, SADMC_E_STATE, "no running walkers");
-  int64_t ng = 0;
-  for (const auto &x : ws)
-    if (!x.status && x.len > 0) ng = std::max<int64_t>(ng, (int64_t)llround((x.bins_min - gmin) / w) + x.len);
+  EntropyRange range;
+  HIPCHECK(c, launch_entropy_range(s->d_ws, s->W, w, (EntropyRange *)c->d_ent, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(&range, c->d_ent, sizeof range, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  if (!range.any) return fail(c, SADMC_E_STATE, "no running walkers");
+  const double gmin = range.gmin;
+  const int64_t ng = range.ng;
   if (!(e_a < e_b)) {
-    e_a = lo;
-    e_b = hi;
+    e_a = range.lo;
+    e_b = range.hi;
   }
   const int64_t ga = (int64_t)std::ceil((e_a - gmin) / w - 0.5), gb = (int64_t)std::floor((e_b - gmin) / w - 0.5) + 1;
   if (!(gb > ga)) return fail(c, SADMC_E_STATE, "no common energy window for the walkers");
   if (e_first) *e_first = gmin + 0.5 * w;
   if (nbins) *nbins = ng;
   if (ng > max_bins) return fail(c, SADMC_E_CAPACITY, "output holds fewer bins than the set covers");
-  void *ew = nullptr;
-  double *dS = nullptr, *dPS = nullptr;
-  int64_t *dC = nullptr, *dPC = nullptr;
   const int chunks = entropy_chunks(s->W, ng);
+  const size_t s_off = fixed, c_off = s_off + up(sizeof(double) * ng), ps_off = c_off + up(sizeof(int64_t) * ng),
+               pc_off = ps_off + up(sizeof(double) * ng * chunks), total = pc_off + up(sizeof(int64_t) * ng * chunks);
+  if (c->ent_bytes < total) {
+    (void)hipFree(c->d_ent);
+    c->d_ent = nullptr;
+    c->ent_bytes = 0;
+    HIPCHECK(c, hipMalloc(&c->d_ent, total));
+    c->ent_bytes = total;
+  }
+  char *base = (char *)c->d_ent;
+  double *dS = (double *)(base + s_off), *dPS = (double *)(base + ps_off);
+  int64_t *dC = (int64_t *)(base + c_off), *dPC = (int64_t *)(base + pc_off);
   int rc = SADMC_OK;
-  if (hipMalloc(&ew, entropy_walker_bytes() * s->W) != hipSuccess || hipMalloc(&dS, sizeof(double) * ng) != hipSuccess ||
-      hipMalloc(&dC, sizeof(int64_t) * ng) != hipSuccess ||
-      hipMalloc(&dPS, sizeof(double) * ng * chunks) != hipSuccess ||
-      hipMalloc(&dPC, sizeof(int64_t) * ng * chunks) != hipSuccess) {
-    rc = fail(c, SADMC_E_HIP, "allocation failed");
-  } else if (launch_entropy(s->d_ws, s->d_hot, s->W, s->cap, w, gmin, ga, gb, ng, ew, chunks, dPS, dPC, dS, dC,
-                            c->stream) != hipSuccess ||
-             (S && hipMemcpyAsync(S, dS, sizeof(double) * ng, hipMemcpyDeviceToHost, c->stream) != hipSuccess) ||
-             (count && hipMemcpyAsync(count, dC, sizeof(int64_t) * ng, hipMemcpyDeviceToHost, c->stream) != hipSuccess) ||
-             hipStreamSynchronize(c->stream) != hipSuccess) {
+  if (launch_entropy(s->d_ws, s->d_hot, s->W, s->cap, w, gmin, ga, gb, ng, base + ew_off, chunks, dPS, dPC, dS, dC,
+                     c->stream) != hipSuccess ||
+      (S && hipMemcpyAsync(S, dS, sizeof(double) * ng, hipMemcpyDeviceToHost, c->stream) != hipSuccess) ||
+      (count && hipMemcpyAsync(count, dC, sizeof(int64_t) * ng, hipMemcpyDeviceToHost, c->stream) != hipSuccess) ||
+      hipStreamSynchronize(c->stream) != hipSuccess) {
     rc = fail(c, SADMC_E_HIP, "entropy reduction failed");
   }
-  (void)hipFree(ew);
-  (void)hipFree(dS);
-  (void)hipFree(dC);
-  (void)hipFree(dPS);
-  (void)hipFree(dPC);
   return rc;
 }
 

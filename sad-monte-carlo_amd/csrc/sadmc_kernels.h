@@ -53,6 +53,14 @@ hipError_t launch_entropy(const sadmc_walker_state *ws, const HotBin *hot, int W
                           int64_t *pcnt, double *S, int64_t *count, hipStream_t s);
 int entropy_chunks(int W, int64_t ng);
 size_t entropy_walker_bytes();
+// The common grid of a walker set (k_entropy_range).
+struct EntropyRange {
+  double gmin, lo, hi;
+  int64_t ng;
+  int32_t any, pad;
+};
+hipError_t launch_entropy_range(const sadmc_walker_state *ws, int W, double width, EntropyRange *out,
+                                hipStream_t s);
 hipError_t launch_prop_update(const float *partial, int nblk, int count, float *master, float *m, float *v,
                               uint16_t *wbf, float lr, int step, float *grad_out, float *loss_out, hipStream_t s);
 hipError_t launch_wca(int meth, int kmax, KernelKind kk, const SetArgs &a, const LaunchArgs &la, hipStream_t s);
