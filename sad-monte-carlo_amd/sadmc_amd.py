@@ -115,6 +115,7 @@ HIRES_DTYPE = np.dtype([
     ("total_count", "<u8"), ("min_count", "<u8"), ("max_count", "<u8"), ("e_max_count", "<f8"),
 ])
 
+
 class WalkerStateC(C.Structure):
     _fields_ = [(name, C.c_uint8 * WALKER_STATE_DTYPE.itemsize) for name in ("raw",)]
 
@@ -311,13 +312,19 @@ class Context:
         """(E, S, count) of a SAD set reduced on the device (bins no walker covers dropped)."""
         ea, eb = window if window is not None else (0.0, 0.0)
         e0, nb = C.c_double(), C.c_int64()
-        rc = lib().sadmc_mean_entropy(self.h, s, ea, eb, 0, C.byref(e0), C.byref(nb), None, None)
-        if rc not in (0, -3):
-            self._check(rc, "mean_entropy")
-        S = np.zeros(nb.value)
-        cnt = np.zeros(nb.value, np.int64)
-        self._check(lib().sadmc_mean_entropy(self.h, s, ea, eb, nb.value, C.byref(e0), C.byref(nb), _p(S), _p(cnt)),
-                    "mean_entropy")
+        # every walker's bins lie in the set's window of `cap` slots, so the
+        # common grid has at most `cap` bins: one call normally suffices
+        n, cap = C.c_int32(), C.c_int64()
+        self._check(lib().sadmc_walker_count(self.h, s, C.byref(n), C.byref(cap)), "walker_count")
+        S = np.zeros(max(cap.value, 1))
+        cnt = np.zeros(max(cap.value, 1), np.int64)
+        rc = lib().sadmc_mean_entropy(self.h, s, ea, eb, len(S), C.byref(e0), C.byref(nb), _p(S), _p(cnt))
+        if rc == -3:
+            S = np.zeros(nb.value)
+            cnt = np.zeros(nb.value, np.int64)
+            rc = lib().sadmc_mean_entropy(self.h, s, ea, eb, nb.value, C.byref(e0), C.byref(nb), _p(S), _p(cnt))
+        self._check(rc, "mean_entropy")
+        S, cnt = S[:nb.value], cnt[:nb.value]
         w = self.sets[s][0].energy_bin if self.sets[s][0].energy_bin > 0 else (4.0 if self.sets[s][0].system == ISING
                                                                                else 1.0)
         E = e0.value + np.arange(nb.value) * w
