@@ -566,6 +566,8 @@ def main():
     ap.add_argument("--binning-linear", action="store_true", help="with --binning: the linear interpolation")
     ap.add_argument("--zmc", type=int, default=0,
                     help="only run LJ31 energy-ceiling replica runs for this many rounds")
+    ap.add_argument("--pt-ladders", type=int, default=1024,
+                    help="with --tempering: ladders of 16 replicas (1024 x 16 = the 16384 walkers of the LJ31 case)")
     ap.add_argument("--zmc-runs", type=int, default=2048,
                     help="with --zmc: independent runs (2048 = 8 runs per CU, all resident at once)")
     ap.add_argument("--entropy", type=int, default=0,
@@ -602,7 +604,7 @@ def main():
                           "higher_is_better": False, "value": res["device_s"], **res}))
         return
     if a.tempering > 0:
-        res = tempering(a.tempering, 512, 16, a.lanes, 0.0 if a.no_cpu else 15.0)
+        res = tempering(a.tempering, a.pt_ladders, 16, a.lanes, 0.0 if a.no_cpu else 15.0)
         print(json.dumps({"metric": "LJ31 parallel-tempering replica moves/s on 1 GPU", "unit": "moves/s",
                           "higher_is_better": True, "value": res["moves_per_s"], **res}))
         return
