@@ -180,20 +180,41 @@ __global__ __launch_bounds__(256) void k_entropy_combine(int64_t ng, int chunks,
   count[g] = n;
 }
 
-// The set's common grid from the walker states, on the device (one block):
-// gmin = the smallest bins_min, lo / hi = the largest too_lo / smallest
-// too_hi over the walkers still running with bins, then ng = the largest
-// round((bins_min - gmin) / width) + len.  out: gmin, lo, hi, ng (as
-// int64 bits), any.  Minima and maxima do not depend on the order.
-__global__ __launch_bounds__(1024) void k_entropy_range(const sadmc_walker_state *ws, int W, double width,
-                                                        EntropyRange *out) {
-  __shared__ double sg[1024], sl[1024], sh[1024];
-  __shared__ int64_t sn[1024];
-  __shared__ int sa[1024];
+// The set's common grid from the walker states, on the device: gmin = the
+// smallest bins_min, lo / hi = the largest too_lo / smallest too_hi over the
+// walkers still running with bins, then ng = the largest
+// round((bins_min - gmin) / width) + len.  kEntropyRangeBlocks blocks reduce
+// slices of the walkers into partial records, one block combines them
+// (twice: ng needs gmin).  Minima and maxima do not depend on the order.
+template <typename T, typename F>
+__device__ __forceinline__ T block_reduce(T v, T *sh, F op) {
   const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int d = blockDim.x / 2; d > 0; d >>= 1) {
+    if (t < d) sh[t] = op(sh[t], sh[t + d]);
+    __syncthreads();
+  }
+  const T r = sh[0];
+  __syncthreads();
+  return r;
+}
+struct DMin {
+  __device__ double operator()(double a, double b) const { return b < a ? b : a; }
+};
+struct DMax {
+  __device__ double operator()(double a, double b) const { return b > a ? b : a; }
+};
+struct IMax {
+  __device__ int64_t operator()(int64_t a, int64_t b) const { return b > a ? b : a; }
+};
+
+__global__ __launch_bounds__(256) void k_entropy_range_part(const sadmc_walker_state *ws, int W, EntropyRange *part) {
+  __shared__ double sd[256];
+  __shared__ int64_t si[256];
   double g = __builtin_inf(), lo = -__builtin_inf(), hi = __builtin_inf();
-  int any = 0;
-  for (int w = t; w < W; w += 1024) {
+  int64_t any = 0;
+  for (int w = blockIdx.x * 256 + threadIdx.x; w < W; w += gridDim.x * 256) {
     const sadmc_walker_state &x = ws[w];
     if (x.status || x.len <= 0) continue;
     g = x.bins_min < g ? x.bins_min : g;
@@ -201,50 +222,72 @@ __global__ __launch_bounds__(1024) void k_entropy_range(const sadmc_walker_state
     hi = x.too_hi < hi ? x.too_hi : hi;
     any = 1;
   }
-  sg[t] = g;
-  sl[t] = lo;
-  sh[t] = hi;
-  sa[t] = any;
-  __syncthreads();
-  for (int d = 512; d > 0; d >>= 1) {
-    if (t < d) {
-      sg[t] = sg[t + d] < sg[t] ? sg[t + d] : sg[t];
-      sl[t] = sl[t + d] > sl[t] ? sl[t + d] : sl[t];
-      sh[t] = sh[t + d] < sh[t] ? sh[t + d] : sh[t];
-      sa[t] = sa[t] | sa[t + d];
-    }
-    __syncthreads();
+  g = block_reduce(g, sd, DMin());
+  lo = block_reduce(lo, sd, DMax());
+  hi = block_reduce(hi, sd, DMin());
+  any = block_reduce(any, si, IMax());
+  if (threadIdx.x == 0) {
+    EntropyRange r;
+    r.gmin = g;
+    r.lo = lo;
+    r.hi = hi;
+    r.ng = 0;
+    r.any = (int32_t)any;
+    r.pad = 0;
+    part[blockIdx.x] = r;
   }
-  const double gmin = sg[0];
+}
+
+__global__ __launch_bounds__(256) void k_entropy_range_ng(const sadmc_walker_state *ws, int W, double width,
+                                                          const EntropyRange *out, EntropyRange *part) {
+  __shared__ int64_t si[256];
+  const double gmin = out->gmin;
   int64_t ng = 0;
-  for (int w = t; w < W; w += 1024) {
+  for (int w = blockIdx.x * 256 + threadIdx.x; w < W; w += gridDim.x * 256) {
     const sadmc_walker_state &x = ws[w];
     if (x.status || x.len <= 0) continue;
     const int64_t e = (int64_t)llround((x.bins_min - gmin) / width) + x.len;
     ng = e > ng ? e : ng;
   }
-  __syncthreads();
-  sn[t] = ng;
-  __syncthreads();
-  for (int d = 512; d > 0; d >>= 1) {
-    if (t < d) sn[t] = sn[t + d] > sn[t] ? sn[t + d] : sn[t];
-    __syncthreads();
+  ng = block_reduce(ng, si, IMax());
+  if (threadIdx.x == 0) part[blockIdx.x].ng = ng;
+}
+
+// One block of kEntropyRangeBlocks lanes combines the partial records; with
+// `ng_only` only their ng.
+__global__ __launch_bounds__(kEntropyRangeBlocks) void k_entropy_range_join(const EntropyRange *part,
+                                                                            EntropyRange *out, int ng_only) {
+  __shared__ double sd[kEntropyRangeBlocks];
+  __shared__ int64_t si[kEntropyRangeBlocks];
+  const EntropyRange p = part[threadIdx.x];
+  if (ng_only) {
+    const int64_t ng = block_reduce(p.ng, si, IMax());
+    if (threadIdx.x == 0) out->ng = ng;
+    return;
   }
-  if (t == 0) {
+  const double g = block_reduce(p.gmin, sd, DMin());
+  const double lo = block_reduce(p.lo, sd, DMax());
+  const double hi = block_reduce(p.hi, sd, DMin());
+  const int64_t any = block_reduce((int64_t)p.any, si, IMax());
+  if (threadIdx.x == 0) {
     EntropyRange r;
-    r.gmin = gmin;
-    r.lo = sl[0];
-    r.hi = sh[0];
-    r.ng = sn[0];
-    r.any = sa[0];
+    r.gmin = g;
+    r.lo = lo;
+    r.hi = hi;
+    r.ng = 0;
+    r.any = (int32_t)any;
     r.pad = 0;
     *out = r;
   }
 }
 
 hipError_t launch_entropy_range(const sadmc_walker_state *ws, int W, double width, EntropyRange *out,
-                                hipStream_t s) {
-  hipLaunchKernelGGL(k_entropy_range, dim3(1), dim3(1024), 0, s, ws, W, width, out);
+                                EntropyRange *part, hipStream_t s) {
+  const dim3 nb(kEntropyRangeBlocks), nt(256);
+  hipLaunchKernelGGL(k_entropy_range_part, nb, nt, 0, s, ws, W, part);
+  hipLaunchKernelGGL(k_entropy_range_join, dim3(1), nb, 0, s, (const EntropyRange *)part, out, 0);
+  hipLaunchKernelGGL(k_entropy_range_ng, nb, nt, 0, s, ws, W, width, (const EntropyRange *)out, part);
+  hipLaunchKernelGGL(k_entropy_range_join, dim3(1), nb, 0, s, (const EntropyRange *)part, out, 1);
   return hipGetLastError();
 }
 

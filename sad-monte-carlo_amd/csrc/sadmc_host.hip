@@ -1022,7 +1022,7 @@ int sadmc_mean_entropy(sadmc_ctx *c, int32_t set, double e_a, double e_b, int64_
   // chunks' partial sums; one allocation reused by later calls
   const size_t al = 256;
   auto up = [&](size_t b) { return (b + al - 1) / al * al; };
-  const size_t ew_off = up(sizeof(EntropyRange));
+  const size_t ew_off = up(sizeof(EntropyRange) * (1 + kEntropyRangeBlocks));
   const size_t fixed = ew_off + up(entropy_walker_bytes() * s->W);
   if (c->ent_bytes < fixed) {
     (void)hipFree(c->d_ent);
@@ -1032,7 +1032,8 @@ int sadmc_mean_entropy(sadmc_ctx *c, int32_t set, double e_a, double e_b, int64_
     c->ent_bytes = fixed;
   }
   EntropyRange range;
-  HIPCHECK(c, launch_entropy_range(s->d_ws, s->W, w, (EntropyRange *)c->d_ent, c->stream));
+  HIPCHECK(c, launch_entropy_range(s->d_ws, s->W, w, (EntropyRange *)c->d_ent, (EntropyRange *)c->d_ent + 1,
+                                   c->stream));
   HIPCHECK(c, hipMemcpyAsync(&range, c->d_ent, sizeof range, hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   if (!range.any) return fail(c, SADMC_E_STATE, "no running walkers");

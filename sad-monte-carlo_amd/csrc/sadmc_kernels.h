@@ -59,8 +59,10 @@ struct EntropyRange {
   int64_t ng;
   int32_t any, pad;
 };
+// `part` holds kEntropyRangeBlocks records of scratch.
+constexpr int kEntropyRangeBlocks = 64;
 hipError_t launch_entropy_range(const sadmc_walker_state *ws, int W, double width, EntropyRange *out,
-                                hipStream_t s);
+                                EntropyRange *part, hipStream_t s);
 hipError_t launch_prop_update(const float *partial, int nblk, int count, float *master, float *m, float *v,
                               uint16_t *wbf, float lr, int step, float *grad_out, float *loss_out, hipStream_t s);
 hipError_t launch_wca(int meth, int kmax, KernelKind kk, const SetArgs &a, const LaunchArgs &la, hipStream_t s);
