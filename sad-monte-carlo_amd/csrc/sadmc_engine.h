@@ -1185,6 +1185,13 @@ struct Engine {
       if (args().refresh > 0) {
         E = full_energy();
         next_check += (uint64_t)args().refresh;
+        // the refreshed energy may lie in another bin than the tracked one
+        // (move_once indexes the bins by the energy itself, energy.rs:904-974);
+        // the previous move's pending sums belong to the bin it left
+        if (e_index(E) != cur_i) {
+          apply_pending();
+          refresh_indices();
+        }
       } else {
         verify_energy();
         next_check += (uint64_t)len * (uint64_t)len * 1000ULL;

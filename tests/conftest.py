@@ -71,6 +71,23 @@ def compare_walker(gpu_state, gpu_bins, orc, method, label=""):
     np.testing.assert_array_equal(gpu_bins["pressure_total"], ob["pressure_total"], err_msg=f"{label} p_total")
 
 
+def run_oracles(params, moves, threads=None):
+    """Oracle walkers for `params` (a list of oracle Params), each run for
+    `moves` moves, on parallel threads (ctypes releases the GIL inside the C
+    call); returns the walkers in order."""
+    from concurrent.futures import ThreadPoolExecutor
+    import oracle as O
+
+    def one(p):
+        w = O.Walker(p)
+        w.run(moves)
+        return w
+
+    threads = threads or min(len(params), 16, len(os.sched_getaffinity(0)))
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        return list(ex.map(one, params))
+
+
 @pytest.fixture(scope="session")
 def gpu_ctx_factory():
     import sadmc_amd as S

@@ -114,6 +114,19 @@ def test_lj_sad_bit_exact(gpu_ctx_factory, name, n, radius, lanes, kw):
     _run_and_compare(gpu_ctx_factory, sp, np.arange(6, dtype=np.uint64) + 100, radius, 8000, chunks=2, label=name)
 
 
+def test_lj31_fp32_refresh_into_another_bin(gpu_ctx_factory):
+    """The fixed FP64 refresh of the precision configuration can move the
+    tracked energy across a bin edge; the move that follows must then use
+    the bin of the refreshed energy, as move_once indexes the bins by the
+    energy itself (energy.rs:904-974).  The oracle shows these seeds'
+    refreshes at moves 1024-9216 crossing an edge."""
+    print("case lj31 fp32 refresh across a bin edge", flush=True)
+    sp = S.make_params(system=S.LJ, n=31, lanes=8, method=S.SAD, method_param=0.05, move_param=0.1,
+                       energy_bin=0.01, has_max_e=1, max_e=0.0, precision=S.FP32_PAIRS, refresh=1024)
+    seeds = np.array([97, 255, 406, 1275, 1309, 1492, 1721, 1804, 2289, 2119], dtype=np.uint64)
+    _run_and_compare(gpu_ctx_factory, sp, seeds, 3.0, 12000, chunks=3, label="refresh crossing")
+
+
 @pytest.mark.parametrize("n,lanes", [(32, 8), (64, 16), (128, 32)])
 def test_wca_sad_bit_exact(gpu_ctx_factory, n, lanes):
     print(f"case wca n={n} lanes={lanes}", flush=True)
