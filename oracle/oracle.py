@@ -101,6 +101,9 @@ def lib():
     L.or_mc_energy.argtypes = [C.c_void_p]; L.or_mc_energy.restype = C.c_double
     L.or_mc_trace.argtypes = [C.c_void_p, C.c_uint64]
     L.or_mc_trace_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]; L.or_mc_trace_copy.restype = C.c_uint64
+    L.or_mc_last_move.argtypes = [C.c_void_p, C.c_void_p]; L.or_mc_last_move.restype = None
+    L.or_mc_load_state.argtypes = [C.c_void_p, C.POINTER(Scalars)] + [C.c_void_p] * 8
+    L.or_mc_load_state.restype = C.c_int
     L.or_ising_init_spins.argtypes = [C.c_int32, C.c_int32, C.c_uint64, C.c_void_p]
     L.or_ising_energy.argtypes = [C.c_int32, C.c_void_p]; L.or_ising_energy.restype = C.c_double
     L.or_lj_init.argtypes = [C.c_int32, C.c_double, C.c_uint64, C.c_void_p]; L.or_lj_init.restype = C.c_int
@@ -366,6 +369,40 @@ class Walker:
         a = np.zeros(cap, np.uint8)
         n = lib().or_mc_trace_copy(self.h, _ptr(e), _ptr(a))
         return e[:n], a[:n]
+
+    def last_move(self):
+        """The last move's decision inputs: proposed energy (NaN: None), the
+        learned move's log acceptance ratio and uniform (NaN: none drawn) and
+        its wall margin |to|^2 - R^2."""
+        out = np.zeros(4)
+        lib().or_mc_last_move(self.h, _ptr(out))
+        return dict(e2=out[0], lr=out[1], u=out[2], wall=out[3])
+
+    def load_state(self, scalars, bins, coords):
+        """Overwrite this (LJ/WCA SAD) walker with another engine's state:
+        `scalars` a dict with the scalars() keys, `bins` a dict with the
+        bins() keys, `coords` SoA positions."""
+        need = {"E", "error", "moves", "accepted", "acceptance_rate", "translation_scale", "rng", "bins_min",
+                "bins_width", "len", "max_S", "max_S_index", "method", "status", "too_lo", "too_hi", "lambda_",
+                "min_T", "tL", "tF", "num_states", "highest_hist"}
+        missing = need - set(scalars)
+        assert not missing, f"load_state: missing {sorted(missing)}"
+        sc = Scalars()
+        for name, _ in Scalars._fields_:
+            if name in scalars:
+                v = scalars[name]
+                if name == "rng":
+                    sc.rng[0], sc.rng[1] = int(v[0]), int(v[1])
+                else:
+                    setattr(sc, name, type(getattr(sc, name))(v))
+        arr = {k: np.ascontiguousarray(bins[k], dtype=dt) for k, dt in (
+            ("lnw", np.float64), ("histogram", np.uint64), ("t_found", np.uint64), ("energy_total", np.float64),
+            ("energy_squared_total", np.float64), ("visited", np.uint8), ("round_trips", np.uint64))}
+        c = np.ascontiguousarray(coords, dtype=np.float64)
+        rc = lib().or_mc_load_state(self.h, C.byref(sc), *[_ptr(arr[k]) for k in (
+            "lnw", "histogram", "t_found", "energy_total", "energy_squared_total", "visited", "round_trips")],
+            _ptr(c))
+        assert rc == 0, "load_state: LJ/WCA SAD walkers only"
 
     def scalars(self):
         s = Scalars()

@@ -223,6 +223,11 @@ struct or_mc {
   double *tr_e;
   uint8_t *tr_a;
   uint64_t tr_cap, tr_n;
+  /* the last move's decision inputs (test hook): proposed energy (NaN when
+   * the move was None), log acceptance ratio and uniform of the
+   * Metropolis-Hastings test (NaN when no uniform was drawn), |to|^2 - R^2
+   * of a learned LJ move's wall test */
+  double lm_e2, lm_lr, lm_u, lm_wall;
   int status;
 };
 
@@ -797,7 +802,12 @@ static int idx_ok(or_mc *m, int64_t i) {
 
 /* Metropolis-Hastings test on the log acceptance ratio lr = (ln w1 - ln w2)
  * + ln H of the learned move (the reference's tests when ln H = 0). */
-static int reject_lr(or_mc *m, double lr) { return lr < 0.0 && or_gen_f64(m->rng) > exp(lr); }
+static int reject_lr(or_mc *m, double lr) {
+  m->lm_lr = lr;
+  if (!(lr < 0.0)) return 0;
+  m->lm_u = or_gen_f64(m->rng);
+  return m->lm_u > exp(lr);
+}
 
 /* energy.rs:440-512; lnh = ln H of a learned move, 0 for the cube move. */
 static int reject_move(or_mc *m, double e1, double e2, double lnh) {
@@ -1090,6 +1100,7 @@ static int lj_plan_learned(or_mc *m, double *e_out, double *lnh) {
   float sig = expf(s);
   double tx = fx + (double)(sig * xi[0]), ty = fy + (double)(sig * xi[1]), tz = fz + (double)(sig * xi[2]);
   double r2to = norm2(tx, ty, tz), r2from = norm2(fx, fy, fz);
+  m->lm_wall = r2to - m->R2;
   if (r2to > m->R2 && r2to > r2from) return 0;
   prop_features_xyz(m->n, m->x, m->y, m->z, (int)w, tx, ty, tz, f);
   float s2 = prop_forward(m->p.mlp, f, NULL, NULL, NULL);
@@ -1278,7 +1289,9 @@ static void move_once(or_mc *m) {
   double e2;
   int acc = 0;
   double lnh = 0.0;
+  m->lm_e2 = m->lm_lr = m->lm_u = m->lm_wall = NAN;
   int some = m->p.move_kind == 1 && m->sys == OR_LJ ? lj_plan_learned(m, &e2, &lnh) : sys_plan_move(m, m->tscale, &e2);
+  if (some) m->lm_e2 = e2;
   if (some) {
     int oob = 0;
     if (m->p.has_max_e) oob = e2 > m->p.max_e && e2 > e1;
@@ -1497,6 +1510,56 @@ double or_mc_energy(const or_mc *m) { return m->E; }
 double or_mc_reference_energy(const or_mc *m) { return sys_compute_energy(m, 0); }
 
 /* Trace accessors live on the struct; expose copies. */
+int or_mc_load_state(or_mc *m, const or_scalars *s, const double *lnw, const uint64_t *hist,
+                     const uint64_t *t_found, const double *e_tot, const double *e_sq, const uint8_t *visited,
+                     const uint64_t *round_trips, const double *coords) {
+  if (m->sys != OR_LJ && m->sys != OR_WCA) return -1;
+  if (s->method != OR_SAD || m->method != OR_SAD) return -1;
+  m->E = s->E;
+  m->error = s->error;
+  m->moves = s->moves;
+  m->accepted = s->accepted;
+  m->acc_rate = s->acceptance_rate;
+  m->tscale = s->translation_scale;
+  m->rng[0] = s->rng[0];
+  m->rng[1] = s->rng[1];
+  m->min = s->bins_min;
+  m->width = s->bins_width;
+  bins_reserve(m, s->len);
+  m->len = s->len;
+  const size_t n = (size_t)s->len;
+  memcpy(m->lnw, lnw, sizeof(double) * n);
+  memcpy(m->hist, hist, sizeof(uint64_t) * n);
+  memcpy(m->tfound, t_found, sizeof(uint64_t) * n);
+  memcpy(m->etot, e_tot, sizeof(double) * n);
+  memcpy(m->esq, e_sq, sizeof(double) * n);
+  memcpy(m->vis, visited, n);
+  memcpy(m->rt, round_trips, sizeof(uint64_t) * n);
+  memset(m->pcount, 0, sizeof(uint64_t) * n);
+  memset(m->ptotal, 0, sizeof(double) * n);
+  m->max_S = s->max_S;
+  m->max_S_index = s->max_S_index;
+  m->too_lo = s->too_lo;
+  m->too_hi = s->too_hi;
+  m->lambda = s->lambda;
+  m->min_T = s->min_T;
+  m->tL = s->tL;
+  m->tF = s->tF;
+  m->Ns = s->num_states;
+  m->highest = s->highest_hist;
+  m->status = s->status;
+  memcpy(m->x, coords, sizeof(double) * 3 * (size_t)m->n);
+  m->ch_kind = 0;
+  return 0;
+}
+
+void or_mc_last_move(const or_mc *m, double *out) {
+  out[0] = m->lm_e2;
+  out[1] = m->lm_lr;
+  out[2] = m->lm_u;
+  out[3] = m->lm_wall;
+}
+
 uint64_t or_mc_trace_copy(const or_mc *m, double *energies, uint8_t *accepted) {
   if (energies) memcpy(energies, m->tr_e, sizeof(double) * m->tr_n);
   if (accepted) memcpy(accepted, m->tr_a, m->tr_n);

@@ -135,6 +135,20 @@ double or_mc_energy(const or_mc *mc);
  * copies what was recorded and returns the count. */
 void or_mc_trace(or_mc *mc, uint64_t cap);
 uint64_t or_mc_trace_copy(const or_mc *mc, double *energies, uint8_t *accepted);
+/* The last move's decision inputs: out[0] the proposed energy (NaN: None),
+ * out[1] the log acceptance ratio of a learned move's Metropolis-Hastings
+ * test and out[2] its uniform (NaN: none drawn), out[3] |to|^2 - R^2 of a
+ * learned move's wall test.  Lets a float-tolerance comparison tell a
+ * decision taken inside rounding of its threshold from a real mismatch. */
+void or_mc_last_move(const or_mc *mc, double *out);
+/* Overwrite an LJ/WCA SAD walker's whole state -- scalars, the len logical
+ * bins, positions (SoA) -- with another engine's (test hook: a comparison
+ * at float tolerance resynchronises the oracle on the device walker after
+ * the two part at a decision within rounding of its threshold).  Returns -1
+ * for other systems or methods. */
+int or_mc_load_state(or_mc *mc, const or_scalars *s, const double *lnw, const uint64_t *hist,
+                     const uint64_t *t_found, const double *e_tot, const double *e_sq, const uint8_t *visited,
+                     const uint64_t *round_trips, const double *coords);
 
 /* Scripted system: plan_move returns script[k % len] (NaN = None). */
 or_mc *or_mc_new_scripted(const or_params *p, double e0, const double *script, int64_t len);

@@ -108,3 +108,65 @@ def test_autotune_picks_a_measured_lane_count():
     best, rates = S.autotune_lanes(sp, np.arange(2048, dtype=np.uint64), 3.2, warm_moves=200, trial_moves=200)
     assert best in rates and set(rates) == {4, 8, 16, 32, 64}  # 38 atoms over 2 lanes is over 16 per lane
     assert rates[best] == max(rates.values()) > 0
+
+
+# The mixed-batch workload as the benchmark runs it: every size class, the
+# lanes its autotuner picks (seven of the eight classes then hold 5-8 atoms
+# per lane, the KMAX = 8 branches of the batched kernel), the packing fraction
+# drawn per walker, SAD min_T 0.1, 0.05 sigma moves, FP64 refresh every 1024
+# moves.  Two full workgroups of walkers per set plus a ragged third.
+BENCH_CLASSES = [  # (system, atoms, sphere radius (LJ), lanes)
+    (S.LJ, 13, 2.5, 4), (S.LJ, 19, 2.7, 4), (S.LJ, 31, 3.0, 4), (S.LJ, 38, 3.2, 8),
+    (S.WCA, 32, 0.0, 4), (S.WCA, 64, 0.0, 8), (S.WCA, 128, 0.0, 16), (S.WCA, 256, 0.0, 32),
+]
+
+
+def _bench_sets():
+    rng = np.random.default_rng(7)
+    out = []
+    for k, (system, n, R, lanes) in enumerate(BENCH_CLASSES):
+        W = 2 * (64 // lanes) + 3
+        if system == S.LJ:
+            kw = dict(energy_bin=0.01, has_max_e=1, max_e=0.0)
+            geom = np.full(W, R)
+        else:
+            kw = dict(energy_bin=0.05, has_max_e=1, max_e=2.5 * n)
+            geom = np.cbrt(n / (6.0 * rng.uniform(0.3, 0.5, W) / np.pi))
+        sp = S.make_params(system=system, n=n, lanes=lanes, method=S.SAD, method_param=0.1, move_param=0.05,
+                           refresh=1024, **kw)
+        out.append((sp, np.arange(W, dtype=np.uint64) + 100000 * k, geom))
+    return out
+
+
+def test_bench_composition_batched_graph_per_set_and_oracle(gpu_ctx_factory):
+    from conftest import run_oracles
+    print("case batched bench composition", flush=True)
+    sets = _bench_sets()
+    moves = 3000
+    a, ia = _build(gpu_ctx_factory, sets)
+    a.run(moves)
+    want = _dump(a, ia, sets)
+
+    b, ib = _build(gpu_ctx_factory, sets)
+    b.run_batched(moves)
+    got_b = _dump(b, ib, sets)
+
+    c, ic = _build(gpu_ctx_factory, sets)
+    for k in range(3):
+        c.run_batched(moves // 3, graph=True)
+        c.state(ic[k], 0)  # eager work between replays
+    got_c = _dump(c, ic, sets)
+
+    for k, (w, gb, gc) in enumerate(zip(want, got_b, got_c)):
+        for name, x, y, z in zip(("states", "bins", "coords"), w, gb, gc):
+            assert x.tobytes() == y.tobytes(), f"set {k} {name}: batched differs from per-set"
+            assert x.tobytes() == z.tobytes(), f"set {k} {name}: graph replay differs from per-set"
+        assert np.all(w[0]["status"] == 0)
+        assert np.all(w[0]["moves"] == moves)
+
+    picks = [(sid, sp, seeds, geom, w) for sid, (sp, seeds, geom) in zip(ib, sets) for w in (0, len(seeds) - 1)]
+    orcs = run_oracles([oracle_params(sp, seeds[w], float(geom[w])) for _, sp, seeds, geom, w in picks], moves)
+    for (sid, sp, seeds, geom, w), orc in zip(picks, orcs):
+        label = f"{'LJ' if sp.system == S.LJ else 'WCA'}{sp.n} G{sp.lanes} walker {w}"
+        compare_walker(b.state(sid, w), b.bins(sid, w), orc, sp.method, label=label)
+        np.testing.assert_array_equal(b.coords(sid, w), orc.coords(), err_msg=label)

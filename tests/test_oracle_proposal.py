@@ -157,3 +157,26 @@ def test_probe_records_draw_order_and_targets():
     assert np.all(np.isfinite(J)) and np.all(J >= 0)
     # the smallest scale is almost always accepted: J_0 ~ sigma_0^2 E|xi|^2 = 3e-4
     assert 0.5e-4 < np.median(J[:, 0]) < 1e-3
+
+
+def test_load_state_resynchronises_a_walker():
+    """The oracle's state loader (used to resynchronise on a device walker
+    after a float-level parting): a walker loaded with another's state
+    follows it exactly afterwards."""
+    w = init_weights(3)
+
+    def mk(seed):
+        return O.Walker(O.make_params(system=O.LJ, n=38, radius=3.2, method=O.SAD, method_param=0.03,
+                                      move_param=0.1, energy_bin=0.01, has_max_e=1, max_e=0.0, order_g=8,
+                                      seed=seed, mlp=w))
+
+    a, b = mk(1), mk(2)
+    a.run(3000)
+    b.run(1234)
+    b.load_state(a.scalars(), a.bins(), a.coords())
+    a.run(2000)
+    b.run(2000)
+    assert a.scalars() == b.scalars()
+    for k, v in a.bins().items():
+        np.testing.assert_array_equal(v, b.bins()[k], err_msg=k)
+    np.testing.assert_array_equal(a.coords(), b.coords())
