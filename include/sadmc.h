@@ -167,6 +167,10 @@ int sadmc_init(sadmc_ctx *ctx);
 /* ---- the hot path: K x move_once for every walker (energy.rs:904-974) - */
 /* Enqueue one launch advancing every walker of every set by `moves`. */
 int sadmc_run(sadmc_ctx *ctx, uint64_t moves);
+/* Enqueue one launch advancing only the walkers of set `set` by `moves`
+ * (move_once x moves for each of its walkers, energy.rs:904-974): the drain
+ * a MonteCarlo shim shares between the walkers of one set. */
+int sadmc_run_set(sadmc_ctx *ctx, int32_t set, uint64_t moves);
 /* Wait for the stream; returns the first walker failure as an error code. */
 int sadmc_sync(sadmc_ctx *ctx);
 /* Duration of the last sadmc_run launch in ms (HIP events on the context
@@ -223,9 +227,9 @@ int sadmc_prop_train(sadmc_ctx *ctx, int32_t steps, int32_t batch, float lr, flo
 /* ---- mean entropy on the device (plotting/parse-binning.py:150-171) ---
  * S(E) of every SAD walker of a set (ln w inside [too_lo, too_hi], extended
  * below with slope 1/min_T and outside with ln(H/<H>)), each walker shifted
- * to mean 0 over the energy window [e_a, e_b] (e_a >= e_b: the intersection
- * of the walkers' [too_lo, too_hi]), averaged over the walkers covering each
- * bin.  Writes bin k's centre as e_first + k * width, S[k] and count[k];
+ * to mean 0 over the energy window [e_a, e_b] (either one NaN: the
+ * intersection of the walkers' [too_lo, too_hi]; e_a >= e_b: SADMC_E_ARG),
+ * averaged over the walkers covering each bin.  Writes bin k's centre as e_first + k * width, S[k] and count[k];
  * returns SADMC_E_CAPACITY (with *nbins set) if max_bins is too small. */
 int sadmc_mean_entropy(sadmc_ctx *ctx, int32_t set, double e_a, double e_b, int64_t max_bins, double *e_first,
                        int64_t *nbins, double *S, int64_t *count);
@@ -321,8 +325,8 @@ int sadmc_bin_hires(sadmc_ctx *ctx, int32_t set, int32_t w, sadmc_hires *out, in
 /* ---- energy-ceiling replicas (src/mc/energy_replicas.rs, "ZMC") ---------
  * `sims` independent runs of Ising, LJ or WCA systems (the replicas binary;
  * geom: the LJ sphere radius or the WCA box length of each run),
- * each on one wave with up to max_replicas replicas (replicas * lanes
- * <= 512; the wave's lane groups take the replicas in turn).  from_params (energy_replicas.rs:344-395) runs on the host when
+ * each on one wave with up to max_replicas replicas (the wave's lane
+ * groups take the replicas in turn; a run's systems must fit 64 KiB of LDS).  from_params (energy_replicas.rs:344-395) runs on the host when
  * the set is added: 2^15 randomizations with seed_from_u64(seeds[s]), the
  * replicas [inf, E(1/2)] and [E(1/2), E(1/4)], the generator jumped, the
  * median estimator at E(1/4).  A round is MC::run_once
@@ -406,6 +410,36 @@ int sadmc_download_raw(sadmc_ctx *ctx, int32_t set, sadmc_walker_state *states, 
 /* Resume (mc/mod.rs:70-84): overwrite a whole set's device state. */
 int sadmc_upload_raw(sadmc_ctx *ctx, int32_t set, const sadmc_walker_state *states, const sadmc_bin *bins,
                      const double *coords, const int8_t *spins);
+
+/* ---- System trait queries (src/system/mod.rs:54-98) ------------------- */
+/* System::compute_energy (ising.rs:59-75, lj.rs:236-244, wca.rs:222-230):
+ * the walker's energy recomputed from its configuration on the device, FP64,
+ * pair terms in lane-group order. */
+int sadmc_compute_energy(sadmc_ctx *ctx, int32_t set, int32_t walker, double *e);
+/* System::verify_energy (lj.rs:249-261, wca.rs:237-250): recompute and
+ * compare with the walker's tracked E against expected_accuracy (|E| 1e-14
+ * N^2 for LJ, |E| 1e-13 N^2 for WCA; Ising has the trait's default, which
+ * checks nothing).  *egood gets the recomputed energy, *ok 1 or 0; a failed
+ * check returns SADMC_E_VERIFY (the reference panics). */
+int sadmc_verify_energy(sadmc_ctx *ctx, int32_t set, int32_t walker, double *egood, int32_t *ok);
+/* The constant trait queries of a system of this parameter set and geometry
+ * (sphere radius or box edge): lowest_possible_energy (lj.rs:245-248,
+ * wca.rs:234-236; none for Ising), delta_energy (ising.rs:76-78; none for
+ * the continuous systems), min_moves_to_randomize and dimensionality
+ * (ising.rs:86-91, lj.rs:280-285, wca.rs:268-273), max_size (ising.rs:120-122,
+ * lj.rs:375-377, wca.rs:354-356).  No device needed. */
+typedef struct {
+  int32_t has_lowest_possible_energy, has_delta_energy;
+  double lowest_possible_energy, delta_energy;
+  uint64_t min_moves_to_randomize, dimensionality;
+  double max_size;
+} sadmc_system_info_t;
+int sadmc_system_info(const sadmc_set_params *p, double geom, sadmc_system_info_t *out);
+/* Resume of the WCA "pressure" extra (energy.rs:146-163 Bins::extra;
+ * wca.rs:202-218): per physical bin, n_walkers x cap counts and totals in the
+ * layout of sadmc_upload_raw's bins.  SADMC_E_ARG for a set without
+ * collect_pressure. */
+int sadmc_upload_pressure(sadmc_ctx *ctx, int32_t set, const uint64_t *count, const double *total);
 
 /* ---- host-side helpers (no device needed) ----------------------------- */
 /* The initial configurations sadmc_add_set generates when coords/spins are

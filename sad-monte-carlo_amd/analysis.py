@@ -109,8 +109,12 @@ def mean_entropy(walkers, window=None):
     gmin = min(float(st["bins_min"]) for st, _ in walkers)
     if window is None:
         window = (max(float(st["too_lo"]) for st, _ in walkers), min(float(st["too_hi"]) for st, _ in walkers))
+    elif not window[0] < window[1]:
+        raise ValueError("empty energy window")
     ga = int(np.ceil((window[0] - gmin) / w - 0.5))
     gb = int(np.floor((window[1] - gmin) / w - 0.5)) + 1
+    if gb <= ga:
+        raise ValueError("no common energy window for the walkers")
     offs = [int(round((float(st["bins_min"]) - gmin) / w)) for st, _ in walkers]
     n = max(o + int(st["len"]) for o, (st, _) in zip(offs, walkers))
     total = np.zeros(n)

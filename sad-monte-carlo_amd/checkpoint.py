@@ -1,6 +1,7 @@
 """Checkpoint documents in the layout the reference serialises `EnergyMC`
 with (serde, externally tagged enums; src/mc/energy.rs:145-245,
-src/system/{lj,ising,wca}.rs), as YAML.
+src/system/{lj,ising,wca}.rs), as YAML, JSON or CBOR chosen by the file's
+extension as the reference does (src/mc/mod.rs:72-79, 104-119).
 
 A document carries what `plotting/parse-binning.py:102-171` reads (`moves`,
 `bins.{min, width, histogram, lnw, energy_total, energy_squared_total,
@@ -8,17 +9,31 @@ extra}`, `method.Sad.{too_lo, too_hi, min_T}`) and everything a walker needs
 to resume on the device bit for bit: the method state, the counters, the
 acceptance rate and translation scale, the generator state
 (`rng.{s0, s1}`), `t_found`, `round_trips`, `have_visited_since_maxentropy`,
-`max_S`, and the system (energy, its error estimate, and the positions or
-spins).  Not written: the reference's plugin state (`report`, `movies`,
-`save`, `manager`) and, for WCA, its cell lists (`optcell.rs`), which a
-resume on the device rebuilds from the positions; `save_as` is the file
-name.  Units are plain numbers in sigma and epsilon.
+`max_S`, WCA's `pressure` extra, and the system (energy, its error estimate,
+and the positions or spins).  `time_L` is written as 0: the reference
+initialises it to 0 (energy.rs:862) and never changes it.  Not written: the
+reference's plugin state (`report`, `movies`, `save`, `manager`) and, for
+WCA, its cell lists (`optcell.rs`), which a resume on the device rebuilds
+from the positions; `save_as` is the file name.  Units are plain numbers in
+sigma and epsilon.
 
 `to_document` builds one walker's document from the device views
 (`Context.state`, `Context.bins`, `Context.coords`/`spins`);
 `from_document` turns it back into the raw records `Context.upload_raw`
-takes.  `Context`-level helpers: `save_set` and `resume_set`.
+takes (and the pressure rows `Context.upload_pressure` takes).
+`Context`-level helpers: `save_set` and `resume_set`; files: `save`, `load`.
+
+CBOR follows RFC 8949 with definite lengths, maps with text keys in field
+order, and each float in the shortest of half, single or double precision
+that holds it exactly (the preferred serialisation, which serde_cbor's
+float writer also uses as recalled); byte equality with serde_cbor's output
+is parity unpinned (no Rust build here), while every value round-trips
+exactly.
 """
+import json
+import os
+import struct
+
 import numpy as np
 import yaml
 
@@ -109,7 +124,9 @@ def loads(text):
 
 def from_document(doc, params, cap, e_lo, geom):
     """(walker-state record, physical bin row of length cap, coords or None,
-    spins or None) for Context.upload_raw, from a checkpoint document."""
+    spins or None, pressure rows or None) from a checkpoint document: the
+    first four for Context.upload_raw, the pressure extra's physical rows
+    ({"count", "total"}, length cap) for Context.upload_pressure."""
     st = np.zeros((), S.WALKER_STATE_DTYPE)
     b = doc["bins"]
     n = len(b["lnw"])
@@ -180,15 +197,19 @@ def from_document(doc, params, cap, e_lo, geom):
         h = method["WL"]["hist"]
         rows["wl_hist"][base:base + len(h)] = h
     extra = b.get("extra") or {}
+    pressure = None
     if "pressure" in extra:
         st["has_pressure"] = 1
+        pressure = {"count": np.zeros(cap, np.uint64), "total": np.zeros(cap)}
+        pressure["count"][sl] = extra["pressure"]["count"]
+        pressure["total"][sl] = extra["pressure"]["total"]
     coords = spins = None
     if "Ising" in doc["system"]:
         spins = np.asarray(sysdoc["S"], dtype=np.int8)
     else:
         pos = sysdoc["positions"] if "positions" in sysdoc else sysdoc["cell"]["positions"]
         coords = np.asarray(pos, dtype=np.float64).T.reshape(-1)
-    return st, rows, coords, spins
+    return st, rows, coords, spins, pressure
 
 
 def save_set(ctx, s):
@@ -216,14 +237,199 @@ def resume_set(ctx, s, docs):
     states = np.zeros(W, S.WALKER_STATE_DTYPE)
     rows = np.zeros((W, cap), S.BIN_DTYPE)
     coords, spins = [], []
+    p_count = p_total = None
+    if params.system == S.WCA and params.collect_pressure:
+        p_count, p_total = np.zeros((W, cap), np.uint64), np.zeros((W, cap))
     cur = ctx.download_raw(s, with_bins=False)[0]
     for w, doc in enumerate(docs):
-        st, r, c, sp = from_document(doc, params, cap, e_lo, float(cur[w]["geom"]))
+        st, r, c, sp, pr = from_document(doc, params, cap, e_lo, float(cur[w]["geom"]))
         states[w] = st
         rows[w] = r
         if c is not None:
             coords.append(c)
         if sp is not None:
             spins.append(sp)
+        if pr is not None:
+            if p_count is None:
+                raise ValueError("a pressure extra in the document, but the set does not collect the pressure")
+            p_count[w], p_total[w] = pr["count"], pr["total"]
     ctx.upload_raw(s, states, rows, coords=np.concatenate(coords) if coords else None,
                    spins=np.concatenate(spins) if spins else None)
+    if p_count is not None:
+        ctx.upload_pressure(s, p_count, p_total)
+
+
+# ----------------------------------------------------------------- CBOR
+
+def _cbor_head(major, n):
+    if n < 24:
+        return bytes([(major << 5) | n])
+    for ai, fmt in ((24, ">B"), (25, ">H"), (26, ">I"), (27, ">Q")):
+        if n < 1 << (8 * struct.calcsize(fmt)):
+            return bytes([(major << 5) | ai]) + struct.pack(fmt, n)
+    raise ValueError("integer out of CBOR's range")
+
+
+def _cbor_float(x):
+    if x != x:
+        return b"\xf9\x7e\x00"  # the canonical NaN
+    for head, fmt in ((b"\xf9", ">e"), (b"\xfa", ">f")):
+        try:
+            b = struct.pack(fmt, x)
+        except OverflowError:
+            continue
+        if struct.unpack(fmt, b)[0] == x:
+            return head + b
+    return b"\xfb" + struct.pack(">d", x)
+
+
+def _cbor_enc(v, out):
+    if v is None:
+        out.append(b"\xf6")
+    elif v is True:
+        out.append(b"\xf5")
+    elif v is False:
+        out.append(b"\xf4")
+    elif isinstance(v, int):
+        out.append(_cbor_head(0, v) if v >= 0 else _cbor_head(1, -1 - v))
+    elif isinstance(v, float):
+        out.append(_cbor_float(v))
+    elif isinstance(v, str):
+        b = v.encode("utf-8")
+        out.append(_cbor_head(3, len(b)))
+        out.append(b)
+    elif isinstance(v, (list, tuple)):
+        out.append(_cbor_head(4, len(v)))
+        for x in v:
+            _cbor_enc(x, out)
+    elif isinstance(v, dict):
+        out.append(_cbor_head(5, len(v)))
+        for k, x in v.items():
+            _cbor_enc(str(k), out)
+            _cbor_enc(x, out)
+    else:
+        raise TypeError(f"no CBOR encoding for {type(v).__name__}")
+
+
+def cbor_dumps(doc):
+    out = []
+    _cbor_enc(doc, out)
+    return b"".join(out)
+
+
+def cbor_loads(data):
+    pos = 0
+
+    def take(k):
+        nonlocal pos
+        if pos + k > len(data):
+            raise ValueError("truncated CBOR")
+        b = data[pos:pos + k]
+        pos += k
+        return b
+
+    def arg(ai):
+        if ai < 24:
+            return ai
+        if ai in (24, 25, 26, 27):
+            return int.from_bytes(take(1 << (ai - 24)), "big")
+        if ai == 31:
+            return None  # indefinite length
+        raise ValueError("reserved CBOR additional information")
+
+    def item():
+        ib = take(1)[0]
+        major, ai = ib >> 5, ib & 31
+        if major == 0:
+            return arg(ai)
+        if major == 1:
+            return -1 - arg(ai)
+        if major in (2, 3):
+            n = arg(ai)
+            if n is None:
+                parts = []
+                while data[pos] != 0xFF:
+                    parts.append(item())
+                take(1)
+                raw = b"".join(p if isinstance(p, bytes) else p.encode("utf-8") for p in parts)
+            else:
+                raw = take(n)
+            return raw if major == 2 else raw.decode("utf-8")
+        if major == 4:
+            n = arg(ai)
+            if n is None:
+                xs = []
+                while data[pos] != 0xFF:
+                    xs.append(item())
+                take(1)
+                return xs
+            return [item() for _ in range(n)]
+        if major == 5:
+            n = arg(ai)
+            d = {}
+            if n is None:
+                while data[pos] != 0xFF:
+                    k = item()
+                    d[k] = item()
+                take(1)
+                return d
+            for _ in range(n):
+                k = item()
+                d[k] = item()
+            return d
+        if major == 6:
+            arg(ai)
+            return item()  # tags carry no meaning in these documents
+        if ai == 20:
+            return False
+        if ai == 21:
+            return True
+        if ai in (22, 23):
+            return None
+        if ai == 25:
+            return float(struct.unpack(">e", take(2))[0])
+        if ai == 26:
+            return float(struct.unpack(">f", take(4))[0])
+        if ai == 27:
+            return struct.unpack(">d", take(8))[0]
+        raise ValueError(f"unsupported CBOR simple value {ai}")
+
+    v = item()
+    if pos != len(data):
+        raise ValueError("trailing bytes after the CBOR item")
+    return v
+
+
+# ---------------------------------------------------------------- files
+
+def save(doc, path):
+    """Write one document, in the format the extension names (yaml, json or
+    cbor, src/mc/mod.rs:113-119); another extension is an error, as the
+    reference panics.  The file is replaced atomically (AtomicFile)."""
+    ext = os.path.splitext(str(path))[1].lstrip(".")
+    if ext == "yaml":
+        data = dumps(doc).encode("utf-8")
+    elif ext == "json":
+        data = json.dumps(doc).encode("utf-8")
+    elif ext == "cbor":
+        data = cbor_dumps(doc)
+    else:
+        raise ValueError(f"I don't know how to create file {path!r}")
+    tmp = f"{path}.tmp{os.getpid()}"
+    with open(tmp, "wb") as f:
+        f.write(data)
+    os.replace(tmp, path)
+
+
+def load(path):
+    """Read one document by the extension (src/mc/mod.rs:72-79)."""
+    ext = os.path.splitext(str(path))[1].lstrip(".")
+    with open(path, "rb") as f:
+        data = f.read()
+    if ext == "yaml":
+        return loads(data.decode("utf-8"))
+    if ext == "json":
+        return json.loads(data.decode("utf-8"))
+    if ext == "cbor":
+        return cbor_loads(data)
+    raise ValueError(f"I don't know how to read file {path!r}")

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "sadmc_kernels.h"
+#include "sadmc_internal.h"
 
 using namespace sadmc;
 
@@ -67,6 +68,7 @@ struct SetHost {
   PExtra *d_extra = nullptr;
   Pending *d_pending = nullptr;
   int32_t *d_flag = nullptr;
+  double *d_eout = nullptr;  // a recomputed energy (sadmc_compute_energy)
   SetArgs args;
 };
 
@@ -366,6 +368,8 @@ static void free_set(SetHost &s) {
   (void)hipFree(s.d_extra);
   (void)hipFree(s.d_pending);
   (void)hipFree(s.d_flag);
+  (void)hipFree(s.d_eout);
+  s.d_eout = nullptr;
   s.d_ws = nullptr;
   s.d_pos = nullptr;
   s.d_spins = nullptr;
@@ -613,6 +617,7 @@ int sadmc_init(sadmc_ctx *c) {
       HIPCHECK(c, hipMemsetAsync(s.d_wlh, 0, sizeof(uint64_t) * nb, c->stream));
     }
     HIPCHECK(c, hipMalloc(&s.d_pending, sizeof(Pending)));
+    HIPCHECK(c, hipMalloc(&s.d_eout, sizeof(double)));
     HIPCHECK(c, hipMalloc(&s.d_flag, sizeof(int32_t)));
     HIPCHECK(c, hipMemsetAsync(s.d_flag, 0, sizeof(int32_t), c->stream));
     std::vector<sadmc_walker_state> ws(W);
@@ -672,6 +677,21 @@ int sadmc_run(sadmc_ctx *c, uint64_t moves) {
   return SADMC_OK;
 }
 
+int sadmc_run_set(sadmc_ctx *c, int32_t set, uint64_t moves) {
+  if (!c) return SADMC_E_ARG;
+  if (!c->inited) return fail(c, SADMC_E_STATE, "sadmc_init first");
+  if (set < 0 || set >= (int32_t)c->sets.size()) return fail(c, SADMC_E_ARG, "bad set");
+  HIPCHECK(c, hipSetDevice(c->device));
+  HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
+  SetHost &s = c->sets[set];
+  LaunchArgs la{};
+  la.moves = moves;
+  HIPCHECK(c, launch(s.p.system, s.p.precision, s.p.method, s.KMAX, KK_RUN, s.args, la, c->stream));
+  HIPCHECK(c, hipEventRecord(c->ev1, c->stream));
+  c->timed = true;
+  return SADMC_OK;
+}
+
 int sadmc_sync(sadmc_ctx *c) {
   if (!c) return SADMC_E_ARG;
   HIPCHECK(c, hipStreamSynchronize(c->stream));
@@ -718,10 +738,18 @@ static int build_batch(sadmc_ctx *c) {
   for (size_t k = 0; k < tiles.size(); k++) bt[k] = tiles[k].t;
   std::vector<SetArgs> sa(c->sets.size());
   for (size_t k = 0; k < c->sets.size(); k++) sa[k] = c->sets[k].args;
-  HIPCHECK(c, hipMalloc(&c->d_sets, sizeof(SetArgs) * sa.size()));
-  HIPCHECK(c, hipMemcpy(c->d_sets, sa.data(), sizeof(SetArgs) * sa.size(), hipMemcpyHostToDevice));
-  HIPCHECK(c, hipMalloc(&c->d_tiles, sizeof(BatchTile) * bt.size()));
-  HIPCHECK(c, hipMemcpy(c->d_tiles, bt.data(), sizeof(BatchTile) * bt.size(), hipMemcpyHostToDevice));
+  // both tables or neither: a failure part way leaves nothing allocated
+  hipError_t e = hipMalloc(&c->d_sets, sizeof(SetArgs) * sa.size());
+  if (e == hipSuccess) e = hipMemcpy(c->d_sets, sa.data(), sizeof(SetArgs) * sa.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMalloc(&c->d_tiles, sizeof(BatchTile) * bt.size());
+  if (e == hipSuccess) e = hipMemcpy(c->d_tiles, bt.data(), sizeof(BatchTile) * bt.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(c->d_sets);
+    (void)hipFree(c->d_tiles);
+    c->d_sets = nullptr;
+    c->d_tiles = nullptr;
+    return fail(c, SADMC_E_HIP, std::string("batch tables: ") + hipGetErrorString(e));
+  }
   c->ntiles = (int)bt.size();
   c->batch_lds = lds;
   return SADMC_OK;
@@ -1011,37 +1039,69 @@ int sadmc_prop_train(sadmc_ctx *c, int32_t steps, int32_t batch, float lr, float
   return SADMC_OK;
 }
 
-int sadmc_mean_entropy(sadmc_ctx *c, int32_t set, double e_a, double e_b, int64_t max_bins, double *e_first,
-                       int64_t *nbins, double *S, int64_t *count) {
+}  // extern "C"
+
+int ctx_device(const sadmc_ctx *c) { return c->device; }
+int ctx_fail(sadmc_ctx *c, int code, const char *msg) { return fail(c, code, msg); }
+
+// The walker-set entropy reduction with its scratch either grown on demand
+// in the context (fixed_base null) or taken from a fixed device buffer of
+// `fixed_cap` bytes at `fixed_base` (no allocation; sizes beyond it fail
+// with SADMC_E_CAPACITY).
+// Scratch layout: the grid summary and its partial records, the per-walker
+// constants, S and counts, the chunks' partial sums.  Every kernel writes
+// each scratch entry it later reads, so nothing needs clearing first.
+int mean_entropy_impl(sadmc_ctx *c, int32_t set, double e_a, double e_b, int64_t max_bins, double *e_first,
+                      int64_t *nbins, double *S, int64_t *count, char *fixed_base, size_t fixed_cap) {
   SetHost *s = get_set(c, set, 0);
   if (!s || !c->inited) return fail(c, SADMC_E_ARG, "bad set or not initialised");
   if (s->p.method != SADMC_SAD) return fail(c, SADMC_E_ARG, "the entropy reduction reads SAD walkers");
   HIPCHECK(c, hipSetDevice(c->device));
   const double w = s->width;
-  // scratch: the grid summary, the per-walker constants, S and counts, the
-  // chunks' partial sums; one allocation reused by later calls
   const size_t al = 256;
   auto up = [&](size_t b) { return (b + al - 1) / al * al; };
   const size_t ew_off = up(sizeof(EntropyRange) * (1 + kEntropyRangeBlocks));
   const size_t fixed = ew_off + up(entropy_walker_bytes() * s->W);
-  if (c->ent_bytes < fixed) {
-    (void)hipFree(c->d_ent);
-    c->d_ent = nullptr;
-    c->ent_bytes = 0;
-    HIPCHECK(c, hipMalloc(&c->d_ent, fixed));
-    c->ent_bytes = fixed;
-  }
+  // room for `bytes` of scratch; returns its base or null (error set)
+  auto room = [&](size_t bytes) -> char * {
+    if (fixed_base) {
+      if (bytes > fixed_cap) {
+        char buf[200];
+        std::snprintf(buf, sizeof buf,
+                      "the reduction needs %zu bytes of scratch; the allocation-free entry point holds %zu", bytes,
+                      fixed_cap);
+        fail(c, SADMC_E_CAPACITY, buf);
+        return nullptr;
+      }
+      return fixed_base;
+    }
+    if (c->ent_bytes < bytes) {
+      (void)hipFree(c->d_ent);
+      c->d_ent = nullptr;
+      c->ent_bytes = 0;
+      hipError_t e = hipMalloc(&c->d_ent, bytes);
+      if (e != hipSuccess) {
+        fail(c, SADMC_E_HIP, std::string("entropy scratch: ") + hipGetErrorString(e));
+        return nullptr;
+      }
+      c->ent_bytes = bytes;
+    }
+    return (char *)c->d_ent;
+  };
+  char *base = room(fixed);
+  if (!base) return fixed_base ? SADMC_E_CAPACITY : SADMC_E_HIP;
   EntropyRange range;
-  HIPCHECK(c, launch_entropy_range(s->d_ws, s->W, w, (EntropyRange *)c->d_ent, (EntropyRange *)c->d_ent + 1,
-                                   c->stream));
-  HIPCHECK(c, hipMemcpyAsync(&range, c->d_ent, sizeof range, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, launch_entropy_range(s->d_ws, s->W, w, (EntropyRange *)base, (EntropyRange *)base + 1, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(&range, base, sizeof range, hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   if (!range.any) return fail(c, SADMC_E_STATE, "no running walkers");
   const double gmin = range.gmin;
   const int64_t ng = range.ng;
-  if (!(e_a < e_b)) {
+  if (std::isnan(e_a) || std::isnan(e_b)) {
     e_a = range.lo;
     e_b = range.hi;
+  } else if (!(e_a < e_b)) {
+    return fail(c, SADMC_E_ARG, "empty energy window (e_a >= e_b)");
   }
   const int64_t ga = (int64_t)std::ceil((e_a - gmin) / w - 0.5), gb = (int64_t)std::floor((e_b - gmin) / w - 0.5) + 1;
   if (!(gb > ga)) return fail(c, SADMC_E_STATE, "no common energy window for the walkers");
@@ -1051,14 +1111,8 @@ int sadmc_mean_entropy(sadmc_ctx *c, int32_t set, double e_a, double e_b, int64_
   const int chunks = entropy_chunks(s->W, ng);
   const size_t s_off = fixed, c_off = s_off + up(sizeof(double) * ng), ps_off = c_off + up(sizeof(int64_t) * ng),
                pc_off = ps_off + up(sizeof(double) * ng * chunks), total = pc_off + up(sizeof(int64_t) * ng * chunks);
-  if (c->ent_bytes < total) {
-    (void)hipFree(c->d_ent);
-    c->d_ent = nullptr;
-    c->ent_bytes = 0;
-    HIPCHECK(c, hipMalloc(&c->d_ent, total));
-    c->ent_bytes = total;
-  }
-  char *base = (char *)c->d_ent;
+  base = room(total);
+  if (!base) return fixed_base ? SADMC_E_CAPACITY : SADMC_E_HIP;
   double *dS = (double *)(base + s_off), *dPS = (double *)(base + ps_off);
   int64_t *dC = (int64_t *)(base + c_off), *dPC = (int64_t *)(base + pc_off);
   int rc = SADMC_OK;
@@ -1070,6 +1124,95 @@ int sadmc_mean_entropy(sadmc_ctx *c, int32_t set, double e_a, double e_b, int64_
     rc = fail(c, SADMC_E_HIP, "entropy reduction failed");
   }
   return rc;
+}
+
+extern "C" {
+
+int sadmc_mean_entropy(sadmc_ctx *c, int32_t set, double e_a, double e_b, int64_t max_bins, double *e_first,
+                       int64_t *nbins, double *S, int64_t *count) {
+  if (!c) return SADMC_E_ARG;
+  return mean_entropy_impl(c, set, e_a, e_b, max_bins, e_first, nbins, S, count, nullptr, 0);
+}
+
+int sadmc_compute_energy(sadmc_ctx *c, int32_t set, int32_t walker, double *e) {
+  SetHost *s = get_set(c, set, walker);
+  if (!s || !e || !c->inited) return fail(c, SADMC_E_ARG, "bad set/walker or not initialised");
+  HIPCHECK(c, hipSetDevice(c->device));
+  LaunchArgs la{};
+  la.walker = walker;
+  la.eout = s->d_eout;
+  HIPCHECK(c, launch(s->p.system, s->p.precision, s->p.method, s->KMAX, KK_ENERGY, s->args, la, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(e, s->d_eout, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  return SADMC_OK;
+}
+
+int sadmc_verify_energy(sadmc_ctx *c, int32_t set, int32_t walker, double *egood, int32_t *ok) {
+  SetHost *s = get_set(c, set, walker);
+  if (!s || !c->inited) return fail(c, SADMC_E_ARG, "bad set/walker or not initialised");
+  double eg = 0.0;
+  int rc = sadmc_compute_energy(c, set, walker, &eg);
+  if (rc) return rc;
+  sadmc_walker_state st;
+  rc = sadmc_get_walker_state(c, set, walker, &st);
+  if (rc) return rc;
+  if (egood) *egood = eg;
+  // lj.rs:106-108 / wca.rs:178-180 expected_accuracy; Ising keeps the
+  // trait's default verify_energy, which checks nothing
+  const double nn = (double)s->p.n;
+  const double expected = s->p.system == SADMC_LJ    ? std::fabs(st.E) * 1e-14 * nn * nn
+                          : s->p.system == SADMC_WCA ? std::fabs(st.E) * 1e-13 * nn * nn
+                                                     : INFINITY;
+  const bool good = !(std::fabs(eg - st.E) > expected);
+  if (ok) *ok = good ? 1 : 0;
+  if (!good) {
+    char buf[200];
+    std::snprintf(buf, sizeof buf, "set %d walker %d: error in E is %.17g when it should be < %.17g", set, walker,
+                  eg - st.E, expected);
+    return fail(c, SADMC_E_VERIFY, buf);
+  }
+  return SADMC_OK;
+}
+
+int sadmc_system_info(const sadmc_set_params *p, double geom, sadmc_system_info_t *out) {
+  if (!p || !out) return SADMC_E_ARG;
+  std::memset(out, 0, sizeof *out);
+  const int n = p->n;
+  if (p->system == SADMC_ISING) {
+    if (n < 2 || n > 8) return SADMC_E_ARG;
+    out->has_delta_energy = 1;  // ising.rs:76-78
+    out->delta_energy = 4.0;
+    out->min_moves_to_randomize = (uint64_t)n * (uint64_t)n;  // ising.rs:86-91
+    out->dimensionality = out->min_moves_to_randomize;
+    out->max_size = 0.5;  // ising.rs:120-122
+  } else if (p->system == SADMC_LJ || p->system == SADMC_WCA) {
+    if (n < 2) return SADMC_E_ARG;
+    out->has_lowest_possible_energy = 1;  // lj.rs:245-248, wca.rs:234-236
+    out->lowest_possible_energy = p->system == SADMC_LJ ? -0.5 * (double)n * ((double)n - 1.0) : 0.0;
+    out->min_moves_to_randomize = (uint64_t)n;  // lj.rs:280-285, wca.rs:268-273
+    out->dimensionality = 3 * (uint64_t)n;
+    // lj.rs:375-377 (the sphere radius), wca.rs:354-356 (the box diagonal)
+    out->max_size = p->system == SADMC_LJ ? geom : std::sqrt(3.0 * geom * geom);
+  } else {
+    return SADMC_E_ARG;
+  }
+  return SADMC_OK;
+}
+
+int sadmc_upload_pressure(sadmc_ctx *c, int32_t set, const uint64_t *count, const double *total) {
+  if (!c || set < 0 || set >= (int32_t)c->sets.size() || !c->inited || !count || !total)
+    return fail(c, SADMC_E_ARG, "bad arguments");
+  SetHost &s = c->sets[set];
+  if (!s.d_extra) return fail(c, SADMC_E_ARG, "the set does not collect the pressure");
+  const size_t nb = (size_t)s.W * s.cap;
+  std::vector<PExtra> x(nb);
+  for (size_t k = 0; k < nb; k++) {
+    x[k].count = count[k];
+    x[k].total = total[k];
+  }
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  HIPCHECK(c, hipMemcpy(s.d_extra, x.data(), sizeof(PExtra) * nb, hipMemcpyHostToDevice));
+  return SADMC_OK;
 }
 
 int sadmc_walker_count(const sadmc_ctx *c, int32_t set, int32_t *n, int64_t *cap) {
@@ -1807,7 +1950,6 @@ int sadmc_zmc_add(sadmc_ctx *c, const sadmc_set_params *p, int32_t S, const uint
   } else {
     return fail(c, SADMC_E_ARG, "unknown system");
   }
-  if ((int64_t)RM * z.G > 512) return fail(c, SADMC_E_ARG, "max_replicas * lanes must be <= 512");
   if (zmc_lds_bytes(RM, z.npad) > 65536) return fail(c, SADMC_E_ARG, "a run's systems do not fit 64 KiB of LDS");
   z.threads = 64;  // one wave per run; its lane groups take the slots in turn
   const int steps = z.sys == SYS_ISING ? n * n : n;

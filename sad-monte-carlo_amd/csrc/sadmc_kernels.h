@@ -11,7 +11,7 @@
 
 namespace sadmc {
 
-enum KernelKind { KK_RUN = 0, KK_INIT = 1, KK_PLAN = 2, KK_CONFIRM = 3, KK_PROBE = 4 };
+enum KernelKind { KK_RUN = 0, KK_INIT = 1, KK_PLAN = 2, KK_CONFIRM = 3, KK_PROBE = 4, KK_ENERGY = 5 };
 
 struct LaunchArgs {
   uint64_t moves;
@@ -22,6 +22,7 @@ struct LaunchArgs {
   uint64_t seed;  // probe: walker w uses the stream seeded by seed + w
   int count;      // probe: records per walker
   float *out;     // probe: [W][count][24]
+  double *eout;   // energy: the recomputed energy of `walker`
 };
 
 // One workgroup of the batched launch: walkers [first_walker, first_walker +
@@ -203,6 +204,20 @@ __global__ __launch_bounds__(64) void k_confirm(SetArgs a, int walker, const Pen
   e.store(walker);
 }
 
+// System::compute_energy (ising.rs:59-75, lj.rs:236-244, wca.rs:222-230) of
+// one walker's current configuration, FP64 in lane-group order.
+template <int SYS, int KMAX, int PREC, int METH>
+__global__ __launch_bounds__(64) void k_energy(SetArgs a, int walker, double *out) {
+  extern __shared__ double lds[];
+  if ((int)threadIdx.x >= a.G) return;
+  Engine<SYS, KMAX, PREC, METH> e;
+  e.load_system(a, walker, threadIdx.x, lds);
+  e.L = a.ws[walker].geom;
+  e.halfL = 0.5 * e.L;
+  const double en = e.full_energy();
+  if (threadIdx.x == 0) *out = en;
+}
+
 template <int SYS, int KMAX, int PREC, int METH>
 static hipError_t launch_t(KernelKind kk, const SetArgs &a, const LaunchArgs &la, hipStream_t s) {
   const int G = a.G;
@@ -223,6 +238,9 @@ static hipError_t launch_t(KernelKind kk, const SetArgs &a, const LaunchArgs &la
       break;
     case KK_CONFIRM:
       hipLaunchKernelGGL((k_confirm<SYS, KMAX, PREC, METH>), dim3(1), block, lds1, s, a, la.walker, la.pending);
+      break;
+    case KK_ENERGY:
+      hipLaunchKernelGGL((k_energy<SYS, KMAX, PREC, METH>), dim3(1), block, lds1, s, a, la.walker, la.eout);
       break;
     default:
       return hipErrorInvalidValue;

@@ -116,6 +116,15 @@ HIRES_DTYPE = np.dtype([
 ])
 
 
+class SystemInfo(C.Structure):
+    """sadmc_system_info_t: the constant System trait queries."""
+    _fields_ = [
+        ("has_lowest_possible_energy", C.c_int32), ("has_delta_energy", C.c_int32),
+        ("lowest_possible_energy", C.c_double), ("delta_energy", C.c_double),
+        ("min_moves_to_randomize", C.c_uint64), ("dimensionality", C.c_uint64), ("max_size", C.c_double),
+    ]
+
+
 class WalkerStateC(C.Structure):
     _fields_ = [(name, C.c_uint8 * WALKER_STATE_DTYPE.itemsize) for name in ("raw",)]
 
@@ -124,7 +133,8 @@ class WalkerStateC(C.Structure):
 # CPU test suite).
 EXPORTS = [
     "sadmc_abi_version", "sadmc_create", "sadmc_destroy", "sadmc_last_error", "sadmc_add_set", "sadmc_init",
-    "sadmc_run", "sadmc_sync", "sadmc_last_run_ms", "sadmc_plan_move", "sadmc_confirm", "sadmc_walker_count",
+    "sadmc_run", "sadmc_run_set", "sadmc_sync", "sadmc_last_run_ms",
+    "sadmc_compute_energy", "sadmc_verify_energy", "sadmc_system_info", "sadmc_upload_pressure", "sadmc_plan_move", "sadmc_confirm", "sadmc_walker_count",
     "sadmc_get_walker_state", "sadmc_walker_bins", "sadmc_walker_coords", "sadmc_walker_spins",
     "sadmc_download_raw", "sadmc_upload_raw", "sadmc_init_coords", "sadmc_init_spins", "sadmc_bin_window",
     "sadmc_run_batched", "sadmc_graph_run", "sadmc_autotune_lanes",
@@ -158,6 +168,11 @@ def lib():
     L.sadmc_init.argtypes = [vp]
     L.sadmc_run.argtypes = [vp, u64]
     L.sadmc_sync.argtypes = [vp]
+    L.sadmc_run_set.argtypes = [vp, i32, u64]
+    L.sadmc_compute_energy.argtypes = [vp, i32, i32, C.POINTER(d)]
+    L.sadmc_verify_energy.argtypes = [vp, i32, i32, C.POINTER(d), C.POINTER(i32)]
+    L.sadmc_system_info.argtypes = [C.POINTER(SetParams), d, C.POINTER(SystemInfo)]
+    L.sadmc_upload_pressure.argtypes = [vp, i32, vp, vp]
     L.sadmc_last_run_ms.argtypes = [vp, C.POINTER(d)]
     L.sadmc_plan_move.argtypes = [vp, i32, i32, d, C.POINTER(d), C.POINTER(i32)]
     L.sadmc_confirm.argtypes = [vp, i32, i32]
@@ -302,6 +317,30 @@ class Context:
         if sync:
             self.sync()
 
+    def run_set(self, s, moves, sync=True):
+        """Advance only the walkers of set s by `moves`."""
+        self._check(lib().sadmc_run_set(self.h, s, moves), "run_set")
+        if sync:
+            self.sync()
+
+    def compute_energy(self, s, w):
+        """System::compute_energy of walker w, recomputed on the device."""
+        e = C.c_double()
+        self._check(lib().sadmc_compute_energy(self.h, s, w, C.byref(e)), "compute_energy")
+        return e.value
+
+    def verify_energy(self, s, w):
+        """System::verify_energy: (ok, recomputed energy); a failed check
+        raises SadmcError (code -4), as the reference panics."""
+        e, ok = C.c_double(), C.c_int32()
+        self._check(lib().sadmc_verify_energy(self.h, s, w, C.byref(e), C.byref(ok)), "verify_energy")
+        return bool(ok.value), e.value
+
+    def upload_pressure(self, s, count, total):
+        """The WCA pressure extra of every walker, [W][cap] physical bins."""
+        self._check(lib().sadmc_upload_pressure(self.h, s, _p(np.ascontiguousarray(count, np.uint64)),
+                                                _p(np.ascontiguousarray(total, np.float64))), "upload_pressure")
+
     def run_batched(self, moves, sync=True, graph=False):
         fn = lib().sadmc_graph_run if graph else lib().sadmc_run_batched
         self._check(fn(self.h, moves), "graph_run" if graph else "run_batched")
@@ -310,7 +349,7 @@ class Context:
 
     def mean_entropy(self, s, window=None):
         """(E, S, count) of a SAD set reduced on the device (bins no walker covers dropped)."""
-        ea, eb = window if window is not None else (0.0, 0.0)
+        ea, eb = window if window is not None else (float("nan"), float("nan"))
         e0, nb = C.c_double(), C.c_int64()
         # every walker's bins lie in the set's window of `cap` slots, so the
         # common grid has at most `cap` bins: one call normally suffices
@@ -579,6 +618,20 @@ class Context:
                                            _p(None if coords is None else np.ascontiguousarray(coords, np.float64)),
                                            _p(None if spins is None else np.ascontiguousarray(spins, np.int8))),
                     "upload_raw")
+
+
+def system_info(params, geom=0.0):
+    """The System trait's constant queries for a parameter set: a dict with
+    lowest_possible_energy / delta_energy (None where the reference returns
+    None), min_moves_to_randomize, dimensionality and max_size."""
+    out = SystemInfo()
+    rc = lib().sadmc_system_info(C.byref(params), geom, C.byref(out))
+    if rc:
+        raise SadmcError(rc, "system_info")
+    return dict(lowest_possible_energy=out.lowest_possible_energy if out.has_lowest_possible_energy else None,
+                delta_energy=out.delta_energy if out.has_delta_energy else None,
+                min_moves_to_randomize=out.min_moves_to_randomize, dimensionality=out.dimensionality,
+                max_size=out.max_size)
 
 
 def logical_bins(state, bins_row):

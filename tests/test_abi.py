@@ -111,3 +111,23 @@ def test_no_cpu_fallback_without_device():
 def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run(["python3", os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_system_info_matches_the_reference_trait_constants():
+    """System trait constants (src/system/mod.rs:54-98) per system:
+    lowest_possible_energy (lj.rs:245-248, wca.rs:234-236; none for Ising),
+    delta_energy (ising.rs:76-78), min_moves_to_randomize / dimensionality
+    (ising.rs:86-91, lj.rs:280-285, wca.rs:268-273), max_size (ising.rs:120-122,
+    lj.rs:375-377, wca.rs:354-356)."""
+    i = S.system_info(S.make_params(system=S.ISING, n=8))
+    assert i == dict(lowest_possible_energy=None, delta_energy=4.0, min_moves_to_randomize=64, dimensionality=64,
+                     max_size=0.5)
+    lj = S.system_info(S.make_params(system=S.LJ, n=31), 3.0)
+    assert lj == dict(lowest_possible_energy=-465.0, delta_energy=None, min_moves_to_randomize=31,
+                      dimensionality=93, max_size=3.0)
+    w = S.system_info(S.make_params(system=S.WCA, n=64), 4.0)
+    assert w["lowest_possible_energy"] == 0.0 and w["delta_energy"] is None
+    assert (w["min_moves_to_randomize"], w["dimensionality"]) == (64, 192)
+    assert w["max_size"] == np.sqrt(48.0)
+    with pytest.raises(S.SadmcError):
+        S.system_info(S.make_params(system=S.ISING, n=9))

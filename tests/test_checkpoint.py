@@ -59,7 +59,8 @@ def test_document_to_device_records():
     doc = CK.loads(CK.dumps(CK.to_document(p, st, bins, coords=coords)))
     cap = 4500
     e_lo = -45.326801
-    rec, rows, c, sp = CK.from_document(doc, p, cap, e_lo, 2.5)
+    rec, rows, c, sp, pr = CK.from_document(doc, p, cap, e_lo, 2.5)
+    assert pr is None
     base = int(rec["base"])
     assert 0 <= base and base + 40 <= cap
     for k in ("E", "error", "moves", "accepted_moves", "acceptance_rate", "translation_scale", "bins_min",
@@ -114,3 +115,95 @@ def test_resume_from_document_is_bit_identical(gpu_ctx_factory):
     CK.resume_set(d, sd, [CK.loads(t) for t in text])
     d.run(10000)
     assert CK.save_set(c, sc) == CK.save_set(d, sd)
+
+
+# RFC 8949 Appendix A: published encodings of data items (preferred
+# serialisation: the shortest float that holds the value exactly).
+RFC8949_VECTORS = [
+    (0, "00"), (1, "01"), (10, "0a"), (23, "17"), (24, "1818"), (25, "1819"), (100, "1864"), (1000, "1903e8"),
+    (1000000, "1a000f4240"), (1000000000000, "1b000000e8d4a51000"), (18446744073709551615, "1bffffffffffffffff"),
+    (-1, "20"), (-10, "29"), (-100, "3863"), (-1000, "3903e7"),
+    (0.0, "f90000"), (-0.0, "f98000"), (1.0, "f93c00"), (1.1, "fb3ff199999999999a"), (1.5, "f93e00"),
+    (65504.0, "f97bff"), (100000.0, "fa47c35000"), (3.4028234663852886e+38, "fa7f7fffff"),
+    (1.0e+300, "fb7e37e43c8800759c"), (5.960464477539063e-8, "f90001"), (0.00006103515625, "f90400"),
+    (-4.0, "f9c400"), (-4.1, "fbc010666666666666"), (float("inf"), "f97c00"), (float("-inf"), "f9fc00"),
+    (False, "f4"), (True, "f5"), (None, "f6"),
+    ("", "60"), ("a", "6161"), ("IETF", "6449455446"), ("\"\\", "62225c"), ("ü", "62c3bc"),
+    ("水", "63e6b0b4"),
+    ([], "80"), ([1, 2, 3], "83010203"), ([1, [2, 3], [4, 5]], "8301820203820405"),
+    (list(range(1, 26)), "98190102030405060708090a0b0c0d0e0f101112131415161718181819"),
+    ({}, "a0"), ({"a": 1, "b": [2, 3]}, "a26161016162820203"), (["a", {"b": "c"}], "826161a161626163"),
+]
+
+
+@pytest.mark.parametrize("value,hexstr", RFC8949_VECTORS, ids=[h for _, h in RFC8949_VECTORS])
+def test_cbor_rfc8949_vectors(value, hexstr):
+    assert CK.cbor_dumps(value).hex() == hexstr
+    back = CK.cbor_loads(bytes.fromhex(hexstr))
+    assert back == value and type(back) is type(value)
+    if isinstance(value, float):
+        assert np.float64(back).tobytes() == np.float64(value).tobytes()
+
+
+def test_cbor_decodes_indefinite_lengths_and_nan():
+    # RFC 8949 Appendix A: indefinite-length array and map
+    assert CK.cbor_loads(bytes.fromhex("9f018202039f0405ffff")) == [1, [2, 3], [4, 5]]
+    assert CK.cbor_loads(bytes.fromhex("bf61610161629f0203ffff")) == {"a": 1, "b": [2, 3]}
+    assert np.isnan(CK.cbor_loads(bytes.fromhex("f97e00")))
+    with pytest.raises(ValueError):
+        CK.cbor_loads(bytes.fromhex("8301"))  # truncated
+
+
+@pytest.mark.parametrize("ext", ["yaml", "json", "cbor"])
+def test_document_files_by_extension(tmp_path, ext):
+    """save/load pick the format from the extension (src/mc/mod.rs:72-79,
+    113-119); every value of a document survives exactly."""
+    p = S.make_params(system=S.WCA, n=32, method=S.SAD, method_param=0.1, move_param=0.05, energy_bin=0.05,
+                      has_max_e=1, max_e=80.0, collect_pressure=1)
+    st, bins, _ = synthetic_walker()
+    st["has_pressure"] = 1
+    rng = np.random.default_rng(9)
+    bins["pressure_count"] = rng.integers(0, 50, 40).astype(np.uint64)
+    bins["pressure_total"] = rng.normal(size=40) * 1e3
+    coords = rng.uniform(0, 3.5, 3 * 32)
+    doc = CK.to_document(p, st, bins, coords=coords, save_as=f"w.{ext}")
+    path = tmp_path / f"w.{ext}"
+    CK.save(doc, path)
+    assert CK.load(path) == doc
+    with pytest.raises(ValueError):
+        CK.save(doc, tmp_path / "w.toml")
+    # the pressure extra reaches the physical rows of the device layout
+    st2, rows, c, sp, pr = CK.from_document(CK.load(path), p, 4000, 0.0, 3.5)
+    sl = slice(int(st2["base"]), int(st2["base"]) + 40)
+    np.testing.assert_array_equal(pr["count"][sl], bins["pressure_count"])
+    np.testing.assert_array_equal(pr["total"][sl], bins["pressure_total"])
+    assert pr["count"].sum() == bins["pressure_count"].sum()
+    assert int(st2["has_pressure"]) == 1
+
+
+@pytest.mark.gpu
+def test_resume_wca_with_pressure_is_bit_identical(gpu_ctx_factory, tmp_path):
+    """A WCA walker set collecting the pressure, checkpointed to CBOR
+    mid-run and resumed on a fresh context, continues bit for bit,
+    pressure extra included."""
+    print("case resume wca pressure from cbor", flush=True)
+    sp = S.make_params(system=S.WCA, n=32, lanes=8, method=S.SAD, method_param=0.1, move_param=0.05,
+                       energy_bin=0.05, has_max_e=1, max_e=80.0, collect_pressure=1)
+    seeds = np.arange(6, dtype=np.uint64) + 7
+    box = np.cbrt(32 / (6.0 * np.array([0.3, 0.35, 0.4, 0.45, 0.5, 0.42]) / np.pi))
+    a = gpu_ctx_factory()
+    sa = a.add_set(sp, seeds, geom=box)
+    a.init()
+    a.run(6000)
+    for w, d in enumerate(CK.save_set(a, sa)):
+        assert d["bins"]["extra"]["pressure"]["count"], "pressure collected"
+        CK.save(d, tmp_path / f"w{w}.cbor")
+    a.run(4000)
+    b = gpu_ctx_factory()
+    sb = b.add_set(sp, seeds, geom=box)
+    b.init()
+    CK.resume_set(b, sb, [CK.load(tmp_path / f"w{w}.cbor") for w in range(len(seeds))])
+    b.run(4000)
+    da, db = CK.save_set(a, sa), CK.save_set(b, sb)
+    for w in range(len(seeds)):
+        assert da[w] == db[w], f"walker {w}"

@@ -80,6 +80,17 @@ def test_lj31_eight_lanes(gpu_ctx_factory):
     _compare(ctx, zid, sp, [9], 3.0, 3000, [], 0.05, 16, 4)
 
 
+def test_lj13_more_slot_lanes_than_a_workgroup(gpu_ctx_factory):
+    """48 slots of 16 lanes (768 lanes' worth) on one wave: its four lane
+    groups take the slots in turn, so only LDS bounds the replica count."""
+    print("case zmc lj13 S=2 G=16 R=48", flush=True)
+    ctx = gpu_ctx_factory()
+    sp = S.make_params(system=S.LJ, n=13, lanes=16, move_param=0.1)
+    seeds = [3, 4]
+    zid = ctx.zmc_add(sp, seeds, geom=2.5, min_T=0.05, isbnb=64, max_replicas=48)
+    _compare(ctx, zid, sp, seeds, 2.5, 12000, [5000], 0.05, 64, 6)
+
+
 def test_wca8_runs_bit_exact(gpu_ctx_factory):
     # WCA's randomization (atoms added back through the error estimate) and
     # the pressure collected above each cutoff every N^2 moves
@@ -104,8 +115,9 @@ def test_zmc_capacity_and_arguments(gpu_ctx_factory):
     ctx = gpu_ctx_factory()
     with pytest.raises(S.SadmcError, match="cutoff wide"):
         ctx.zmc_add(S.make_params(system=S.WCA, n=32), [1], geom=1.0)
-    with pytest.raises(S.SadmcError, match="512"):
-        ctx.zmc_add(S.make_params(system=S.LJ, n=13, lanes=16), [1], geom=2.5, max_replicas=64)
+    # a run's systems must fit the 64 KiB of LDS its wave uses
+    with pytest.raises(S.SadmcError, match="64 KiB"):
+        ctx.zmc_add(S.make_params(system=S.LJ, n=38, lanes=16), [1], geom=3.2, max_replicas=64)
     zid = ctx.zmc_add(S.make_params(system=S.ISING, n=4, ising_init=1), [5], max_replicas=4)
     with pytest.raises(S.SadmcError, match="split beyond 4"):
         ctx.zmc_run(zid, 20000)
