@@ -233,6 +233,16 @@ int sadmc_prop_train(sadmc_ctx *ctx, int32_t steps, int32_t batch, float lr, flo
  * returns SADMC_E_CAPACITY (with *nbins set) if max_bins is too small. */
 int sadmc_mean_entropy(sadmc_ctx *ctx, int32_t set, double e_a, double e_b, int64_t max_bins, double *e_first,
                        int64_t *nbins, double *S, int64_t *count);
+/* The same reduction allocating nothing: no reference counterpart; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Its scratch is a
+ * fixed 32 MiB device array per GPU, held until the process ends, instead of
+ * the context's growable buffer; a set whose reduction needs more (about
+ * 96 bytes per walker plus 16 bytes per grid bin and chunk) is refused with
+ * SADMC_E_CAPACITY.  Same arguments and results as sadmc_mean_entropy.  All
+ * calls on one GPU share the array: calls that overlap on one GPU race. */
+int sadmc_mean_entropy_noalloc(sadmc_ctx *ctx, int32_t set, double e_a, double e_b, int64_t max_bins,
+                               double *e_first, int64_t *nbins, double *S, int64_t *count);
 
 /* ---- EnergyMC over a histogram Binning (src/mc/energy_binning.rs with
  * src/mc/binning/histogram.rs: the `binning` binary) -----------------------

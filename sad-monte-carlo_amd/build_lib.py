@@ -13,7 +13,8 @@ import tempfile
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 UNITS = ["sadmc_host.hip", "k_ising.hip", "k_lj64.hip", "k_lj32.hip", "k_wca.hip", "k_batched.hip",
-         "k_learned64.hip", "k_learned32.hip", "k_train.hip", "k_entropy.hip", "k_pt.hip", "k_binning.hip", "k_zmc.hip"]
+         "k_learned64.hip", "k_learned32.hip", "k_train.hip", "k_entropy.hip", "k_pt.hip", "k_binning.hip", "k_zmc.hip",
+         "sadmc_noalloc.hip"]
 OUT = os.path.join(HERE, "libsadmc_amd.so")
 ARCH = "--offload-arch=gfx950"
 CFLAGS = [ARCH, "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang",

@@ -140,7 +140,7 @@ EXPORTS = [
     "sadmc_run_batched", "sadmc_graph_run", "sadmc_autotune_lanes",
     "sadmc_plan_learned", "sadmc_prop_init", "sadmc_prop_set_weights", "sadmc_prop_get_weights",
     "sadmc_prop_collect", "sadmc_prop_records", "sadmc_prop_load_records", "sadmc_prop_gradient",
-    "sadmc_prop_train", "sadmc_mean_entropy",
+    "sadmc_prop_train", "sadmc_mean_entropy", "sadmc_mean_entropy_noalloc",
     "sadmc_pt_add", "sadmc_pt_run", "sadmc_pt_replicas", "sadmc_pt_coords", "sadmc_pt_spins", "sadmc_pt_ladder_rng",
     "sadmc_bin_add_set", "sadmc_bin_run", "sadmc_bin_state", "sadmc_bin_bins", "sadmc_bin_coords", "sadmc_bin_spins",
     "sadmc_bin_hires",
