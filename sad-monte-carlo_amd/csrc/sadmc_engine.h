@@ -122,7 +122,9 @@ __device__ __forceinline__ uint64_t sample_zone(uint64_t &s0, uint64_t &s1, uint
 
 
 // MV = 1: the learned Gaussian move with its Hastings term (LJ only).
-template <int SYS, int KMAX, int PREC, int METH, int MV = 0>
+// GC > 0: the lanes per walker fixed at compile time (must equal the set's
+// G), which folds the group sums, lane indices and atom-slot arithmetic.
+template <int SYS, int KMAX, int PREC, int METH, int MV = 0, int GC = 0>
 struct Engine {
   const SetArgs *Ap;  // kernel argument segment (uniform; read with scalar loads)
   SADMC_DF const SetArgs &args() const { return *Ap; }
@@ -199,7 +201,7 @@ struct Engine {
   SADMC_DF void load_system(const SetArgs &a, int walker, int lane_in_group, double *lds_group) {
     Ap = &a;
     lane = lane_in_group;
-    G = a.G;
+    G = GC ? GC : a.G;
     K = a.K;
     n = a.n;
     H = a.hot + (int64_t)walker * a.cap;
@@ -987,7 +989,12 @@ struct Engine {
     }
     if (c_hist > highest) {
       highest = c_hist;
+      // the cached ln w at too_lo / too_hi and N_s change here only when
+      // the range grows; otherwise ln w at those bins changes only through
+      // the current bin, which the increment above already tracks
+      bool grew = false;
       if (E > too_hi) {
+        grew = true;
         if (ihi >= len) {
           status = SADMC_WS_INDEX;
           return;
@@ -1026,6 +1033,7 @@ struct Engine {
           return;
         }
       } else if (E < too_lo) {
+        grew = true;
         if (ilo >= len) {
           status = SADMC_WS_INDEX;
           return;
@@ -1073,9 +1081,11 @@ struct Engine {
         too_lo = nl;
         ilo = ilo2;
       }
-      Ns_d = (double)Ns;
-      lnw_lo = lnw_at(base + ilo);
-      lnw_hi = lnw_at(base + ihi);
+      if (grew) {
+        Ns_d = (double)Ns;
+        lnw_lo = lnw_at(base + ilo);
+        lnw_hi = lnw_at(base + ihi);
+      }
     }
     if (tL == moves) {
       if (ilo >= len || ihi >= len) {

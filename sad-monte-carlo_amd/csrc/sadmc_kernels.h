@@ -134,15 +134,16 @@ hipError_t launch_pt(int sys, int kmax, const SetArgs &a, const PtArgs &p, int l
 
 #ifdef SADMC_KERNEL_TU
 
-template <int SYS, int KMAX, int PREC, int METH>
+// GC > 0: the set's lanes per walker as a compile-time constant.
+template <int SYS, int KMAX, int PREC, int METH, int GC = 0>
 __global__ __launch_bounds__(64) void k_run(SetArgs a, uint64_t moves) {
   extern __shared__ double lds[];
-  const int G = a.G;
+  const int G = GC ? GC : a.G;
   const int group = threadIdx.x / G;
   const int lane = threadIdx.x & (G - 1);
   const int walker = blockIdx.x * (64 / G) + group;
   if (walker >= a.W) return;
-  Engine<SYS, KMAX, PREC, METH> e;
+  Engine<SYS, KMAX, PREC, METH, 0, GC> e;
   e.load(a, walker, lane, lds + group * 3 * a.npad);
   e.run(moves);
   e.store(walker);
@@ -227,6 +228,22 @@ static hipError_t launch_t(KernelKind kk, const SetArgs &a, const LaunchArgs &la
   const dim3 grid((a.W + per_block - 1) / per_block), block(64);
   switch (kk) {
     case KK_RUN:
+      // SAD sets of 2-8 atoms per lane at 4, 8 or 16 lanes run a kernel with
+      // the lanes fixed at compile time
+      if constexpr (SYS != SYS_ISING && METH == M_SAD && KMAX >= 2 && KMAX <= 8) {
+        if (a.G == 4) {
+          hipLaunchKernelGGL((k_run<SYS, KMAX, PREC, METH, 4>), grid, block, lds, s, a, la.moves);
+          break;
+        }
+        if (a.G == 8) {
+          hipLaunchKernelGGL((k_run<SYS, KMAX, PREC, METH, 8>), grid, block, lds, s, a, la.moves);
+          break;
+        }
+        if (a.G == 16) {
+          hipLaunchKernelGGL((k_run<SYS, KMAX, PREC, METH, 16>), grid, block, lds, s, a, la.moves);
+          break;
+        }
+      }
       hipLaunchKernelGGL((k_run<SYS, KMAX, PREC, METH>), grid, block, lds, s, a, la.moves);
       break;
     case KK_INIT:
