@@ -156,9 +156,10 @@ def learned_cv(budget, walkers, lanes, chunk, out_path, system="lj38"):
     checkpoint stays within 2% of the reference on the temperature window."""
     import analysis as A
     T = np.linspace(0.05, 0.40, 71)
-    windows = {"T0.05-0.40": T >= 0.05, "T0.10-0.40": T >= 0.10, "T0.15-0.40": T >= 0.15}
+    windows = {"T0.05-0.40": T >= 0.05, "T0.10-0.40": T >= 0.10, "T0.15-0.40": T >= 0.15,
+               "T0.30-0.40": T >= 0.30 - 1e-9}
 
-    n, radius, min_t = {"lj38": (38, 3.2, 0.03), "lj13": (13, 2.5, 0.05)}[system]
+    n, radius, min_t = {"lj38": (38, 3.2, 0.03), "lj13": (13, 2.5, 0.05), "lj31": (31, 3.0, 0.05)}[system]
 
     def params(kind):
         return S.make_params(system=S.LJ, n=n, lanes=lanes, method=S.SAD, method_param=min_t, move_param=0.1,
@@ -239,6 +240,112 @@ def learned_cv(budget, walkers, lanes, chunk, out_path, system="lj38"):
     return res
 
 
+LIT_DIR = os.path.join(ROOT, "tests", "golden", "lj31_cv")
+# the literature files the reference holds; True: total heat capacity per atom
+LIT_SOURCES = [("REM", "LJ31_Cv_Reference", True), ("RESTMC", "LJ31_Cv_Reference_alt", False),
+               ("MCET", "LJ31_Cv_Reference_3", True), ("EMC", "LJ31_Cv_Reference_4", True),
+               ("t-REM", "tRem_Ref", False)]
+
+
+def lj31_literature():
+    """name -> (T, configurational Cv), with plotting/final_heat_capacity.py:33-53's (c - 3/2) * 31."""
+    out = {}
+    for name, f, per_atom in LIT_SOURCES:
+        T, c = np.loadtxt(os.path.join(LIT_DIR, f + ".csv"), delimiter=",", unpack=True)
+        o = np.argsort(T)
+        out[name] = (T[o], ((c - 1.5) * 31 if per_atom else c)[o])
+    return out
+
+
+def combine_entropy(parts, width):
+    """One mean S(E) from several (E, S, count) reductions that share their
+    normalising window: the counts-weighted mean on their common bin grid."""
+    base = min(float(E[0]) for E, _, _ in parts)
+    idx = [np.rint((E - base) / width).astype(np.int64) for E, _, _ in parts]
+    n = int(max(i.max() for i in idx)) + 1
+    tot, cnt = np.zeros(n), np.zeros(n)
+    for i, (_, S_, c) in zip(idx, parts):
+        np.add.at(tot, i, S_ * c)
+        np.add.at(cnt, i, c)
+    keep = cnt > 0
+    return base + np.arange(n)[keep] * width, tot[keep] / cnt[keep]
+
+
+def lj31_cv(total, walkers, lanes, chunk, out_path):
+    """Cv(T) of the LJ31 configuration (sphere 3.0, SAD min_T 0.05, bins 0.01,
+    FP32 pairs with FP64 sums and an FP64 refresh every 1024 moves) from two
+    independent halves of the walkers (seeds 0.. and 1000000..), every `chunk`
+    moves up to `total`: the halves against each other (plain vs plain) and
+    the combined curve against the reference's literature curves."""
+    import analysis as A
+    T = np.linspace(0.05, 0.40, 71)
+    half = walkers // 2
+    sp = lj_params(S.FP32_PAIRS, lanes)
+    ctx = S.Context(0)
+    sa = ctx.add_set(sp, np.arange(half, dtype=np.uint64), geom=RADIUS)
+    sb = ctx.add_set(sp, np.arange(half, dtype=np.uint64) + 1000000, geom=RADIUS)
+    ctx.init()
+    checkpoints = []
+    done, t = 0, 0.0
+    while done < total:
+        k = min(chunk, total - done)
+        ctx.run(k)
+        done += k
+        t += ctx.last_run_ms() / 1e3
+        # one energy window common to both halves, so their S share the constant
+        st = np.concatenate([ctx.download_raw(s, with_bins=False)[0] for s in (sa, sb)])
+        window = (float(st["too_lo"].max()), float(st["too_hi"].min()))
+        curves = []
+        grids = []
+        for s in (sa, sb):
+            E, Sg, cnt = ctx.mean_entropy(s, window)
+            grids.append((E, Sg, cnt))
+            curves.append(A.heat_capacity(T, E, Sg))
+        Ec, Sc = combine_entropy(grids, 0.01)
+        cv = A.heat_capacity(T, Ec, Sc)
+        checkpoints.append(dict(moves=done, gpu_s=t, cv_a=curves[0].tolist(), cv_b=curves[1].tolist(),
+                                cv=cv.tolist()))
+        log(f"[lj31 cv] {done} moves, {t:.1f} s: Cv(0.1) {cv[10]:.1f} Cv(0.2) {cv[30]:.1f} Cv(0.3) {cv[50]:.1f}; "
+            f"halves differ by {np.max(np.abs(curves[0] / curves[1] - 1)):.3f}")
+    ctx.close()
+    final = np.array(checkpoints[-1]["cv"])
+    windows = {"T0.05-0.10": (0.05, 0.10), "T0.10-0.20": (0.10, 0.20), "T0.20-0.30": (0.20, 0.30),
+               "T0.30-0.40": (0.30, 0.40)}
+    lit = lj31_literature()
+    vs_lit = {}
+    for name, (lt, lc) in lit.items():
+        row = {}
+        for w, (a, b) in windows.items():
+            m = (T >= a - 1e-9) & (T <= b + 1e-9) & (T >= lt[0]) & (T <= lt[-1])
+            ref = np.interp(T[m], lt, lc)
+            row[w] = float(np.max(np.abs(final[m] / ref - 1.0)))
+        vs_lit[name] = row
+    spread = {}
+    for w, (a, b) in windows.items():
+        m = (T >= a - 1e-9) & (T <= b + 1e-9)
+        c = np.array([np.interp(T[m], lt, lc) for lt, lc in lit.values()])
+        spread[w] = float(np.max((c.max(0) - c.min(0)) / c.mean(0)))
+    halves = {w: float(np.max(np.abs(np.array(checkpoints[-1]["cv_a"])[(T >= a - 1e-9) & (T <= b + 1e-9)]
+                                     / np.array(checkpoints[-1]["cv_b"])[(T >= a - 1e-9) & (T <= b + 1e-9)] - 1)))
+              for w, (a, b) in windows.items()}
+
+    def time_to_2pct(key, a, b):
+        m = (T >= a - 1e-9) & (T <= b + 1e-9)
+        dev = [float(np.max(np.abs(np.array(c[key])[m] / final[m] - 1))) for c in checkpoints[:-1]]
+        for i in range(len(dev)):
+            if all(d < 0.02 for d in dev[i:]):
+                return checkpoints[i]["gpu_s"]
+        return None
+
+    with open(out_path, "w") as f:
+        json.dump({"T": T.tolist(), "checkpoints": checkpoints,
+                   "literature": {k: [v[0].tolist(), v[1].tolist()] for k, v in lit.items()}}, f)
+    return {"walkers": walkers, "moves_per_walker": total, "gpu_s": checkpoints[-1]["gpu_s"],
+            "max_rel_dev_vs_literature": vs_lit, "literature_spread": spread, "halves_max_rel_dev": halves,
+            "plain_time_to_2pct_s": {w: time_to_2pct("cv", a, b) for w, (a, b) in windows.items()},
+            "Cv_at": {str(x): float(final[int(round((x - 0.05) / 0.005))]) for x in (0.05, 0.1, 0.2, 0.3, 0.35)}}
+
+
 def batched(moves, per_size, launches):
     """The mixed batch: LJ clusters N = 13, 19, 31, 38 in spheres of radius
     2.5, 2.7, 3.0, 3.2 sigma and periodic WCA fluids N = 32, 64, 128, 256
@@ -260,19 +367,25 @@ def batched(moves, per_size, launches):
     for k, (kw, geom) in enumerate(specs):
         sp = S.make_params(method=S.SAD, method_param=0.1, move_param=0.05, refresh=1024, **kw)
         seeds = np.arange(per_size, dtype=np.uint64) + 100000 * k
-        # trial on as many walkers as the whole batch holds, past the start-up phase
+        # start from each class timed alone (on as many walkers as the whole batch holds)
         trial = np.arange(len(specs) * per_size, dtype=np.uint64)
         best, rates = S.autotune_lanes(sp, trial, np.resize(geom, len(trial)), warm_moves=20000, trial_moves=2000)
         sp.lanes = best
-        log(f"[batched] {'LJ' if kw['system'] == S.LJ else 'WCA'}{kw['n']}: lanes {best} "
+        log(f"[batched] {'LJ' if kw['system'] == S.LJ else 'WCA'}{kw['n']}: alone, lanes {best} "
             + " ".join(f"{g}:{r / 1e9:.2f}" for g, r in sorted(rates.items())))
         sets.append((sp, seeds, geom))
+    # then jointly: every class's lanes and the workgroup's wavefronts, the whole batch timed
+    lanes, waves, rate = S.autotune_batch(sets, warm_moves=20000, trial_moves=2000)
+    for (sp, _, _), g in zip(sets, lanes):
+        sp.lanes = g
+    log(f"[batched] joint: lanes {lanes}, {waves} wavefronts per workgroup, {rate / 1e9:.2f} G moves/s")
     out = {}
     for mode in ("per_set", "graph"):
         ctx = S.Context(0)
         for sp, seeds, geom in sets:
             ctx.add_set(sp, seeds, geom=geom)
         ctx.init()
+        ctx.set_block_waves(-1, waves)
         # one untimed launch (captures the graph)
         if mode == "graph":
             ctx.run_batched(moves, graph=True)
@@ -290,6 +403,16 @@ def batched(moves, per_size, launches):
         out[mode] = {"moves_per_s": total / (np.mean(ms) / 1e3), "ms_per_launch": float(np.mean(ms))}
         log(f"[batched] {mode}: {out[mode]['moves_per_s'] / 1e9:.3f} G moves/s")
     out["lanes"] = {f"{'LJ' if sp.system == S.LJ else 'WCA'}{sp.n}": sp.lanes for sp, _, _ in sets}
+    out["waves_per_workgroup"] = waves
+    # algorithmic work of one launch: 30 (N-1) flop per LJ move, 48 (N-1) per WCA move (pair terms of the
+    # old and new position, minimum image for WCA); 80 B of bin traffic per move
+    flop = sum(per_size * moves * (30 if sp.system == S.LJ else 48) * (sp.n - 1) for sp, _, _ in sets)
+    nmov = len(sets) * per_size * moves
+    t = out["graph"]["ms_per_launch"] / 1e3
+    out["roofline"] = {"valu_fp64": {"achieved": flop / t / 1e12, "peak": FP64_VECTOR_PEAK, "unit": "TFLOP/s",
+                                     "frac": flop / t / 1e12 / FP64_VECTOR_PEAK},
+                       "hbm": {"achieved": BYTES_PER_MOVE * nmov / t / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                               "frac": BYTES_PER_MOVE * nmov / t / 1e9 / HBM_PEAK_GBS}}
     return out
 
 
@@ -558,7 +681,7 @@ def main():
     ap.add_argument("--learned-cv", type=float, default=0.0,
                     help="only run the LJ38 wall-time-to-Cv comparison with this GPU-time budget (s) per method")
     ap.add_argument("--learned-cv-out", default="learned_cv.json")
-    ap.add_argument("--learned-cv-system", default="lj38", choices=["lj38", "lj13"])
+    ap.add_argument("--learned-cv-system", default="lj38", choices=["lj38", "lj13", "lj31"])
     ap.add_argument("--batched", type=int, default=0,
                     help="only run the mixed LJ/WCA batch (this many moves per launch)")
     ap.add_argument("--binning", type=int, default=0,
@@ -572,6 +695,10 @@ def main():
                     help="with --zmc: independent runs (2048 = 8 runs per CU, all resident at once)")
     ap.add_argument("--entropy", type=int, default=0,
                     help="only time the walker-set S(E) reduction after this many moves per walker")
+    ap.add_argument("--lj31-cv", type=float, default=0.0,
+                    help="only run the LJ31 Cv(T) comparison: this many moves per walker (two seed halves)")
+    ap.add_argument("--lj31-cv-chunk", type=float, default=1e7, help="with --lj31-cv: moves between checkpoints")
+    ap.add_argument("--lj31-cv-out", default="lj31_cv.json")
     ap.add_argument("--tempering", type=int, default=0,
                     help="only run LJ31 parallel tempering ladders for this many rounds")
     a = ap.parse_args()
@@ -608,8 +735,13 @@ def main():
         print(json.dumps({"metric": "LJ31 parallel-tempering replica moves/s on 1 GPU", "unit": "moves/s",
                           "higher_is_better": True, "value": res["moves_per_s"], **res}))
         return
+    if a.lj31_cv > 0:
+        res = lj31_cv(int(a.lj31_cv), a.walkers, a.lanes, int(a.lj31_cv_chunk), a.lj31_cv_out)
+        print(json.dumps({"metric": "LJ31 Cv(T): plain walkers vs the literature and vs each other", "unit": "",
+                          **res}))
+        return
     if a.learned_cv > 0:
-        lanes = a.lanes if a.learned_cv_system == "lj38" else 4
+        lanes = a.lanes if a.learned_cv_system != "lj13" else 4
         res = learned_cv(a.learned_cv, a.walkers, lanes, 1000000, a.learned_cv_out, a.learned_cv_system)
         print(json.dumps({"metric": f"{a.learned_cv_system.upper()} wall time to Cv(T) within 2% (plain vs learned)",
                           "unit": "s",

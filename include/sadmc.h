@@ -193,6 +193,23 @@ int sadmc_autotune_lanes(int device, const sadmc_set_params *p, int32_t n_walker
                          const double *geom, uint64_t warm_moves, uint64_t trial_moves, int32_t *best_lanes,
                          double *rates);
 
+/* Wavefronts per workgroup (1, 2 or 4) of set `set`'s run launch, or with
+ * set = -1 of the batched launch (whose workgroup then runs that many
+ * consecutive tiles); results do not depend on it.  SADMC_E_ARG beyond 64 KiB
+ * of LDS per workgroup.  No reference counterpart (launch shape). */
+int sadmc_set_block_waves(sadmc_ctx *ctx, int32_t set, int32_t waves);
+/* Joint tuning of a mixed batch (no reference counterpart): the lanes per
+ * walker of each of the nsets sets and the wavefronts per workgroup of the
+ * batched launch, chosen by timing the whole batch (graph replay of
+ * `trial_moves` moves after `warm_moves`): a coordinate descent over each
+ * set's power-of-two lane counts (<= 16 atoms per lane), starting from
+ * ps[k].lanes (0: the default), then 1, 2 or 4 wavefronts per workgroup.
+ * seeds / geom: the sets' walkers concatenated in set order.  rate_out:
+ * the chosen configuration's moves/s. */
+int sadmc_autotune_batch(int device, int32_t nsets, const sadmc_set_params *ps, const int32_t *n_walkers,
+                         const uint64_t *seeds, const double *geom, uint64_t warm_moves, uint64_t trial_moves,
+                         int32_t *lanes_out, int32_t *waves_out, double *rate_out);
+
 /* ---- single-walker pair for parity tests: MovableSystem::plan_move and
  * ConfirmSystem::confirm (system/mod.rs:101-120).  plan_move consumes the
  * walker's RNG exactly as inside move_once; *some = 0 means None. */

@@ -23,17 +23,23 @@ __device__ __forceinline__ void batched_walker(const SetArgs &a, int walker, int
 
 // Register budget of two waves per SIMD: the kernel holds every variant, and
 // without the bound the widest one would cap the whole batch at one wave.
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_batched(
-    const SetArgs *sets, const BatchTile *tiles, uint64_t moves) {
+// A workgroup of blockDim.x / 64 wavefronts takes consecutive tiles, each
+// wave with `wave_lds` doubles of LDS.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_batched(
+    const SetArgs *sets, const BatchTile *tiles, int ntiles, int wave_lds, uint64_t moves) {
   extern __shared__ double lds[];
-  const BatchTile t = tiles[blockIdx.x];
+  const int wave = threadIdx.x / 64;
+  const int ti = blockIdx.x * (blockDim.x / 64) + wave;
+  if (ti >= ntiles) return;
+  const BatchTile t = tiles[ti];
   const SetArgs &a = sets[t.set];
   const int G = a.G;
-  const int group = threadIdx.x / G;
-  const int lane = threadIdx.x & (G - 1);
+  const int wl = threadIdx.x & 63;
+  const int group = wl / G;
+  const int lane = wl & (G - 1);
   const int walker = t.first_walker + group;
   if (walker >= a.W) return;
-  double *mine = lds + group * 3 * a.npad;
+  double *mine = lds + (size_t)wave * wave_lds + group * 3 * a.npad;
   const int kmax = a.K <= 1 ? 1 : a.K <= 2 ? 2 : a.K <= 4 ? 4 : a.K <= 8 ? 8 : 16;
   if (t.sys == SYS_LJ) {
     switch (kmax) {
@@ -54,9 +60,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
   }
 }
 
-hipError_t launch_batched(const SetArgs *d_sets, const BatchTile *d_tiles, int ntiles, size_t lds_bytes,
+hipError_t launch_batched(const SetArgs *d_sets, const BatchTile *d_tiles, int ntiles, size_t wave_lds, int waves,
                           uint64_t moves, hipStream_t s) {
-  hipLaunchKernelGGL(k_batched, dim3(ntiles), dim3(64), lds_bytes, s, d_sets, d_tiles, moves);
+  const int wv = waves > 1 ? waves : 1;
+  hipLaunchKernelGGL(k_batched, dim3((ntiles + wv - 1) / wv), dim3(64 * wv), wave_lds * wv, s, d_sets, d_tiles,
+                     ntiles, (int)(wave_lds / sizeof(double)), moves);
   return hipGetLastError();
 }
 

@@ -69,6 +69,7 @@ struct SetHost {
   Pending *d_pending = nullptr;
   int32_t *d_flag = nullptr;
   double *d_eout = nullptr;  // a recomputed energy (sadmc_compute_energy)
+  int waves = 1;             // wavefronts per workgroup of the run kernel
   SetArgs args;
 };
 
@@ -127,7 +128,8 @@ struct sadmc_ctx {
   SetArgs *d_sets = nullptr;
   BatchTile *d_tiles = nullptr;
   int ntiles = 0;
-  size_t batch_lds = 0;
+  size_t batch_lds = 0;  // LDS per wavefront of the batched launch
+  int batch_waves = 1;   // wavefronts per workgroup of the batched launch
   std::map<uint64_t, hipGraphExec_t> graphs;  // captured batched launch per move count
   std::vector<hipGraph_t> graph_defs;
   // learned proposal: FP32 master weights, Adam moments, BF16 copy the
@@ -670,6 +672,7 @@ int sadmc_run(sadmc_ctx *c, uint64_t moves) {
   for (auto &s : c->sets) {
     LaunchArgs la{};
     la.moves = moves;
+    la.waves = s.waves;
     HIPCHECK(c, launch(s.p.system, s.p.precision, s.p.method, s.KMAX, KK_RUN, s.args, la, c->stream));
   }
   HIPCHECK(c, hipEventRecord(c->ev1, c->stream));
@@ -686,6 +689,7 @@ int sadmc_run_set(sadmc_ctx *c, int32_t set, uint64_t moves) {
   SetHost &s = c->sets[set];
   LaunchArgs la{};
   la.moves = moves;
+  la.waves = s.waves;
   HIPCHECK(c, launch(s.p.system, s.p.precision, s.p.method, s.KMAX, KK_RUN, s.args, la, c->stream));
   HIPCHECK(c, hipEventRecord(c->ev1, c->stream));
   c->timed = true;
@@ -762,7 +766,7 @@ int sadmc_run_batched(sadmc_ctx *c, uint64_t moves) {
   int rc = build_batch(c);
   if (rc) return rc;
   HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
-  HIPCHECK(c, launch_batched(c->d_sets, c->d_tiles, c->ntiles, c->batch_lds, moves, c->stream));
+  HIPCHECK(c, launch_batched(c->d_sets, c->d_tiles, c->ntiles, c->batch_lds, c->batch_waves, moves, c->stream));
   HIPCHECK(c, hipEventRecord(c->ev1, c->stream));
   c->timed = true;
   return SADMC_OK;
@@ -779,7 +783,8 @@ int sadmc_graph_run(sadmc_ctx *c, uint64_t moves) {
     hipGraph_t g;
     hipGraphExec_t ge;
     HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    hipError_t le = launch_batched(c->d_sets, c->d_tiles, c->ntiles, c->batch_lds, moves, c->stream);
+    hipError_t le = launch_batched(c->d_sets, c->d_tiles, c->ntiles, c->batch_lds, c->batch_waves, moves,
+                                   c->stream);
     HIPCHECK(c, hipStreamEndCapture(c->stream, &g));
     HIPCHECK(c, le);
     HIPCHECK(c, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
@@ -835,6 +840,114 @@ int sadmc_autotune_lanes(int device, const sadmc_set_params *p, int32_t n_walker
   }
   if (!bestg) return last ? last : SADMC_E_ARG;
   *best_lanes = bestg;
+  return SADMC_OK;
+}
+
+int sadmc_set_block_waves(sadmc_ctx *c, int32_t set, int32_t waves) {
+  if (!c) return SADMC_E_ARG;
+  if (waves != 1 && waves != 2 && waves != 4) return fail(c, SADMC_E_ARG, "waves per workgroup must be 1, 2 or 4");
+  const size_t lds_max = 65536;
+  if (set == -1) {
+    size_t wave_lds = 0;
+    for (const SetHost &s : c->sets)
+      if (s.p.system != SADMC_ISING)
+        wave_lds = std::max(wave_lds, (size_t)(64 / s.G) * 3 * s.npad * sizeof(double));
+    if (wave_lds * waves > lds_max) return fail(c, SADMC_E_ARG, "the batched workgroup would exceed 64 KiB of LDS");
+    if (waves != c->batch_waves) {
+      // captured graphs hold the old launch shape
+      for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.second);
+      c->graphs.clear();
+      c->batch_waves = waves;
+    }
+    return SADMC_OK;
+  }
+  if (set < 0 || set >= (int32_t)c->sets.size()) return fail(c, SADMC_E_ARG, "bad set");
+  SetHost &s = c->sets[set];
+  const size_t wave_lds = s.p.system == SADMC_ISING ? 0 : (size_t)(64 / s.G) * 3 * s.npad * sizeof(double);
+  if (wave_lds * waves > lds_max) return fail(c, SADMC_E_ARG, "the workgroup would exceed 64 KiB of LDS");
+  s.waves = waves;
+  return SADMC_OK;
+}
+
+// One timed trial of a mixed batch: every set with its lanes, the batched
+// launch with `waves` wavefronts per workgroup, graph-captured; moves/s.
+static int time_batch(int device, int32_t nsets, const sadmc_set_params *ps, const int32_t *nw,
+                      const uint64_t *seeds, const double *geom, const int32_t *lanes, int waves, uint64_t warm,
+                      uint64_t trial, double *rate) {
+  sadmc_ctx *c = nullptr;
+  int rc = sadmc_create(&c, device);
+  if (rc) return rc;
+  int64_t off = 0, total = 0;
+  for (int32_t k = 0; k < nsets && !rc; k++) {
+    sadmc_set_params q = ps[k];
+    q.lanes = lanes[k];
+    rc = sadmc_add_set(c, &q, nw[k], seeds + off, geom + off, nullptr, nullptr, nullptr);
+    off += nw[k];
+    total += nw[k];
+  }
+  if (!rc) rc = sadmc_init(c);
+  if (!rc) rc = sadmc_set_block_waves(c, -1, waves);
+  if (!rc && warm) rc = sadmc_graph_run(c, warm);
+  if (!rc) rc = sadmc_sync(c);
+  if (!rc) rc = sadmc_graph_run(c, trial);  // captures the trial's graph
+  if (!rc) rc = sadmc_sync(c);
+  if (!rc) rc = sadmc_graph_run(c, trial);
+  if (!rc) rc = sadmc_sync(c);
+  double ms = 0.0;
+  if (!rc) rc = sadmc_last_run_ms(c, &ms);
+  sadmc_destroy(c);
+  if (rc) return rc;
+  *rate = (double)total * (double)trial / (ms * 1e-3);
+  return SADMC_OK;
+}
+
+int sadmc_autotune_batch(int device, int32_t nsets, const sadmc_set_params *ps, const int32_t *n_walkers,
+                         const uint64_t *seeds, const double *geom, uint64_t warm_moves, uint64_t trial_moves,
+                         int32_t *lanes_out, int32_t *waves_out, double *rate_out) {
+  if (nsets <= 0 || !ps || !n_walkers || !seeds || !geom || !lanes_out || !waves_out || !trial_moves)
+    return SADMC_E_ARG;
+  std::vector<int32_t> lanes(nsets);
+  std::vector<std::vector<int32_t>> cand(nsets);
+  for (int32_t k = 0; k < nsets; k++) {
+    if (ps[k].system == SADMC_ISING || n_walkers[k] <= 0) return SADMC_E_ARG;
+    for (int G = 2; G <= 64; G <<= 1)
+      if ((ps[k].n + G - 1) / G <= 16) cand[k].push_back(G);
+    lanes[k] = ps[k].lanes > 0 ? ps[k].lanes : auto_lanes(ps[k].n);
+  }
+  int waves = 1;
+  double best = 0.0;
+  int rc = time_batch(device, nsets, ps, n_walkers, seeds, geom, lanes.data(), waves, warm_moves, trial_moves, &best);
+  if (rc) return rc;
+  // coordinate descent over each set's lanes, the whole batch timed each time
+  for (int32_t k = 0; k < nsets; k++) {
+    const int32_t keep = lanes[k];
+    int32_t pick = keep;
+    for (int32_t G : cand[k]) {
+      if (G == keep) continue;
+      lanes[k] = G;
+      double r = 0.0;
+      if (time_batch(device, nsets, ps, n_walkers, seeds, geom, lanes.data(), waves, warm_moves, trial_moves, &r) ==
+              SADMC_OK &&
+          r > best) {
+        best = r;
+        pick = G;
+      }
+    }
+    lanes[k] = pick;
+  }
+  // then the wavefronts per workgroup
+  for (int wv : {2, 4}) {
+    double r = 0.0;
+    if (time_batch(device, nsets, ps, n_walkers, seeds, geom, lanes.data(), wv, warm_moves, trial_moves, &r) ==
+            SADMC_OK &&
+        r > best) {
+      best = r;
+      waves = wv;
+    }
+  }
+  for (int32_t k = 0; k < nsets; k++) lanes_out[k] = lanes[k];
+  *waves_out = waves;
+  if (rate_out) *rate_out = best;
   return SADMC_OK;
 }
 

@@ -23,6 +23,7 @@ struct LaunchArgs {
   int count;      // probe: records per walker
   float *out;     // probe: [W][count][24]
   double *eout;   // energy: the recomputed energy of `walker`
+  int waves;      // run: wavefronts per workgroup (1, 2 or 4)
 };
 
 // One workgroup of the batched launch: walkers [first_walker, first_walker +
@@ -32,7 +33,9 @@ struct BatchTile {
 };
 
 // One entry point per system kind; each lives in its own translation unit.
-hipError_t launch_batched(const SetArgs *d_sets, const BatchTile *d_tiles, int ntiles, size_t lds_bytes,
+// `waves` wavefronts per workgroup, each running its own tile with
+// `wave_lds` bytes of LDS.
+hipError_t launch_batched(const SetArgs *d_sets, const BatchTile *d_tiles, int ntiles, size_t wave_lds, int waves,
                           uint64_t moves, hipStream_t s);
 hipError_t launch_ising(int meth, KernelKind kk, const SetArgs &a, const LaunchArgs &la, hipStream_t s);
 hipError_t launch_lj64(int meth, int kmax, KernelKind kk, const SetArgs &a, const LaunchArgs &la, hipStream_t s);
@@ -134,14 +137,15 @@ hipError_t launch_pt(int sys, int kmax, const SetArgs &a, const PtArgs &p, int l
 
 #ifdef SADMC_KERNEL_TU
 
-// GC > 0: the set's lanes per walker as a compile-time constant.
+// GC > 0: the set's lanes per walker as a compile-time constant.  A
+// workgroup of one to four wavefronts holds blockDim.x / G walkers.
 template <int SYS, int KMAX, int PREC, int METH, int GC = 0>
-__global__ __launch_bounds__(64) void k_run(SetArgs a, uint64_t moves) {
+__global__ __launch_bounds__(256) void k_run(SetArgs a, uint64_t moves) {
   extern __shared__ double lds[];
   const int G = GC ? GC : a.G;
   const int group = threadIdx.x / G;
   const int lane = threadIdx.x & (G - 1);
-  const int walker = blockIdx.x * (64 / G) + group;
+  const int walker = blockIdx.x * (blockDim.x / G) + group;
   if (walker >= a.W) return;
   Engine<SYS, KMAX, PREC, METH, 0, GC> e;
   e.load(a, walker, lane, lds + group * 3 * a.npad);
@@ -226,25 +230,30 @@ static hipError_t launch_t(KernelKind kk, const SetArgs &a, const LaunchArgs &la
   const size_t lds = SYS == SYS_ISING ? 0 : (size_t)per_block * 3 * a.npad * sizeof(double);
   const size_t lds1 = SYS == SYS_ISING ? 0 : (size_t)3 * a.npad * sizeof(double);
   const dim3 grid((a.W + per_block - 1) / per_block), block(64);
+  // the run kernel: la.waves wavefronts per workgroup
+  const int wv = la.waves > 1 ? la.waves : 1;
+  const int run_per_block = wv * per_block;
+  const size_t run_lds = SYS == SYS_ISING ? 0 : (size_t)run_per_block * 3 * a.npad * sizeof(double);
+  const dim3 run_grid((a.W + run_per_block - 1) / run_per_block), run_block(64 * wv);
   switch (kk) {
     case KK_RUN:
       // SAD sets of 2-8 atoms per lane at 4, 8 or 16 lanes run a kernel with
       // the lanes fixed at compile time
       if constexpr (SYS != SYS_ISING && METH == M_SAD && KMAX >= 2 && KMAX <= 8) {
         if (a.G == 4) {
-          hipLaunchKernelGGL((k_run<SYS, KMAX, PREC, METH, 4>), grid, block, lds, s, a, la.moves);
+          hipLaunchKernelGGL((k_run<SYS, KMAX, PREC, METH, 4>), run_grid, run_block, run_lds, s, a, la.moves);
           break;
         }
         if (a.G == 8) {
-          hipLaunchKernelGGL((k_run<SYS, KMAX, PREC, METH, 8>), grid, block, lds, s, a, la.moves);
+          hipLaunchKernelGGL((k_run<SYS, KMAX, PREC, METH, 8>), run_grid, run_block, run_lds, s, a, la.moves);
           break;
         }
         if (a.G == 16) {
-          hipLaunchKernelGGL((k_run<SYS, KMAX, PREC, METH, 16>), grid, block, lds, s, a, la.moves);
+          hipLaunchKernelGGL((k_run<SYS, KMAX, PREC, METH, 16>), run_grid, run_block, run_lds, s, a, la.moves);
           break;
         }
       }
-      hipLaunchKernelGGL((k_run<SYS, KMAX, PREC, METH>), grid, block, lds, s, a, la.moves);
+      hipLaunchKernelGGL((k_run<SYS, KMAX, PREC, METH>), run_grid, run_block, run_lds, s, a, la.moves);
       break;
     case KK_INIT:
       hipLaunchKernelGGL((k_init<SYS, KMAX, PREC, METH>), grid, block, lds, s, a, la.e_lo);

@@ -137,7 +137,8 @@ EXPORTS = [
     "sadmc_compute_energy", "sadmc_verify_energy", "sadmc_system_info", "sadmc_upload_pressure", "sadmc_plan_move", "sadmc_confirm", "sadmc_walker_count",
     "sadmc_get_walker_state", "sadmc_walker_bins", "sadmc_walker_coords", "sadmc_walker_spins",
     "sadmc_download_raw", "sadmc_upload_raw", "sadmc_init_coords", "sadmc_init_spins", "sadmc_bin_window",
-    "sadmc_run_batched", "sadmc_graph_run", "sadmc_autotune_lanes",
+    "sadmc_run_batched", "sadmc_graph_run", "sadmc_autotune_lanes", "sadmc_set_block_waves",
+    "sadmc_autotune_batch",
     "sadmc_plan_learned", "sadmc_prop_init", "sadmc_prop_set_weights", "sadmc_prop_get_weights",
     "sadmc_prop_collect", "sadmc_prop_records", "sadmc_prop_load_records", "sadmc_prop_gradient",
     "sadmc_prop_train", "sadmc_mean_entropy", "sadmc_mean_entropy_noalloc",
@@ -190,6 +191,8 @@ def lib():
     L.sadmc_graph_run.argtypes = [vp, u64]
     L.sadmc_autotune_lanes.argtypes = [C.c_int, C.POINTER(SetParams), i32, vp, vp, u64, u64, C.POINTER(i32),
                                        C.POINTER(d)]
+    L.sadmc_set_block_waves.argtypes = [vp, i32, i32]
+    L.sadmc_autotune_batch.argtypes = [C.c_int, i32, vp, vp, vp, vp, u64, u64, vp, C.POINTER(i32), C.POINTER(d)]
     L.sadmc_plan_learned.argtypes = [vp, i32, i32, C.POINTER(d), C.POINTER(d), vp, C.POINTER(i32)]
     L.sadmc_prop_init.argtypes = [vp, u64]
     L.sadmc_prop_set_weights.argtypes = [vp, vp, i32]
@@ -340,6 +343,11 @@ class Context:
         """The WCA pressure extra of every walker, [W][cap] physical bins."""
         self._check(lib().sadmc_upload_pressure(self.h, s, _p(np.ascontiguousarray(count, np.uint64)),
                                                 _p(np.ascontiguousarray(total, np.float64))), "upload_pressure")
+
+    def set_block_waves(self, s, waves):
+        """Wavefronts per workgroup (1, 2, 4) of set s's run launch; s = -1:
+        of the batched launch."""
+        self._check(lib().sadmc_set_block_waves(self.h, s, waves), "set_block_waves")
 
     def run_batched(self, moves, sync=True, graph=False):
         fn = lib().sadmc_graph_run if graph else lib().sadmc_run_batched
@@ -618,6 +626,24 @@ class Context:
                                            _p(None if coords is None else np.ascontiguousarray(coords, np.float64)),
                                            _p(None if spins is None else np.ascontiguousarray(spins, np.int8))),
                     "upload_raw")
+
+
+def autotune_batch(sets, warm_moves=20000, trial_moves=2000, device=0):
+    """Joint tuning of a mixed batch: `sets` is a list of (params, seeds,
+    geom); returns (lanes per set, wavefronts per workgroup, moves/s)."""
+    n = len(sets)
+    ps = (SetParams * n)(*[p for p, _, _ in sets])
+    nw = np.array([len(sd) for _, sd, _ in sets], np.int32)
+    seeds = np.ascontiguousarray(np.concatenate([np.asarray(sd, np.uint64) for _, sd, _ in sets]))
+    geom = np.ascontiguousarray(np.concatenate([np.broadcast_to(np.asarray(g, np.float64), (len(sd),))
+                                                for _, sd, g in sets]))
+    lanes = np.zeros(n, np.int32)
+    waves, rate = C.c_int32(), C.c_double()
+    rc = lib().sadmc_autotune_batch(device, n, C.cast(ps, C.c_void_p), _p(nw), _p(seeds), _p(geom), warm_moves,
+                                    trial_moves, _p(lanes), C.byref(waves), C.byref(rate))
+    if rc:
+        raise SadmcError(rc, "autotune_batch")
+    return [int(x) for x in lanes], waves.value, rate.value
 
 
 def system_info(params, geom=0.0):

@@ -170,3 +170,38 @@ def test_bench_composition_batched_graph_per_set_and_oracle(gpu_ctx_factory):
         label = f"{'LJ' if sp.system == S.LJ else 'WCA'}{sp.n} G{sp.lanes} walker {w}"
         compare_walker(b.state(sid, w), b.bins(sid, w), orc, sp.method, label=label)
         np.testing.assert_array_equal(b.coords(sid, w), orc.coords(), err_msg=label)
+
+
+def test_workgroup_waves_give_the_same_walkers(gpu_ctx_factory):
+    """Two and four wavefronts per workgroup, in the per-set and the batched
+    launches, leave every walker as one wavefront per workgroup does."""
+    print("case workgroup waves", flush=True)
+    sets = _bench_sets()
+    moves = 2000
+    a, ia = _build(gpu_ctx_factory, sets)
+    a.run(moves)
+    want = _dump(a, ia, sets)
+    for waves in (2, 4):
+        b, ib = _build(gpu_ctx_factory, sets)
+        for sid in ib:
+            b.set_block_waves(sid, waves)
+        b.run(moves)
+        c, ic = _build(gpu_ctx_factory, sets)
+        c.set_block_waves(-1, waves)
+        c.run_batched(moves // 2, graph=True)
+        c.run_batched(moves // 2, graph=True)
+        for k, (w, gb, gc) in enumerate(zip(want, _dump(b, ib, sets), _dump(c, ic, sets))):
+            for name, x, y, z in zip(("states", "bins", "coords"), w, gb, gc):
+                assert x.tobytes() == y.tobytes(), f"set {k} {name}: per-set launch at {waves} waves differs"
+                assert x.tobytes() == z.tobytes(), f"set {k} {name}: batched launch at {waves} waves differs"
+    with pytest.raises(S.SadmcError):
+        a.set_block_waves(-1, 3)
+
+
+def test_joint_autotune_returns_a_timed_configuration():
+    print("case joint autotune", flush=True)
+    sets = [(sp, seeds, geom) for sp, seeds, geom in _bench_sets()[::3]]
+    lanes, waves, rate = S.autotune_batch(sets, warm_moves=500, trial_moves=300)
+    assert waves in (1, 2, 4) and rate > 0
+    for (sp, _, _), g in zip(sets, lanes):
+        assert g & (g - 1) == 0 and 2 <= g <= 64 and -(-sp.n // g) <= 16
