@@ -43,6 +43,7 @@ FP64_VECTOR_PEAK = 78.6
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_MOVE = 80                 # lnw at two bins, read-modify-write of hist, lnw, e_tot, e_sq
 PROFILE = os.path.join(ROOT, "profiles", "k_run_lj31_fp32_sad_16384w.md")
+LEARNED_CV_PROFILE = os.path.join(ROOT, "profiles", "learned_cv_lj31_16384w.md")
 WARMUP_MOVES = 2000                 # moves per warmup launch
 
 
@@ -75,6 +76,24 @@ def measured_traffic(a):
         for line in open(PROFILE):
             if line.startswith("traffic_bytes_per_launch:"):
                 return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
+def recorded_learned_cv():
+    """The second half of the metric, wall time to Cv(T) within 2 % for plain
+    and learned walkers: a run of 15+ minutes of GPU time (`--lj31-cv`, then
+    `--learned-cv ... --learned-cv-ref`), so the default line carries the
+    result recorded in profiles/ with its source instead of re-measuring it;
+    None if the summary is not there."""
+    try:
+        for line in open(LEARNED_CV_PROFILE):
+            if line.startswith("learned_cv:"):
+                res = json.loads(line.split(":", 1)[1])
+                res["source"] = os.path.relpath(LEARNED_CV_PROFILE, os.path.dirname(os.path.abspath(__file__)))
+                res["measured_in_this_run"] = False
+                return res
     except OSError:
         pass
     return None
@@ -143,7 +162,7 @@ def cpu_baseline(precision, moves, rounds):
                        f"reference's summation order, -O3 -march=native, {wall:.1f} s on {cpu}")
 
 
-def learned_cv(budget, walkers, lanes, chunk, out_path, system="lj38"):
+def learned_cv(budget, walkers, lanes, chunk, out_path, system="lj38", ref_path=None):
     """Wall time to Cv(T) within 2% for LJ38 (sphere 3.2 sigma, SAD min_T =
     0.03, 16384 walkers; or LJ13, sphere 2.5, min_T = 0.05): plain cube-move
     walkers against learned-proposal walkers.  The reference curve is the plain walkers' Cv after `budget`
@@ -160,10 +179,12 @@ def learned_cv(budget, walkers, lanes, chunk, out_path, system="lj38"):
                "T0.30-0.40": T >= 0.30 - 1e-9}
 
     n, radius, min_t = {"lj38": (38, 3.2, 0.03), "lj13": (13, 2.5, 0.05), "lj31": (31, 3.0, 0.05)}[system]
+    # LJ31: the precision configuration (FP32 pairs, FP64 refresh every 1024 moves), as --lj31-cv runs it
+    prec = dict(precision=S.FP32_PAIRS, refresh=1024) if system == "lj31" else {}
 
     def params(kind):
         return S.make_params(system=S.LJ, n=n, lanes=lanes, method=S.SAD, method_param=min_t, move_param=0.1,
-                             energy_bin=0.01, has_max_e=1, max_e=0.0, move_kind=kind)
+                             energy_bin=0.01, has_max_e=1, max_e=0.0, move_kind=kind, **prec)
 
     def checkpoint(ctx, sid, t, tag, curves):
         try:
@@ -176,16 +197,23 @@ def learned_cv(budget, walkers, lanes, chunk, out_path, system="lj38"):
         log(f"[{tag}] t={t:7.1f} s  Cv(0.1)={cv[10]:8.2f} Cv(0.2)={cv[30]:8.2f} Cv(0.3)={cv[50]:8.2f}")
 
     seeds = np.arange(walkers, dtype=np.uint64)
-    # plain walkers
-    ctx = S.Context(0)
-    sid = ctx.add_set(params(S.MOVE_CUBE), seeds, geom=radius)
-    ctx.init()
-    plain, t = [], 0.0
-    while t < budget:
-        ctx.run(chunk)
-        t += ctx.last_run_ms() / 1e3
-        checkpoint(ctx, sid, t, "plain", plain)
-    ctx.close()
+    if ref_path:
+        # the plain walkers' curves from a --lj31-cv run: its last checkpoint is the reference
+        with open(ref_path) as f:
+            r = json.load(f)
+        assert np.allclose(r["T"], T)
+        plain = [(c["gpu_s"], np.array(c["cv"])) for c in r["checkpoints"]]
+    else:
+        # plain walkers
+        ctx = S.Context(0)
+        sid = ctx.add_set(params(S.MOVE_CUBE), seeds, geom=radius)
+        ctx.init()
+        plain, t = [], 0.0
+        while t < budget:
+            ctx.run(chunk)
+            t += ctx.last_run_ms() / 1e3
+            checkpoint(ctx, sid, t, "plain", plain)
+        ctx.close()
     ref = plain[-1][1]
     # learned walkers: cube moves while records are collected, training, learned moves
     ctx = S.Context(0)
@@ -682,6 +710,9 @@ def main():
                     help="only run the LJ38 wall-time-to-Cv comparison with this GPU-time budget (s) per method")
     ap.add_argument("--learned-cv-out", default="learned_cv.json")
     ap.add_argument("--learned-cv-system", default="lj38", choices=["lj38", "lj13", "lj31"])
+    ap.add_argument("--learned-cv-ref", default=None,
+                    help="with --learned-cv: the plain curves from a --lj31-cv output instead of a plain run")
+    ap.add_argument("--learned-cv-chunk", type=float, default=1e6, help="with --learned-cv: moves per checkpoint")
     ap.add_argument("--batched", type=int, default=0,
                     help="only run the mixed LJ/WCA batch (this many moves per launch)")
     ap.add_argument("--binning", type=int, default=0,
@@ -742,7 +773,8 @@ def main():
         return
     if a.learned_cv > 0:
         lanes = a.lanes if a.learned_cv_system != "lj13" else 4
-        res = learned_cv(a.learned_cv, a.walkers, lanes, 1000000, a.learned_cv_out, a.learned_cv_system)
+        res = learned_cv(a.learned_cv, a.walkers, lanes, int(a.learned_cv_chunk), a.learned_cv_out,
+                         a.learned_cv_system, a.learned_cv_ref)
         print(json.dumps({"metric": f"{a.learned_cv_system.upper()} wall time to Cv(T) within 2% (plain vs learned)",
                           "unit": "s",
                           "higher_is_better": False, "budget_s": a.learned_cv, "walkers": a.walkers,
@@ -791,6 +823,7 @@ def main():
                               "frac": achieved / FP32_VECTOR_PEAK}},
         "energy_error": {"max_rel": main_run["max_rel_energy_error"], "mean_rel": main_run["mean_rel_energy_error"]},
         "variants": variants,
+        "learned_cv": recorded_learned_cv(),
     }
     if not a.no_cpu:
         log("cpu baseline: C oracle, one walker per thread")
