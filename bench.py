@@ -374,7 +374,13 @@ def lj31_cv(total, walkers, lanes, chunk, out_path):
             "Cv_at": {str(x): float(final[int(round((x - 0.05) / 0.005))]) for x in (0.05, 0.1, 0.2, 0.3, 0.35)}}
 
 
-def batched(moves, per_size, launches):
+# The joint autotuner's choice for the mixed batch below (lanes per walker by
+# size class, wavefronts per workgroup), as `--batched` measures it; the
+# default run times the batch at this configuration instead of re-tuning.
+BATCH_TUNED = ({"LJ13": 8, "LJ19": 4, "LJ31": 4, "LJ38": 8, "WCA32": 4, "WCA64": 8, "WCA128": 16, "WCA256": 32}, 1)
+
+
+def batched(moves, per_size, launches, tuned=None):
     """The mixed batch: LJ clusters N = 13, 19, 31, 38 in spheres of radius
     2.5, 2.7, 3.0, 3.2 sigma and periodic WCA fluids N = 32, 64, 128, 256
     with the packing fraction drawn per walker from U[0.3, 0.5]; `per_size`
@@ -382,7 +388,8 @@ def batched(moves, per_size, launches):
     energy refresh every 1024 moves.  Lanes per walker come from the
     autotuner (timed on as many walkers as the batch holds); the whole batch
     runs as one persistent launch captured in a HIP graph, timed against the
-    per-set launches of the same walkers."""
+    per-set launches of the same walkers.  tuned = (lanes by class name,
+    wavefronts per workgroup) skips the autotuner."""
     rng = np.random.default_rng(7)
     specs = []
     for n, R in ((13, 2.5), (19, 2.7), (31, 3.0), (38, 3.2)):
@@ -395,6 +402,10 @@ def batched(moves, per_size, launches):
     for k, (kw, geom) in enumerate(specs):
         sp = S.make_params(method=S.SAD, method_param=0.1, move_param=0.05, refresh=1024, **kw)
         seeds = np.arange(per_size, dtype=np.uint64) + 100000 * k
+        if tuned:
+            sp.lanes = tuned[0][f"{'LJ' if kw['system'] == S.LJ else 'WCA'}{kw['n']}"]
+            sets.append((sp, seeds, geom))
+            continue
         # start from each class timed alone (on as many walkers as the whole batch holds)
         trial = np.arange(len(specs) * per_size, dtype=np.uint64)
         best, rates = S.autotune_lanes(sp, trial, np.resize(geom, len(trial)), warm_moves=20000, trial_moves=2000)
@@ -402,11 +413,14 @@ def batched(moves, per_size, launches):
         log(f"[batched] {'LJ' if kw['system'] == S.LJ else 'WCA'}{kw['n']}: alone, lanes {best} "
             + " ".join(f"{g}:{r / 1e9:.2f}" for g, r in sorted(rates.items())))
         sets.append((sp, seeds, geom))
-    # then jointly: every class's lanes and the workgroup's wavefronts, the whole batch timed
-    lanes, waves, rate = S.autotune_batch(sets, warm_moves=20000, trial_moves=2000)
-    for (sp, _, _), g in zip(sets, lanes):
-        sp.lanes = g
-    log(f"[batched] joint: lanes {lanes}, {waves} wavefronts per workgroup, {rate / 1e9:.2f} G moves/s")
+    if tuned:
+        waves = tuned[1]
+    else:
+        # then jointly: every class's lanes and the workgroup's wavefronts, the whole batch timed
+        lanes, waves, rate = S.autotune_batch(sets, warm_moves=20000, trial_moves=2000)
+        for (sp, _, _), g in zip(sets, lanes):
+            sp.lanes = g
+        log(f"[batched] joint: lanes {lanes}, {waves} wavefronts per workgroup, {rate / 1e9:.2f} G moves/s")
     out = {}
     for mode in ("per_set", "graph"):
         ctx = S.Context(0)
@@ -682,6 +696,39 @@ def entropy(moves, walkers, lanes, repeats):
     return out
 
 
+def other_paths(walkers, lanes):
+    """Short runs of the engine's other kernels on the same box as the
+    headline line, each on its own workload (the `--batched`, `--binning`,
+    `--zmc`, `--tempering` and `--entropy` modes at reduced lengths, GPU
+    only): moves/s (or seconds) and the fraction of the roofline that bounds
+    each kernel."""
+    def summary(res, value_key="moves_per_s", unit="moves/s"):
+        r = res["roofline"]
+        return {"value": res[value_key], "unit": unit, "roofline_bound": r["bound"], "roofline_frac": r["frac"]}
+
+    out = {}
+    log("[paths] mixed LJ/WCA batch, graph-captured, at the recorded autotuned configuration")
+    b = batched(20000, 4096, 3, tuned=BATCH_TUNED)
+    out["k_batched"] = {"workload": "8 size classes x 4096 walkers (LJ13/19/31/38, WCA32/64/128/256), 20000 "
+                                    "moves per launch, 3 launches",
+                        "value": b["graph"]["moves_per_s"], "unit": "moves/s",
+                        "per_set_launches_moves_per_s": b["per_set"]["moves_per_s"],
+                        "roofline_bound": "valu-fp64", "roofline_frac": b["roofline"]["valu_fp64"]["frac"]}
+    log("[paths] binning-path walkers")
+    r = binning_path(50000, walkers, lanes, 0.0)
+    out["k_bin"] = {"workload": f"LJ31 histogram binning, {walkers} walkers, moves 2000-52000", **summary(r)}
+    log("[paths] energy-ceiling replicas")
+    r = zmc(1000, 2048, lanes, 0.0)
+    out["k_zmc"] = {"workload": "LJ31, 2048 runs of up to 16 replicas, rounds 200-1200", **summary(r)}
+    log("[paths] parallel tempering")
+    r = tempering(4000, 1024, 16, lanes, 0.0)
+    out["k_pt"] = {"workload": "LJ31, 1024 ladders x 16 replicas, 4000 rounds", **summary(r)}
+    log("[paths] walker-set entropy reduction")
+    r = entropy(100000, walkers, lanes, 5)
+    out["k_entropy"] = {"workload": f"LJ31, {walkers} walkers after 1e5 moves", **summary(r, "device_s", "s")}
+    return out
+
+
 def copy_probe(nbytes):
     """A plain device-to-device copy of known size, for scaling the HBM byte
     counters (FETCH_SIZE, WRITE_SIZE) of a profiler run."""
@@ -705,6 +752,7 @@ def main():
     ap.add_argument("--cpu-rounds", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-paths", action="store_true", help="skip the short runs of the other kernels")
     ap.add_argument("--copy-probe", type=int, default=0, help="only run a device copy of this many bytes")
     ap.add_argument("--learned-cv", type=float, default=0.0,
                     help="only run the LJ38 wall-time-to-Cv comparison with this GPU-time budget (s) per method")
@@ -825,6 +873,8 @@ def main():
         "variants": variants,
         "learned_cv": recorded_learned_cv(),
     }
+    if not a.no_paths:
+        out["paths"] = other_paths(a.walkers, a.lanes)
     if not a.no_cpu:
         log("cpu baseline: C oracle, one walker per thread")
         out["cpu_baseline"] = cpu_baseline(1, run_moves, a.cpu_rounds)
