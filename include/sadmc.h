@@ -165,7 +165,10 @@ int sadmc_add_set(sadmc_ctx *ctx, const sadmc_set_params *p, int32_t n_walkers, 
 int sadmc_init(sadmc_ctx *ctx);
 
 /* ---- the hot path: K x move_once for every walker (energy.rs:904-974) - */
-/* Enqueue one launch advancing every walker of every set by `moves`. */
+/* Enqueue one launch per set advancing every walker of every set by
+ * `moves`.  With several sets the launches run side by side on up to four
+ * streams forked from and joined back to the context's stream, so work
+ * enqueued before or after the call stays ordered with all of them. */
 int sadmc_run(sadmc_ctx *ctx, uint64_t moves);
 /* Enqueue one launch advancing only the walkers of set `set` by `moves`
  * (move_once x moves for each of its walkers, energy.rs:904-974): the drain

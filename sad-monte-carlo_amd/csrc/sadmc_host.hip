@@ -116,6 +116,8 @@ struct BinSetHost {
 
 }  // namespace
 
+constexpr int kSideStreams = 4;
+
 struct sadmc_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -124,6 +126,11 @@ struct sadmc_ctx {
   bool inited = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool timed = false;
+  // sadmc_run with several sets: their launches fork from `stream` onto up
+  // to kSideStreams streams (default priority) and join back, so sets that
+  // each fill only part of the GPU run side by side
+  hipStream_t side[kSideStreams] = {};
+  hipEvent_t ev_fork = nullptr, ev_join[kSideStreams] = {};
   // batched launch: per-set arguments and the block -> tile table on the device
   SetArgs *d_sets = nullptr;
   BatchTile *d_tiles = nullptr;
@@ -408,6 +415,12 @@ void sadmc_destroy(sadmc_ctx *c) {
     for (void *p : {(void *)t.d_rep, (void *)t.d_slot_e, (void *)t.d_pos, (void *)t.d_geom, (void *)t.d_slot_of,
                     (void *)t.d_spins, (void *)t.d_lrng})
       (void)hipFree(p);
+  for (int k = 0; k < kSideStreams; k++) {
+    if (c->side[k]) (void)hipStreamSynchronize(c->side[k]);
+    if (c->side[k]) (void)hipStreamDestroy(c->side[k]);
+    if (c->ev_join[k]) (void)hipEventDestroy(c->ev_join[k]);
+  }
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -664,16 +677,46 @@ int sadmc_init(sadmc_ctx *c) {
   return check_flags(c);
 }
 
+// The side streams and their fork/join events, created on first use (a
+// call after a failed creation creates only what is missing).
+static int ensure_side_streams(sadmc_ctx *c) {
+  if (c->ev_fork) return SADMC_OK;
+  for (int k = 0; k < kSideStreams; k++) {
+    if (!c->side[k]) HIPCHECK(c, hipStreamCreate(&c->side[k]));
+    if (!c->ev_join[k]) HIPCHECK(c, hipEventCreateWithFlags(&c->ev_join[k], hipEventDisableTiming));
+  }
+  HIPCHECK(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+  return SADMC_OK;
+}
+
 int sadmc_run(sadmc_ctx *c, uint64_t moves) {
   if (!c) return SADMC_E_ARG;
   if (!c->inited) return fail(c, SADMC_E_STATE, "sadmc_init first");
   HIPCHECK(c, hipSetDevice(c->device));
   HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
-  for (auto &s : c->sets) {
+  const int nsets = (int)c->sets.size();
+  const int ns = nsets < kSideStreams ? nsets : kSideStreams;
+  if (ns > 1) {
+    // sets are independent: fork after everything already on the stream,
+    // set i on side stream i mod ns, and join before the closing event
+    int rc = ensure_side_streams(c);
+    if (rc) return rc;
+    HIPCHECK(c, hipEventRecord(c->ev_fork, c->stream));
+    for (int k = 0; k < ns; k++) HIPCHECK(c, hipStreamWaitEvent(c->side[k], c->ev_fork, 0));
+  }
+  for (int i = 0; i < nsets; i++) {
+    SetHost &s = c->sets[i];
     LaunchArgs la{};
     la.moves = moves;
     la.waves = s.waves;
-    HIPCHECK(c, launch(s.p.system, s.p.precision, s.p.method, s.KMAX, KK_RUN, s.args, la, c->stream));
+    hipStream_t st = ns > 1 ? c->side[i % ns] : c->stream;
+    HIPCHECK(c, launch(s.p.system, s.p.precision, s.p.method, s.KMAX, KK_RUN, s.args, la, st));
+  }
+  if (ns > 1) {
+    for (int k = 0; k < ns; k++) {
+      HIPCHECK(c, hipEventRecord(c->ev_join[k], c->side[k]));
+      HIPCHECK(c, hipStreamWaitEvent(c->stream, c->ev_join[k], 0));
+    }
   }
   HIPCHECK(c, hipEventRecord(c->ev1, c->stream));
   c->timed = true;
