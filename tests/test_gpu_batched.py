@@ -116,7 +116,7 @@ def test_autotune_picks_a_measured_lane_count():
 # drawn per walker, SAD min_T 0.1, 0.05 sigma moves, FP64 refresh every 1024
 # moves.  Two full workgroups of walkers per set plus a ragged third.
 BENCH_CLASSES = [  # (system, atoms, sphere radius (LJ), lanes)
-    (S.LJ, 13, 2.5, 4), (S.LJ, 19, 2.7, 4), (S.LJ, 31, 3.0, 4), (S.LJ, 38, 3.2, 8),
+    (S.LJ, 13, 2.5, 8), (S.LJ, 19, 2.7, 4), (S.LJ, 31, 3.0, 4), (S.LJ, 38, 3.2, 8),
     (S.WCA, 32, 0.0, 4), (S.WCA, 64, 0.0, 8), (S.WCA, 128, 0.0, 16), (S.WCA, 256, 0.0, 32),
 ]
 

@@ -79,3 +79,34 @@ def test_lj13_fp64_configuration_1e7_moves(gpu_ctx_factory):
                        energy_bin=0.01, has_max_e=1, max_e=0.0)
     W = 4096
     _run_long(gpu_ctx_factory, sp, W, 2.5, np.linspace(0, W - 1, 8).astype(int), "lj13 fp64", launches=10)
+
+
+def test_batched_bench_composition_1e6_moves(gpu_ctx_factory):
+    """The mixed batch's eight size classes (LJ13/19/31/38, WCA32/64/128/256
+    at the autotuned lanes, FP64 refresh every 1024 moves) as one
+    graph-captured launch, replayed for 1e6 moves: about a thousand refreshes
+    and SAD past its start-up; the first and last walker of every class
+    against the oracle, bit for bit."""
+    from conftest import run_oracles
+    from test_gpu_batched import _bench_sets
+
+    print("case long batched composition", flush=True)
+    moves, launches = 1_000_000, 4
+    sets = _bench_sets()
+    ctx = gpu_ctx_factory()
+    ids = [ctx.add_set(sp, seeds, geom=geom) for sp, seeds, geom in sets]
+    ctx.init()
+    t0 = time.perf_counter()
+    for k in range(launches):
+        ctx.run_batched(moves // launches, graph=True)
+        ctx.sync()
+        print(f"batched: replay {k + 1}/{launches} done at {time.perf_counter() - t0:.1f} s", flush=True)
+    picks = [(sid, sp, seeds, geom, w) for sid, (sp, seeds, geom) in zip(ids, sets) for w in (0, len(seeds) - 1)]
+    orcs = run_oracles([oracle_params(sp, seeds[w], float(geom[w])) for _, sp, seeds, geom, w in picks], moves)
+    print(f"batched: oracle walkers done at {time.perf_counter() - t0:.1f} s", flush=True)
+    for (sid, sp, seeds, geom, w), orc in zip(picks, orcs):
+        label = f"{'LJ' if sp.system == S.LJ else 'WCA'}{sp.n} G{sp.lanes} walker {w}"
+        st = ctx.state(sid, w)
+        assert st["status"] == 0 and st["moves"] == moves, label
+        compare_walker(st, ctx.bins(sid, w), orc, sp.method, label=label)
+        np.testing.assert_array_equal(ctx.coords(sid, w), orc.coords(), err_msg=label)
