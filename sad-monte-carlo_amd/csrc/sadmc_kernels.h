@@ -299,19 +299,21 @@ static hipError_t launch_km(int meth, int kmax, KernelKind kk, const SetArgs &a,
 // ---------------------------------------------------------- learned move
 // LDS: the positions of the 64/G walkers, then the network weights as float,
 // then 32 floats per walker for the hidden-layer exchange.
-// Two waves per SIMD: the register budget the 16384-walker sets need.
-template <int KMAX, int PREC, int METH>
+// Two waves per SIMD: the register budget the 16384-walker sets need.  GC
+// as in k_run (the SAD kernels of 2 to 8 atoms per lane at 4, 8 or 16 lanes
+// have it fixed at compile time: fewer scalar spills, +10 % for LJ38 at 8).
+template <int KMAX, int PREC, int METH, int GC = 0>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_run_learned(SetArgs a,
                                                                                              uint64_t moves) {
   extern __shared__ double lds[];
-  const int G = a.G, per = 64 / G;
+  const int G = GC ? GC : a.G, per = 64 / G;
   float *lw = (float *)(lds + per * 3 * a.npad);
   load_weights(a.mlp, lw);
   const int group = threadIdx.x / G;
   const int lane = threadIdx.x & (G - 1);
   const int walker = blockIdx.x * per + group;
   if (walker >= a.W) return;
-  Engine<SYS_LJ, KMAX, PREC, METH, 1> e;
+  Engine<SYS_LJ, KMAX, PREC, METH, 1, GC> e;
   e.PW = lw;
   e.XC = lw + kPropLds + group * kPropH;
   e.load(a, walker, lane, lds + group * 3 * a.npad);
@@ -369,6 +371,14 @@ static hipError_t launch_learned_t(KernelKind kk, const SetArgs &a, const Launch
   const dim3 grid((a.W + per_block - 1) / per_block), block(64);
   switch (kk) {
     case KK_RUN:
+      if constexpr (METH == M_SAD && KMAX >= 2 && KMAX <= 8) {
+        if (G == 4 || G == 8 || G == 16) {
+          if (G == 4) hipLaunchKernelGGL((k_run_learned<KMAX, PREC, METH, 4>), grid, block, lds, s, a, la.moves);
+          if (G == 8) hipLaunchKernelGGL((k_run_learned<KMAX, PREC, METH, 8>), grid, block, lds, s, a, la.moves);
+          if (G == 16) hipLaunchKernelGGL((k_run_learned<KMAX, PREC, METH, 16>), grid, block, lds, s, a, la.moves);
+          break;
+        }
+      }
       hipLaunchKernelGGL((k_run_learned<KMAX, PREC, METH>), grid, block, lds, s, a, la.moves);
       break;
     case KK_PLAN:
