@@ -718,8 +718,8 @@ def other_paths(walkers, lanes):
     r = binning_path(50000, walkers, lanes, 0.0)
     out["k_bin"] = {"workload": f"LJ31 histogram binning, {walkers} walkers, moves 2000-52000", **summary(r)}
     log("[paths] energy-ceiling replicas")
-    r = zmc(1000, 2048, lanes, 0.0)
-    out["k_zmc"] = {"workload": "LJ31, 2048 runs of up to 16 replicas, rounds 200-1200", **summary(r)}
+    r = zmc(3000, 2048, lanes, 0.0)
+    out["k_zmc"] = {"workload": "LJ31, 2048 runs of up to 16 replicas, rounds 200-3200", **summary(r)}
     log("[paths] parallel tempering")
     r = tempering(4000, 1024, 16, lanes, 0.0)
     out["k_pt"] = {"workload": "LJ31, 1024 ladders x 16 replicas, 4000 rounds", **summary(r)}

@@ -39,10 +39,12 @@ size_t pt_lds_bytes(int R, int npad) {
   return sizeof(double) * 3 * (size_t)R * npad + sizeof(PtMeta) * R + sizeof(double) * 2 * R + sizeof(int32_t) * 2 * R;
 }
 
-template <int SYS, int KMAX>
+// GC > 0: the lanes per replica fixed at compile time (2 to 8 atoms per lane
+// at 4, 8 or 16 lanes), as for the walker kernels.
+template <int SYS, int KMAX, int GC = 0>
 __global__ __launch_bounds__(512) void k_pt(SetArgs a, PtArgs p, uint64_t rounds, int init) {
   extern __shared__ double lds[];
-  const int R = p.R, G = a.G, ladder = blockIdx.x;
+  const int R = p.R, G = GC ? GC : a.G, ladder = blockIdx.x;
   const int tid = threadIdx.x;
   const int slot = tid / G, lane = tid & (G - 1);
   const bool active = slot < R;
@@ -75,7 +77,7 @@ __global__ __launch_bounds__(512) void k_pt(SetArgs a, PtArgs p, uint64_t rounds
   __syncthreads();
   for (int r = tid; r < R; r += blockDim.x) rep_of[slot_of[r]] = r;
 
-  Engine<SYS, KMAX, 0, M_CAN> e;
+  Engine<SYS, KMAX, 0, M_CAN, 0, GC> e;
   const int64_t w = L0 + slot;
   if (active) {
     e.load_system(a, (int)w, lane, pos + (int64_t)slot * 3 * npad);
@@ -213,6 +215,15 @@ template <int SYS, int KMAX>
 static hipError_t launch_pt_t(const SetArgs &a, const PtArgs &p, int ladders, int threads, uint64_t rounds, int init,
                               hipStream_t s) {
   const size_t lds = pt_lds_bytes(p.R, SYS == SYS_ISING ? 0 : a.npad);
+  if constexpr (SYS != SYS_ISING && KMAX >= 2 && KMAX <= 8) {
+    if (a.G == 4 || a.G == 8 || a.G == 16) {
+      const dim3 grid(ladders), block(threads);
+      if (a.G == 4) hipLaunchKernelGGL((k_pt<SYS, KMAX, 4>), grid, block, lds, s, a, p, rounds, init);
+      if (a.G == 8) hipLaunchKernelGGL((k_pt<SYS, KMAX, 8>), grid, block, lds, s, a, p, rounds, init);
+      if (a.G == 16) hipLaunchKernelGGL((k_pt<SYS, KMAX, 16>), grid, block, lds, s, a, p, rounds, init);
+      return hipGetLastError();
+    }
+  }
   hipLaunchKernelGGL((k_pt<SYS, KMAX>), dim3(ladders), dim3(threads), lds, s, a, p, rounds, init);
   return hipGetLastError();
 }

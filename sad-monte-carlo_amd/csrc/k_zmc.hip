@@ -53,8 +53,8 @@ __device__ __forceinline__ void z_collect(ZMeta &m, double E, double low, double
 
 // The WCA energy of atoms [0, m) in lane-group order: lane l sums the pairs
 // (i, j < i) of its atoms i = l, l + G, ... < m.
-template <int SYS, int KMAX>
-__device__ double wca_energy_first(const Engine<SYS, KMAX, 0, M_CAN> &e, int m, double rc2) {
+template <int SYS, int KMAX, int GC>
+__device__ double wca_energy_first(const Engine<SYS, KMAX, 0, M_CAN, 0, GC> &e, int m, double rc2) {
   double acc = 0.0;
   for (int i = e.lane; i < m; i += e.G)
     for (int j = 0; j < i; j++)
@@ -71,10 +71,12 @@ size_t zmc_lds_bytes(int RMAX, int npad) {
          sizeof(double) * 3 * RMAX + sizeof(int32_t) * 2 * RMAX + sizeof(double) * 40 + sizeof(int32_t) * 8;
 }
 
-template <int SYS, int KMAX>
+// GC > 0: the lanes per replica fixed at compile time (2 to 8 atoms per lane
+// at 4, 8 or 16 lanes), as for the walker kernels.
+template <int SYS, int KMAX, int GC = 0>
 __global__ __launch_bounds__(64) void k_zmc(ZArgs z, SetArgs a, uint64_t rounds) {
   extern __shared__ double lds[];
-  const int RM = z.RMAX, G = a.G, sim = blockIdx.x;
+  const int RM = z.RMAX, G = GC ? GC : a.G, sim = blockIdx.x;
   const int tid = threadIdx.x, nthr = blockDim.x;
   // one wave per run: its groups take slots group, group + per, ... in
   // turn, so only slots holding replicas cost anything
@@ -132,8 +134,9 @@ __global__ __launch_bounds__(64) void k_zmc(ZArgs z, SetArgs a, uint64_t rounds)
   __syncthreads();
   for (int r = tid; r < RM; r += nthr) rep_of[slot_of[r]] = r;
 
-  Engine<SYS, KMAX, 0, M_CAN> e;
-  for (int slot = group; slot < RM; slot += per) e.load_system(a, (int)(S0 + slot), lane, pos + (int64_t)slot * 3 * npad);
+  Engine<SYS, KMAX, 0, M_CAN, 0, GC> e;
+  for (int slot = group; slot < RM; slot += per)
+    e.load_system(a, (int)(S0 + slot), lane, pos + (int64_t)slot * 3 * npad);
   {
     if (SYS == SYS_LJ) {
       const double g = z.geom[sim];
@@ -501,6 +504,14 @@ template <int SYS, int KMAX>
 static hipError_t launch_zmc_t(const ZArgs &z, const SetArgs &a, int sims, int threads, uint64_t rounds,
                                hipStream_t s) {
   const size_t lds = zmc_lds_bytes(z.RMAX, SYS == SYS_ISING ? 0 : a.npad);
+  if constexpr (SYS != SYS_ISING && KMAX >= 2 && KMAX <= 8) {
+    if (a.G == 4 || a.G == 8 || a.G == 16) {
+      if (a.G == 4) hipLaunchKernelGGL((k_zmc<SYS, KMAX, 4>), dim3(sims), dim3(threads), lds, s, z, a, rounds);
+      if (a.G == 8) hipLaunchKernelGGL((k_zmc<SYS, KMAX, 8>), dim3(sims), dim3(threads), lds, s, z, a, rounds);
+      if (a.G == 16) hipLaunchKernelGGL((k_zmc<SYS, KMAX, 16>), dim3(sims), dim3(threads), lds, s, z, a, rounds);
+      return hipGetLastError();
+    }
+  }
   hipLaunchKernelGGL((k_zmc<SYS, KMAX>), dim3(sims), dim3(threads), lds, s, z, a, rounds);
   return hipGetLastError();
 }
