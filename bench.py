@@ -474,12 +474,14 @@ def tempering(rounds, ladders, replicas, lanes, cpu_seconds):
     ctx.pt_run(pt, 20)
     t0 = time.perf_counter()
     ctx.pt_run(pt, rounds)
-    wall = time.perf_counter() - t0
+    host_wall = time.perf_counter() - t0
+    wall = ctx.last_run_ms() / 1e3  # the launch on the device (HIP events)
     reps = ctx.pt_replicas(pt)
     ctx.close()
     moves = ladders * replicas * N_ATOMS * rounds
     swaps = reps["accepted_swaps"].sum() / max(1, (reps["accepted_swaps"] + reps["rejected_swaps"]).sum())
-    out = {"moves_per_s": moves / wall, "wall_s": wall, "ladders": ladders, "replicas": replicas, "lanes": lanes,
+    out = {"moves_per_s": moves / wall, "kernel_s": wall, "host_wall_s": host_wall, "ladders": ladders,
+           "replicas": replicas, "lanes": lanes,
            "rounds": rounds, "swap_acceptance": float(swaps),
            # FP64 pair terms, no bin traffic: the VALU roofline bounds it
            "roofline": {"bound": "valu-fp64", "achieved": FLOP_PER_MOVE * moves / wall / 1e12,
@@ -528,12 +530,14 @@ def binning_path(moves, walkers, lanes, cpu_seconds, linear=False):
     ctx.bin_run(s, WARMUP_MOVES)
     t0 = time.perf_counter()
     ctx.bin_run(s, moves)
-    wall = time.perf_counter() - t0
+    host_wall = time.perf_counter() - t0
+    wall = ctx.last_run_ms() / 1e3  # the launch on the device (HIP events)
     lens = [int(ctx.bin_state(s, w)["len"]) for w in range(0, walkers, max(1, walkers // 64))]
     ctx.close()
     total = walkers * moves
     gbs = BYTES_PER_MOVE * total / wall / 1e9
-    out = {"moves_per_s": total / wall, "wall_s": wall, "walkers": walkers, "lanes": lanes, "moves": moves,
+    out = {"moves_per_s": total / wall, "kernel_s": wall, "host_wall_s": host_wall, "walkers": walkers,
+           "lanes": lanes, "moves": moves,
            "binning": "linear" if linear else "histogram",
            "warmup_moves": WARMUP_MOVES, "mean_bins": float(np.mean(lens)),
            "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -602,11 +606,13 @@ def zmc(rounds, runs, lanes, cpu_seconds):
     m0 = sum(ctx.zmc_state(zid, s)[1] for s in range(runs))
     t0 = time.perf_counter()
     ctx.zmc_run(zid, rounds)
-    wall = time.perf_counter() - t0
+    host_wall = time.perf_counter() - t0
+    wall = ctx.last_run_ms() / 1e3  # the launch on the device (HIP events)
     m1 = sum(ctx.zmc_state(zid, s)[1] for s in range(runs))
     reps = [len(ctx.zmc_replicas(zid, s)) for s in range(runs)]
     ctx.close()
-    out = {"moves_per_s": (m1 - m0) / wall, "wall_s": wall, "runs": runs, "lanes": lanes, "rounds": rounds,
+    out = {"moves_per_s": (m1 - m0) / wall, "kernel_s": wall, "host_wall_s": host_wall, "runs": runs,
+           "lanes": lanes, "rounds": rounds,
            "host_init_s": init_s, "replicas_mean": float(np.mean(reps)), "replicas_max": int(np.max(reps)),
            "roofline": {"bound": "valu-fp64", "achieved": FLOP_PER_MOVE * (m1 - m0) / wall / 1e12,
                         "peak": FP64_VECTOR_PEAK, "unit": "TFLOP/s",

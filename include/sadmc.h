@@ -176,8 +176,10 @@ int sadmc_run(sadmc_ctx *ctx, uint64_t moves);
 int sadmc_run_set(sadmc_ctx *ctx, int32_t set, uint64_t moves);
 /* Wait for the stream; returns the first walker failure as an error code. */
 int sadmc_sync(sadmc_ctx *ctx);
-/* Duration of the last sadmc_run launch in ms (HIP events on the context
- * stream around the kernel). */
+/* Duration in ms of the last launch of sadmc_run, sadmc_run_set,
+ * sadmc_pt_run, sadmc_bin_run or sadmc_zmc_run (HIP events on the context
+ * stream around the kernels; for sadmc_run with several sets, around the
+ * fork and the join of its streams). */
 int sadmc_last_run_ms(const sadmc_ctx *ctx, double *ms);
 
 /* ---- mixed batches (walker sets of different systems and sizes) ------

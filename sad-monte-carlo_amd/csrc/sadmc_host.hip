@@ -1660,7 +1660,10 @@ int sadmc_pt_run(sadmc_ctx *c, int32_t pt, uint64_t rounds) {
   if (!t) return fail(c, SADMC_E_ARG, "no such tempering ladder set");
   if (!rounds) return SADMC_OK;
   HIPCHECK(c, hipSetDevice(c->device));
+  HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
   HIPCHECK(c, launch_pt(t->sys, t->KMAX, t->args, t->pa, t->L, t->threads, rounds, 0, c->stream));
+  HIPCHECK(c, hipEventRecord(c->ev1, c->stream));
+  c->timed = true;
   return SADMC_OK;
 }
 
@@ -1884,7 +1887,12 @@ int sadmc_bin_run(sadmc_ctx *c, int32_t set, uint64_t moves) {
   BinSetHost *b = bset_get(c, set);
   if (!b) return fail(c, SADMC_E_ARG, "no such binning walker set");
   HIPCHECK(c, hipSetDevice(c->device));
-  if (moves) HIPCHECK(c, launch_binning(b->sys, b->KMAX, b->ba, b->args, moves, 0, c->stream));
+  if (moves) {
+    HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
+    HIPCHECK(c, launch_binning(b->sys, b->KMAX, b->ba, b->args, moves, 0, c->stream));
+    HIPCHECK(c, hipEventRecord(c->ev1, c->stream));
+    c->timed = true;
+  }
   int32_t flag = 0;
   HIPCHECK(c, hipMemcpyAsync(&flag, b->d_flag, sizeof flag, hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(c, hipStreamSynchronize(c->stream));
@@ -2270,7 +2278,12 @@ int sadmc_zmc_run(sadmc_ctx *c, int32_t id, uint64_t rounds) {
   ZmcHost *z = zmc_get(c, id);
   if (!z) return fail(c, SADMC_E_ARG, "no such replica set");
   HIPCHECK(c, hipSetDevice(c->device));
-  if (rounds) HIPCHECK(c, launch_zmc(z->sys, z->KMAX, z->za, z->args, z->S, z->threads, rounds, c->stream));
+  if (rounds) {
+    HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
+    HIPCHECK(c, launch_zmc(z->sys, z->KMAX, z->za, z->args, z->S, z->threads, rounds, c->stream));
+    HIPCHECK(c, hipEventRecord(c->ev1, c->stream));
+    c->timed = true;
+  }
   std::vector<ZSim> sim(z->S);
   HIPCHECK(c, hipMemcpyAsync(sim.data(), z->d_sim, sizeof(ZSim) * z->S, hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(c, hipStreamSynchronize(c->stream));
