@@ -39,6 +39,8 @@ def _run(ctx_factory, sp, seeds, geom, chunks, label):
     s = ctx.bin_add_set(sp, seeds, geom=None if sp.system == S.ISING else geom)
     for c in chunks:
         ctx.bin_run(s, c)
+    if chunks and chunks[-1] > 0:
+        assert ctx.last_run_ms() > 0.0  # the last launch, timed by device events
     for w, seed in enumerate(seeds):
         g = 0.0 if sp.system == S.ISING else float(np.broadcast_to(geom, (len(seeds),))[w])
         orc = O.BWalker(oracle_params(sp, seed, g))

@@ -125,6 +125,7 @@ def test_ising_ladders_match_exact_canonical_averages(gpu_ctx_factory):
     rounds = 20000
     pt = ctx.pt_add(sp, temps, np.arange(64, dtype=np.uint64) + 100)
     ctx.pt_run(pt, rounds)
+    assert ctx.last_run_ms() > 0.0  # the launch is timed by device events
     reps = ctx.pt_replicas(pt)
     for r, t in enumerate(temps):
         col = reps[:, r]

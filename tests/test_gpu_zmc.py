@@ -31,6 +31,7 @@ def _compare(ctx, zid, sp, seeds, geom, rounds, chunks, min_T, isbnb, min_replic
     for c in chunks:
         ctx.zmc_run(zid, c)
     ctx.zmc_run(zid, rounds - sum(chunks))
+    assert ctx.last_run_ms() > 0.0  # the last launch, timed by device events
     for s, seed in enumerate(seeds):
         orc = _oracle(sp, seed, geom, min_T, isbnb)
         orc.run(rounds)
