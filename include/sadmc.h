@@ -215,6 +215,17 @@ int sadmc_autotune_batch(int device, int32_t nsets, const sadmc_set_params *ps, 
                          const uint64_t *seeds, const double *geom, uint64_t warm_moves, uint64_t trial_moves,
                          int32_t *lanes_out, int32_t *waves_out, double *rate_out);
 
+/* Diagnostic, no reference counterpart: the reciprocal the FP32 pair terms
+ * use (the hardware reciprocal and one FMA Newton step) against the IEEE
+ * quotient 1.0f / x, for every binary32 bit pattern x in [lo, hi)
+ * (hi <= 2^32), on `device`.  what = 0: *mismatches = the number of x whose
+ * two reciprocals differ bit for bit (NaN results count as equal); 0 over
+ * [2^-126, 2^126).  what = 1: the number of x whose fast pair term is
+ * neither NaN nor bit-identical to the division's; 0 over every
+ * non-negative x, which is what lets the pair loops take the fast sum
+ * unless it is NaN and recompute with the division then. */
+int sadmc_check_recip_f32(int device, int32_t what, uint64_t lo, uint64_t hi, uint64_t *mismatches);
+
 /* ---- single-walker pair for parity tests: MovableSystem::plan_move and
  * ConfirmSystem::confirm (system/mod.rs:101-120).  plan_move consumes the
  * walker's RNG exactly as inside move_once; *some = 0 means None. */

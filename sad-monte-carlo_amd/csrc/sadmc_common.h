@@ -131,6 +131,27 @@ SADMC_HD float lj_pair_f32(float r2) {
   return 4.0f * (s6 - s3);
 }
 
+// 1/x by the hardware reciprocal and one FMA Newton step.  For every x in
+// [2^-126, 2^126) this is the correctly rounded reciprocal, bit-identical to
+// the IEEE quotient 1.0f / x (an exhaustive comparison over all 2^31
+// non-negative binary32 patterns on gfx950, sadmc_check_recip_f32, finds no
+// difference there); below 2^-126 (denormals, zero), from 2^126 up and at
+// +inf it is not (the pair loops that use it handle those).
+SADMC_DF float recip_f32_fast(float x) {
+  const float y = __builtin_amdgcn_rcpf(x);
+  return __builtin_fmaf(y, __builtin_fmaf(-x, y, 1.0f), y);
+}
+
+// lj_pair_f32 with recip_f32_fast: the same value for r2 in [2^-126, 2^126).
+SADMC_DF float lj_pair_f32_fast(float r2) {
+  float s = recip_f32_fast(r2);
+  float s2 = s * s;
+  float s4 = s2 * s2;
+  float s6 = s2 * s4;
+  float s3 = s * s2;
+  return 4.0f * (s6 - s3);
+}
+
 // src/system/wca.rs:66-76.
 SADMC_HD double wca_pair(double r2, double rc2) {
   if (r2 < rc2) {

@@ -612,18 +612,12 @@ struct Engine {
         const int j = lane + G * k;
         const bool use = j < n && j != w;
         if (SYS == SYS_LJ) {
-          double t;
-          if (PREC == 1) {
-            float an = (float)(xk(k) - tx), bn = (float)(yk(k) - ty), cn = (float)(zk(k) - tz);
-            float ao = (float)(xk(k) - fx), bo = (float)(yk(k) - fy), co = (float)(zk(k) - fz);
-            float r2n = an * an + bn * bn + cn * cn;
-            float r2o = ao * ao + bo * bo + co * co;
-            t = (double)lj_pair_f32(r2n) - (double)lj_pair_f32(r2o);
-          } else {
-            t = lj_pair(norm2(xk(k) - tx, yk(k) - ty, zk(k) - tz)) - lj_pair(norm2(xk(k) - fx, yk(k) - fy, zk(k) - fz));
+          if (PREC == 0) {
+            const double t = lj_pair(norm2(xk(k) - tx, yk(k) - ty, zk(k) - tz)) -
+                             lj_pair(norm2(xk(k) - fx, yk(k) - fy, zk(k) - fz));
+            const double na = acc + t;
+            acc = use ? na : acc;
           }
-          const double na = acc + t;
-          acc = use ? na : acc;
         } else {
           double vn = wca_pair(norm2(min_image(xk(k) - tx, L, halfL), min_image(yk(k) - ty, L, halfL),
                                      min_image(zk(k) - tz, L, halfL)),
@@ -637,6 +631,7 @@ struct Engine {
         }
       }
     }
+    if (SYS == SYS_LJ && PREC == 1) acc = lj32_lane_diff(w, tx, ty, tz, fx, fy, fz);
     double e = E + group_sum(acc, G);
     if (SYS == SYS_WCA) pdabse = group_sum(accabs, G);
     pw = w;
@@ -665,25 +660,55 @@ struct Engine {
     for (int c = 0; c < kPropNF; c++) f[c] = group_sum_f32(f[c], G);
   }
 
+  // This lane's sum of the binary32 pair-term differences V(|x_j - t|^2) -
+  // V(|x_j - f|^2) over its atoms j != w (lj.rs:86-105 in FP32 pair terms,
+  // accumulated in FP64), 1/r^2 from recip_f32_fast when FAST, else from the
+  // division.
+  template <bool FAST>
+  SADMC_DF double lj32_lane_sum(int w, double tx, double ty, double tz, double fx, double fy, double fz) {
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < KMAX; k++) {
+      if (k < K) {
+        const int j = lane + G * k;
+        const bool use = j < n && j != w;
+        const float an = (float)(xk(k) - tx), bn = (float)(yk(k) - ty), cn = (float)(zk(k) - tz);
+        const float ao = (float)(xk(k) - fx), bo = (float)(yk(k) - fy), co = (float)(zk(k) - fz);
+        const float r2n = an * an + bn * bn + cn * cn;
+        const float r2o = ao * ao + bo * bo + co * co;
+        const double t = FAST ? (double)lj_pair_f32_fast(r2n) - (double)lj_pair_f32_fast(r2o)
+                              : (double)lj_pair_f32(r2n) - (double)lj_pair_f32(r2o);
+        const double na = acc + t;
+        acc = use ? na : acc;
+      }
+    }
+    return acc;
+  }
+
+  // lj32_lane_sum with the fast reciprocal, equal to the division's sum bit
+  // for bit: the two reciprocals agree on [2^-126, 2^126); below 2^-126
+  // 1/r^2 exceeds 2^126, so s^6 overflows and the pair term is NaN with
+  // either; from 2^126 up s^2 underflows and both terms are +0; at +inf
+  // only the fast one is NaN.  So the fast sum differs only where it is
+  // NaN, and then the lanes still running this code recompute with the
+  // division.
+  SADMC_DF double lj32_lane_diff(int w, double tx, double ty, double tz, double fx, double fy, double fz) {
+    double acc = lj32_lane_sum<true>(w, tx, ty, tz, fx, fy, fz);
+    if (__builtin_expect(__any(acc != acc), 0)) acc = lj32_lane_sum<false>(w, tx, ty, tz, fx, fy, fz);
+    return acc;
+  }
+
   // Sum over the group of the pair-term differences for atom w moving from
   // f to t (lj.rs:86-105).
   SADMC_DF double lj_delta(int w, double tx, double ty, double tz, double fx, double fy, double fz) {
+    if (PREC == 1) return group_sum(lj32_lane_diff(w, tx, ty, tz, fx, fy, fz), G);
     double acc = 0.0;
 #pragma unroll
     for (int k = 0; k < KMAX; k++) {
       if (k < K) {
         int j = lane + G * k;
-        if (j < n && j != w) {
-          if (PREC == 1) {
-            float an = (float)(xk(k) - tx), bn = (float)(yk(k) - ty), cn = (float)(zk(k) - tz);
-            float ao = (float)(xk(k) - fx), bo = (float)(yk(k) - fy), co = (float)(zk(k) - fz);
-            float r2n = an * an + bn * bn + cn * cn;
-            float r2o = ao * ao + bo * bo + co * co;
-            acc += (double)lj_pair_f32(r2n) - (double)lj_pair_f32(r2o);
-          } else {
-            acc += lj_pair(norm2(xk(k) - tx, yk(k) - ty, zk(k) - tz)) - lj_pair(norm2(xk(k) - fx, yk(k) - fy, zk(k) - fz));
-          }
-        }
+        if (j < n && j != w)
+          acc += lj_pair(norm2(xk(k) - tx, yk(k) - ty, zk(k) - tz)) - lj_pair(norm2(xk(k) - fx, yk(k) - fy, zk(k) - fz));
       }
     }
     return group_sum(acc, G);

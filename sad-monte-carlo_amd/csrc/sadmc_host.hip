@@ -739,6 +739,22 @@ int sadmc_run_set(sadmc_ctx *c, int32_t set, uint64_t moves) {
   return SADMC_OK;
 }
 
+int sadmc_check_recip_f32(int device, int32_t what, uint64_t lo, uint64_t hi, uint64_t *mismatches) {
+  if (!mismatches || (what != 0 && what != 1) || lo > hi || hi > 0x100000000ULL) return SADMC_E_ARG;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return SADMC_E_NODEVICE;
+  if (hipSetDevice(device) != hipSuccess) return SADMC_E_HIP;
+  unsigned long long *d_bad = nullptr, bad = 0;
+  if (hipMalloc(&d_bad, sizeof bad) != hipSuccess) return SADMC_E_HIP;
+  hipError_t e = hipMemset(d_bad, 0, sizeof bad);
+  if (e == hipSuccess) e = launch_check_recip(what, lo, hi, d_bad, nullptr);
+  if (e == hipSuccess) e = hipMemcpy(&bad, d_bad, sizeof bad, hipMemcpyDeviceToHost);
+  (void)hipFree(d_bad);
+  if (e != hipSuccess) return SADMC_E_HIP;
+  *mismatches = bad;
+  return SADMC_OK;
+}
+
 int sadmc_sync(sadmc_ctx *c) {
   if (!c) return SADMC_E_ARG;
   HIPCHECK(c, hipStreamSynchronize(c->stream));

@@ -40,6 +40,8 @@ hipError_t launch_batched(const SetArgs *d_sets, const BatchTile *d_tiles, int n
 hipError_t launch_ising(int meth, KernelKind kk, const SetArgs &a, const LaunchArgs &la, hipStream_t s);
 hipError_t launch_lj64(int meth, int kmax, KernelKind kk, const SetArgs &a, const LaunchArgs &la, hipStream_t s);
 hipError_t launch_lj32(int meth, int kmax, KernelKind kk, const SetArgs &a, const LaunchArgs &la, hipStream_t s);
+// The fast reciprocal against the division over binary32 patterns [lo, hi) (k_lj32.hip).
+hipError_t launch_check_recip(int what, uint64_t lo, uint64_t hi, unsigned long long *bad, hipStream_t s);
 // Learned-move LJ kernels (run, plan, probe), FP64 or FP32 pair terms.
 hipError_t launch_learned64(int meth, int kmax, KernelKind kk, const SetArgs &a, const LaunchArgs &la,
                             hipStream_t s);

@@ -135,6 +135,7 @@ EXPORTS = [
     "sadmc_abi_version", "sadmc_create", "sadmc_destroy", "sadmc_last_error", "sadmc_add_set", "sadmc_init",
     "sadmc_run", "sadmc_run_set", "sadmc_sync", "sadmc_last_run_ms",
     "sadmc_compute_energy", "sadmc_verify_energy", "sadmc_system_info", "sadmc_upload_pressure", "sadmc_plan_move", "sadmc_confirm", "sadmc_walker_count",
+    "sadmc_check_recip_f32",
     "sadmc_get_walker_state", "sadmc_walker_bins", "sadmc_walker_coords", "sadmc_walker_spins",
     "sadmc_download_raw", "sadmc_upload_raw", "sadmc_init_coords", "sadmc_init_spins", "sadmc_bin_window",
     "sadmc_run_batched", "sadmc_graph_run", "sadmc_autotune_lanes", "sadmc_set_block_waves",
@@ -173,6 +174,7 @@ def lib():
     L.sadmc_compute_energy.argtypes = [vp, i32, i32, C.POINTER(d)]
     L.sadmc_verify_energy.argtypes = [vp, i32, i32, C.POINTER(d), C.POINTER(i32)]
     L.sadmc_system_info.argtypes = [C.POINTER(SetParams), d, C.POINTER(SystemInfo)]
+    L.sadmc_check_recip_f32.argtypes = [C.c_int, i32, C.c_uint64, C.c_uint64, C.POINTER(C.c_uint64)]
     L.sadmc_upload_pressure.argtypes = [vp, i32, vp, vp]
     L.sadmc_last_run_ms.argtypes = [vp, C.POINTER(d)]
     L.sadmc_plan_move.argtypes = [vp, i32, i32, d, C.POINTER(d), C.POINTER(i32)]
@@ -644,6 +646,18 @@ def autotune_batch(sets, warm_moves=20000, trial_moves=2000, device=0):
     if rc:
         raise SadmcError(rc, "autotune_batch")
     return [int(x) for x in lanes], waves.value, rate.value
+
+
+def check_recip_f32(lo, hi, what=0, device=0):
+    """what = 0: how many binary32 patterns in [lo, hi) get a different 1/x
+    from the FP32 pair terms' reciprocal than from the IEEE division (0 over
+    [2^-126, 2^126)); what = 1: how many get a fast pair term that is neither
+    NaN nor equal to the division's (0 over all non-negative patterns)."""
+    out = C.c_uint64()
+    rc = lib().sadmc_check_recip_f32(device, what, C.c_uint64(lo), C.c_uint64(hi), C.byref(out))
+    if rc:
+        raise SadmcError(rc, "check_recip_f32")
+    return int(out.value)
 
 
 def system_info(params, geom=0.0):
