@@ -183,7 +183,7 @@ struct Engine {
   int32_t pw;
   double ptx, pty, ptz, pe, pdabse;
   // learned move: weights and this walker's exchange row in LDS, ln H
-  const float *PW;
+  const uint16_t *PW;
   float *XC;
   double lnh;
   // this move's SAD gamma, computed while the proposed bin's record loads
@@ -708,7 +708,8 @@ struct Engine {
       if (k < K) {
         int j = lane + G * k;
         if (j < n && j != w)
-          acc += lj_pair(norm2(xk(k) - tx, yk(k) - ty, zk(k) - tz)) - lj_pair(norm2(xk(k) - fx, yk(k) - fy, zk(k) - fz));
+          acc += lj_pair(norm2(xk(k) - tx, yk(k) - ty, zk(k) - tz)) -
+                 lj_pair(norm2(xk(k) - fx, yk(k) - fy, zk(k) - fz));
       }
     }
     return group_sum(acc, G);
@@ -721,14 +722,17 @@ struct Engine {
     const double fx = lx[w], fy = ly[w], fz = lz[w];
     float f[kPropNF], xi[3];
     features(fx, fy, fz, w, f);
-    const float s = mlp_forward(PW, XC, f, lane, G);
+    const float s = mlp_eval(PW, XC, f, lane, G);
     gaussian3(s0, s1, xi);
     const float sig = expf(s);
     const double tx = fx + (double)(sig * xi[0]), ty = fy + (double)(sig * xi[1]), tz = fz + (double)(sig * xi[2]);
     const double r2to = norm2(tx, ty, tz), r2from = norm2(fx, fy, fz);
-    if (r2to > R2 && r2to > r2from) return false;
+    // the wall's verdict waits until after the second evaluation, so that
+    // the whole wave reaches it together (its result is then unused)
+    const bool outside = r2to > R2 && r2to > r2from;
     features(tx, ty, tz, w, f);
-    const float s2 = mlp_forward(PW, XC, f, lane, G);
+    const float s2 = mlp_eval(PW, XC, f, lane, G);
+    if (outside) return false;
     const double e = E + lj_delta(w, tx, ty, tz, fx, fy, fz);
     lnh = prop_lnh(s, s2, xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2]);
     pw = w;

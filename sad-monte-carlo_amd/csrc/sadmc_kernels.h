@@ -299,8 +299,9 @@ static hipError_t launch_km(int meth, int kmax, KernelKind kk, const SetArgs &a,
 }
 
 // ---------------------------------------------------------- learned move
-// LDS: the positions of the 64/G walkers, then the network weights as float,
-// then 32 floats per walker for the hidden-layer exchange.
+// LDS: the positions of the 64/G walkers, then the network weights as BF16
+// (in the first half of a region of kPropLds floats), then 32 floats per
+// walker for the hidden-layer exchange of the non-matrix-core form.
 // Two waves per SIMD: the register budget the 16384-walker sets need.  GC
 // as in k_run (the SAD kernels of 2 to 8 atoms per lane at 4, 8 or 16 lanes
 // have it fixed at compile time: fewer scalar spills, +10 % for LJ38 at 8).
@@ -310,13 +311,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
   extern __shared__ double lds[];
   const int G = GC ? GC : a.G, per = 64 / G;
   float *lw = (float *)(lds + per * 3 * a.npad);
-  load_weights(a.mlp, lw);
+  load_weights_bf16(a.mlp, (uint16_t *)lw);
   const int group = threadIdx.x / G;
   const int lane = threadIdx.x & (G - 1);
   const int walker = blockIdx.x * per + group;
   if (walker >= a.W) return;
   Engine<SYS_LJ, KMAX, PREC, METH, 1, GC> e;
-  e.PW = lw;
+  e.PW = (const uint16_t *)lw;
   e.XC = lw + kPropLds + group * kPropH;
   e.load(a, walker, lane, lds + group * 3 * a.npad);
   e.run(moves);
@@ -327,10 +328,10 @@ template <int KMAX, int PREC, int METH>
 __global__ __launch_bounds__(64) void k_plan_learned(SetArgs a, int walker, Pending *out) {
   extern __shared__ double lds[];
   float *lw = (float *)(lds + 3 * a.npad);
-  load_weights(a.mlp, lw);
+  load_weights_bf16(a.mlp, (uint16_t *)lw);
   if ((int)threadIdx.x >= a.G) return;
   Engine<SYS_LJ, KMAX, PREC, METH, 1> e;
-  e.PW = lw;
+  e.PW = (const uint16_t *)lw;
   e.XC = lw + kPropLds;
   e.load(a, walker, threadIdx.x, lds);
   double e2 = 0.0;

@@ -64,46 +64,6 @@ __device__ __forceinline__ void feature_add(float r2, float *f) {
   }
 }
 
-// Forward pass for one walker whose G lanes all hold the features f; lane l
-// computes hidden units l, l+G, ... and the group exchanges the first layer
-// through `xc` (32 floats of LDS per walker).  W: the weights in LDS.
-__device__ __forceinline__ float mlp_forward(const float *W, float *xc, const float *f, int lane, int G) {
-  for (int u = lane; u < kPropH; u += G) {
-    const float4 *w1 = (const float4 *)(W + kW1 + u * kPropNF);
-    const float4 a = w1[0], b = w1[1];
-    float z = W[kB1 + u];
-    z += a.x * f[0];
-    z += a.y * f[1];
-    z += a.z * f[2];
-    z += a.w * f[3];
-    z += b.x * f[4];
-    z += b.y * f[5];
-    z += b.z * f[6];
-    z += b.w * f[7];
-    xc[u] = z > 0.0f ? z : 0.0f;
-  }
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  const float4 *x4 = (const float4 *)xc;
-  float part = 0.0f;
-  for (int u = lane; u < kPropH; u += G) {
-    const float4 *w2 = (const float4 *)(W + kW2 + u * kPropH);
-    float z = W[kB2 + u];
-#pragma unroll
-    for (int k = 0; k < kPropH / 4; k++) {
-      const float4 w = w2[k], h = x4[k];
-      z += w.x * h.x;
-      z += w.y * h.y;
-      z += w.z * h.z;
-      z += w.w * h.w;
-    }
-    const float h = z > 0.0f ? z : 0.0f;
-    part += W[kW3 + u] * h;
-  }
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  float s = W[kB3] + group_sum_f32(part, G);
-  return s < kPropSMin ? kPropSMin : (s > kPropSMax ? kPropSMax : s);
-}
-
 // Three standard normal draws by Box-Muller from four uniforms, with the
 // hardware log2, square root and sin/cos (whose argument is in revolutions,
 // so cos(2 pi u) is one instruction).  The Hastings term is computed from the
@@ -128,6 +88,142 @@ SADMC_DF double prop_lnh(float s, float s_new, float xi2) {
 __device__ __forceinline__ void load_weights(const uint16_t *w, float *lw) {
   for (int i = threadIdx.x; i < kPropNP; i += 64) lw[i] = bf16_to_f32(w[i]);
   __syncthreads();
+}
+
+// Copy the BF16 weights into LDS as they are (kPropLds halfwords, the
+// device array's padded length, as 32-bit words; all lanes of a 64-lane
+// workgroup).
+__device__ __forceinline__ void load_weights_bf16(const uint16_t *w, uint16_t *lw) {
+  const uint32_t *src = (const uint32_t *)w;
+  uint32_t *dst = (uint32_t *)lw;
+  for (int i = threadIdx.x; i < kPropLds / 2; i += 64) dst[i] = src[i];
+  __syncthreads();
+}
+
+// The two BF16 values of a 32-bit word as float: the one at the lower
+// address (low half) and the other.
+__device__ __forceinline__ float bf16_lo(uint32_t v) { return __builtin_bit_cast(float, v << 16); }
+__device__ __forceinline__ float bf16_hi(uint32_t v) { return __builtin_bit_cast(float, v & 0xFFFF0000u); }
+
+// Forward pass for one walker whose G lanes all hold the features f; lane l
+// computes hidden units l, l+G, ... and the group exchanges the first layer
+// through `xc` (32 floats of LDS per walker).  W: the BF16 weights in LDS,
+// widened to float in registers (exact), FP32 arithmetic.
+__device__ __forceinline__ float mlp_forward_bf16(const uint16_t *W, float *xc, const float *f, int lane, int G) {
+  for (int u = lane; u < kPropH; u += G) {
+    const uint4 a = *(const uint4 *)(W + kW1 + u * kPropNF);
+    float z = bf16_to_f32(W[kB1 + u]);
+    z += bf16_lo(a.x) * f[0];
+    z += bf16_hi(a.x) * f[1];
+    z += bf16_lo(a.y) * f[2];
+    z += bf16_hi(a.y) * f[3];
+    z += bf16_lo(a.z) * f[4];
+    z += bf16_hi(a.z) * f[5];
+    z += bf16_lo(a.w) * f[6];
+    z += bf16_hi(a.w) * f[7];
+    xc[u] = z > 0.0f ? z : 0.0f;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  const float4 *x4 = (const float4 *)xc;
+  float part = 0.0f;
+  for (int u = lane; u < kPropH; u += G) {
+    const uint4 *w2 = (const uint4 *)(W + kW2 + u * kPropH);
+    float z = bf16_to_f32(W[kB2 + u]);
+#pragma unroll
+    for (int k = 0; k < kPropH / 8; k++) {
+      const uint4 w = w2[k];
+      const float4 h0 = x4[2 * k], h1 = x4[2 * k + 1];
+      z += bf16_lo(w.x) * h0.x;
+      z += bf16_hi(w.x) * h0.y;
+      z += bf16_lo(w.y) * h0.z;
+      z += bf16_hi(w.y) * h0.w;
+      z += bf16_lo(w.z) * h1.x;
+      z += bf16_hi(w.z) * h1.y;
+      z += bf16_lo(w.w) * h1.z;
+      z += bf16_hi(w.w) * h1.w;
+    }
+    const float h = z > 0.0f ? z : 0.0f;
+    part += bf16_to_f32(W[kW3 + u]) * h;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  float s = bf16_to_f32(W[kB3]) + group_sum_f32(part, G);
+  return s < kPropSMin ? kPropSMin : (s > kPropSMax ? kPropSMax : s);
+}
+
+// The network on the matrix cores, for a wave whose 64 lanes all run this
+// call and whose walkers have G >= 4 lanes each (at most 16 walkers: the
+// columns of 16x16x4 FP32 products; v_mfma_f32_16x16x4f32 takes A[i][k] from
+// lane i + 16k, B[k][j] from lane j + 16k and leaves D[4q + e][j] in element
+// e of lane j + 16q).  Layer 1 is H1 = W1 X^T (two row blocks of 16 units, two
+// steps of 4 features), layer 2 H2 = W2 H1 (the hidden layer stays where
+// layer 1 left it: step (m, e) takes unit 16m + 4q + e from element e of
+// lane j + 16q), then s = b3 + w3 . relu(H2 + b2) summed over the four lanes
+// of a column.  FP32 products and sums as in mlp_forward_bf16, in another
+// order.
+// W: the BF16 weights in LDS; f: the walker's features in every lane of its
+// group; l: the lane in the wave.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float mlp_wave(const uint16_t *W, const float *f, int l, int G) {
+  const int i = l & 15, q = l >> 4, j = l & 15;
+  const int g = l & (G - 1);
+  const bool col = j < 64 / G;  // column j is a walker of this wave
+  const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+  // both row blocks' products side by side (independent accumulators)
+  f32x4 acc0 = zero, acc1 = zero;
+#pragma unroll
+  for (int st = 0; st < 2; st++) {
+    // feature 4 st + q of walker j, from the lane of its group that holds
+    // it at in-group index (4 st + q) mod G
+    const float mine = G >= 8 ? f[g & 7] : f[(4 * st + g) & 7];
+    const float got = __shfl(mine, G * j + ((4 * st + q) & (G - 1)), 64);
+    const float b = col ? got : 0.0f;
+    const float a0 = bf16_to_f32(W[kW1 + i * kPropNF + 4 * st + q]);
+    const float a1 = bf16_to_f32(W[kW1 + (16 + i) * kPropNF + 4 * st + q]);
+    acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b, acc0, 0, 0, 0);
+    acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b, acc1, 0, 0, 0);
+  }
+  float h1[2][4];
+#pragma unroll
+  for (int e = 0; e < 4; e++) {
+    const float z0 = acc0[e] + bf16_to_f32(W[kB1 + 4 * q + e]);
+    const float z1 = acc1[e] + bf16_to_f32(W[kB1 + 16 + 4 * q + e]);
+    h1[0][e] = z0 > 0.0f ? z0 : 0.0f;
+    h1[1][e] = z1 > 0.0f ? z1 : 0.0f;
+  }
+  f32x4 d0 = zero, d1 = zero;
+#pragma unroll
+  for (int m = 0; m < 2; m++) {
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const float a0 = bf16_to_f32(W[kW2 + i * kPropH + 16 * m + 4 * q + e]);
+      const float a1 = bf16_to_f32(W[kW2 + (16 + i) * kPropH + 16 * m + 4 * q + e]);
+      d0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, h1[m][e], d0, 0, 0, 0);
+      d1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, h1[m][e], d1, 0, 0, 0);
+    }
+  }
+  float part = 0.0f;
+#pragma unroll
+  for (int mo = 0; mo < 2; mo++) {
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const int u = 16 * mo + 4 * q + e;
+      const float z = (mo == 0 ? d0[e] : d1[e]) + bf16_to_f32(W[kB2 + u]);
+      part += bf16_to_f32(W[kW3 + u]) * (z > 0.0f ? z : 0.0f);
+    }
+  }
+  part += __shfl_xor(part, 16, 64);
+  part += __shfl_xor(part, 32, 64);
+  float s = bf16_to_f32(W[kB3]) + part;
+  s = s < kPropSMin ? kPropSMin : (s > kPropSMax ? kPropSMax : s);
+  // walker l / G reads its column
+  return __shfl(s, l / G, 64);
+}
+
+// The network for the walker of this lane's group: on the matrix cores when
+// every lane of the wave is here and G >= 4, else mlp_forward_bf16.
+__device__ __forceinline__ float mlp_eval(const uint16_t *W, float *xc, const float *f, int lane, int G) {
+  if (G >= 4 && __builtin_amdgcn_read_exec() == ~0ull) return mlp_wave(W, f, (int)(threadIdx.x & 63), G);
+  return mlp_forward_bf16(W, xc, f, lane, G);
 }
 
 }  // namespace sadmc

@@ -7,12 +7,15 @@
   for the learned-proposal workload (LJ38, sphere 3.2, min_T 0.03) at 8 and
   16 lanes per walker;
 - the persistent learned walker over 2000 SAD moves against the oracle's,
-  move by move, and the workload's full 16384-walker launch;
+  move by move (its waves full, so the network runs on the matrix cores;
+  single moves take the vector-unit form), and the workload's full
+  16384-walker launch;
 - probe records from the same walkers and streams;
 - the training gradient and loss on the same records and weights;
 - Adam's first step and a falling training loss;
-- detailed balance on the device: at infinite temperature two atoms must
-  end uniform in the sphere (closed-form pair-distance law);
+- detailed balance on the device, with either form of the network: at
+  infinite temperature two atoms must end uniform in the sphere
+  (closed-form pair-distance law);
 - what the trained network gains over the fixed step on held-out records.
 """
 import numpy as np
@@ -318,12 +321,15 @@ def test_adam_first_step_and_training_reduces_loss(gpu_ctx_factory):
     assert losses[-50:].mean() < losses[:10].mean() - 0.02
 
 
-def test_learned_walkers_sample_uniform_sphere(gpu_ctx_factory):
-    print("case learned detailed balance", flush=True)
+@pytest.mark.parametrize("lanes", [2, 4])
+def test_learned_walkers_sample_uniform_sphere(gpu_ctx_factory, lanes):
+    """At 2 lanes the network runs on the vector units, at 4 (every wave
+    full: 4096 walkers) on the matrix cores."""
+    print(f"case learned detailed balance, {lanes} lanes", flush=True)
     R = 1.5
     ctx = gpu_ctx_factory()
     ctx.prop_set_weights(_hand_weights())
-    sp = S.make_params(system=S.LJ, n=2, lanes=2, method=S.CANONICAL, method_param=1e300, energy_bin=1e9,
+    sp = S.make_params(system=S.LJ, n=2, lanes=lanes, method=S.CANONICAL, method_param=1e300, energy_bin=1e9,
                        has_max_e=1, max_e=1e12, move_kind=S.MOVE_LEARNED)
     W = 4096
     start = np.tile(np.array([0.5, -0.5, 0.0, 0.0, 0.0, 0.0]), W)
