@@ -164,9 +164,9 @@ __device__ __forceinline__ float mlp_forward_bf16(const uint16_t *W, float *xc, 
 // group; l: the lane in the wave.
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ float mlp_wave(const uint16_t *W, const float *f, int l, int G) {
-  const int i = l & 15, q = l >> 4, j = l & 15;
+  const int i = l & 15, q = l >> 4, j = l & 15;  // A row / B and D column, step
   const int g = l & (G - 1);
-  const bool col = j < 64 / G;  // column j is a walker of this wave
+  const bool col = j < 64 / G;  // column j is a walker of this wave (else its source lane is wrapped and unused)
   const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
   // both row blocks' products side by side (independent accumulators)
   f32x4 acc0 = zero, acc1 = zero;
@@ -175,7 +175,7 @@ __device__ __forceinline__ float mlp_wave(const uint16_t *W, const float *f, int
     // feature 4 st + q of walker j, from the lane of its group that holds
     // it at in-group index (4 st + q) mod G
     const float mine = G >= 8 ? f[g & 7] : f[(4 * st + g) & 7];
-    const float got = __shfl(mine, G * j + ((4 * st + q) & (G - 1)), 64);
+    const float got = __shfl(mine, (G * j + ((4 * st + q) & (G - 1))) & 63, 64);
     const float b = col ? got : 0.0f;
     const float a0 = bf16_to_f32(W[kW1 + i * kPropNF + 4 * st + q]);
     const float a1 = bf16_to_f32(W[kW1 + (16 + i) * kPropNF + 4 * st + q]);
