@@ -702,6 +702,30 @@ def entropy(moves, walkers, lanes, repeats):
     return out
 
 
+def learned_rate(moves, walkers, lanes):
+    """Cost of the learned move: the learned-proposal workload (LJ38 in a
+    3.2 sigma sphere, SAD min_T 0.03, bins 0.01, max energy 0, FP64 pair
+    terms) with an untrained network (prop_init(7)), learned moves against
+    cube moves on the same walkers: one untimed launch of 2e5 moves, then
+    `moves` moves timed by HIP events."""
+    out = {}
+    for name, kind in (("cube", S.MOVE_CUBE), ("learned", S.MOVE_LEARNED)):
+        sp = S.make_params(system=S.LJ, n=38, lanes=lanes, method=S.SAD, method_param=0.03, move_param=0.1,
+                           energy_bin=0.01, has_max_e=1, max_e=0.0, move_kind=kind)
+        ctx = S.Context(0)
+        sid = ctx.add_set(sp, np.arange(walkers, dtype=np.uint64), geom=3.2)
+        ctx.prop_init(7)
+        ctx.init()
+        ctx.run(200000)
+        ctx.run(moves)
+        ms = ctx.last_run_ms()
+        ctx.close()
+        out[name] = walkers * moves / (ms / 1e3)
+        log(f"[learned rate] {name}: {out[name] / 1e9:.3f} G moves/s ({ms:.1f} ms)")
+    out["learned_over_cube_cost"] = out["cube"] / out["learned"]
+    return out
+
+
 def other_paths(walkers, lanes):
     """Short runs of the engine's other kernels on the same box as the
     headline line, each on its own workload (the `--batched`, `--binning`,
@@ -767,6 +791,8 @@ def main():
     ap.add_argument("--learned-cv-ref", default=None,
                     help="with --learned-cv: the plain curves from a --lj31-cv output instead of a plain run")
     ap.add_argument("--learned-cv-chunk", type=float, default=1e6, help="with --learned-cv: moves per checkpoint")
+    ap.add_argument("--learned-rate", type=int, default=0,
+                    help="only time this many learned and cube moves of the learned-proposal workload")
     ap.add_argument("--batched", type=int, default=0,
                     help="only run the mixed LJ/WCA batch (this many moves per launch)")
     ap.add_argument("--binning", type=int, default=0,
@@ -793,6 +819,12 @@ def main():
         sys.exit(2)
     if a.copy_probe:
         copy_probe(a.copy_probe)
+        return
+    if a.learned_rate > 0:
+        res = learned_rate(a.learned_rate, a.walkers, a.lanes)
+        print(json.dumps({"metric": "LJ38 learned-proposal moves/s on 1 GPU", "unit": "moves/s",
+                          "higher_is_better": True, "value": res["learned"], "walkers": a.walkers,
+                          "lanes": a.lanes, "moves": a.learned_rate, **res}))
         return
     if a.batched > 0:
         res = batched(a.batched, 4096, a.steps)
