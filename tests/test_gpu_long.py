@@ -110,3 +110,16 @@ def test_batched_bench_composition_1e6_moves(gpu_ctx_factory):
         assert st["status"] == 0 and st["moves"] == moves, label
         compare_walker(st, ctx.bins(sid, w), orc, sp.method, label=label)
         np.testing.assert_array_equal(ctx.coords(sid, w), orc.coords(), err_msg=label)
+
+
+def test_wca64_fp64_1e7_moves(gpu_ctx_factory):
+    """A WCA fluid of the mixed batch's classes on its own: N = 64 at packing
+    fraction 0.4 (periodic box), cube moves 0.05, SAD min_T 0.1, bins 0.05,
+    max energy 2.5 N, FP64 refresh every 1024 moves, 8 lanes; 64 walkers for
+    1e7 moves, 4 of them against the oracle."""
+    print("case long wca64 fp64 64 walkers", flush=True)
+    sp = S.make_params(system=S.WCA, n=64, lanes=8, method=S.SAD, method_param=0.1, move_param=0.05,
+                       energy_bin=0.05, has_max_e=1, max_e=160.0, refresh=1024)
+    W = 64
+    box = float(np.cbrt(64 / (6.0 * 0.4 / np.pi)))
+    _run_long(gpu_ctx_factory, sp, W, box, np.array([0, 21, 42, 63]), "wca64 fp64", launches=10)
