@@ -123,3 +123,19 @@ def test_wca64_fp64_1e7_moves(gpu_ctx_factory):
     W = 64
     box = float(np.cbrt(64 / (6.0 * 0.4 / np.pi)))
     _run_long(gpu_ctx_factory, sp, W, box, np.array([0, 21, 42, 63]), "wca64 fp64", launches=10)
+
+
+@pytest.mark.parametrize("kind", ["histogram", "linear"])
+def test_binning_walker_lj31_2e6_moves(gpu_ctx_factory, kind):
+    """The `binning` binary's walker on the LJ31 configuration (SAD min_T 0.05,
+    bins 0.01, max energy 0, 8 lanes, FP64) for 2e6 moves in four launches:
+    past the histogram's growth into the phase where the BinCounts trackers
+    carry the statistics; 8 walkers against the oracle, every scalar,
+    statistic and bin."""
+    from test_gpu_binning import _run
+
+    print(f"case long binning lj31 {kind}", flush=True)
+    sp = S.make_params(system=S.LJ, n=31, lanes=8, method=S.SAD, method_param=0.05, move_param=0.1,
+                       energy_bin=0.01, has_max_e=1, max_e=0.0,
+                       binning=S.BIN_HISTOGRAM if kind == "histogram" else S.BIN_LINEAR)
+    _run(gpu_ctx_factory, sp, np.arange(8, dtype=np.uint64) + 40, 3.0, [500_000] * 4, f"binning {kind}")
