@@ -1277,7 +1277,13 @@ static void move_once(or_mc *m) {
   m->moves += 1;
   if (m->sys == OR_LJ || m->sys == OR_WCA) {
     if (m->p.refresh > 0) {
-      if (m->moves % (uint64_t)m->p.refresh == 0) m->E = sys_compute_energy(m, m->G);
+      /* the FP64 refresh of the FP32-pair energy: its drift can carry the
+         energy past the edge of the bins, which then grow to it as for a
+         proposed energy (energy.rs:400-434) */
+      if (m->moves % (uint64_t)m->p.refresh == 0) {
+        m->E = sys_compute_energy(m, m->G);
+        prepare_for_state(m, m->E);
+      }
     } else {
       uint64_t L = (uint64_t)m->len;
       if (m->moves % (L * L * 1000ULL) == 0) sys_verify_energy(m);

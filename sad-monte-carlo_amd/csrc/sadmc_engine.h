@@ -468,11 +468,12 @@ struct Engine {
     }
   }
 
-  // energy.rs:400-434.  New slots get have_visited = true (visit_epoch =
+  // energy.rs:400-434 (prepare_for_state) for every energy in [lo, hi]: the
+  // bins grow to both ends.  New slots get have_visited = true (visit_epoch =
   // epoch) and round_trips = 1; every other field is already zero.
-  SADMC_DF bool prepare_for_state(double e) {
+  SADMC_DF bool prepare_for_range(double lo, double hi) {
     bool front = false, grew = false;
-    while (e < bmin) {
+    while (lo < bmin) {
       if (base == 0) {
         status = SADMC_WS_CAPACITY;
         return false;
@@ -484,7 +485,7 @@ struct Engine {
       bmin -= args().width;
       front = true;
     }
-    while (e >= bmin + args().width * (double)len) {
+    while (hi >= bmin + args().width * (double)len) {
       int32_t q = base + len;
       if (q >= args().cap) {
         status = SADMC_WS_CAPACITY;
@@ -1217,6 +1218,7 @@ struct Engine {
 
   // energy.rs:904-974.
   SADMC_DF void move_once() {
+    bool regrow = false;
     moves += 1;
     tm += 1.0;
     gdirty = true;
@@ -1225,11 +1227,16 @@ struct Engine {
         E = full_energy();
         next_check += (uint64_t)args().refresh;
         // the refreshed energy may lie in another bin than the tracked one
-        // (move_once indexes the bins by the energy itself, energy.rs:904-974);
-        // the previous move's pending sums belong to the bin it left
-        if (e_index(E) != cur_i) {
+        // (move_once indexes the bins by the energy itself, energy.rs:904-974),
+        // or past the window's edge by the FP32 drift: the window then grows
+        // to it as for a proposed energy (energy.rs:400-434), together with
+        // the proposal's growth below (the order of the two growths does not
+        // change the bins they add); the previous move's pending sums belong
+        // to the bin it left
+        if (e_index(E) != cur_i || E < bmin) {
           apply_pending();
           refresh_indices();
+          regrow = cur_p < 0 || E < bmin;
         }
       } else {
         verify_energy();
@@ -1254,52 +1261,61 @@ struct Engine {
     double e2;
     const bool some = plan_move(tscale, usc, e2);
     apply_pending();
+    bool go = false;
     if (some) {
       bool oob = false;
       if (args().has_max_e) oob = e2 > args().max_e && e2 > e1;
       if (args().has_min_e) oob = oob || (e2 < args().min_e && e2 < e1);
-      if (!oob) {
-        if (!prepare_for_state(e2)) return;
-        i1 = cur_i;
-        int32_t i2 = e_index(e2);
-        if (!indices_valid() || !idx_ok(i2)) {
-          status = SADMC_WS_INDEX;
-          return;
-        }
-        const int32_t p2 = base + i2;
-        double w2;
-        uint64_t h2;
-        if (p2 == cur_p) {
-          w2 = c_lnw;
-          h2 = c_hist;
-        } else {
-          HotBin hb = H[p2];
-          w2 = hb.lnw;
-          h2 = hb.hist;
-        }
-        // independent of the record being loaded
-        if (METH == M_SAD) {
-          gpre = sad_gamma();
-          gdirty = false;
-        }
-        spec_draw();
-        if (!reject_move(e1, e2, w2, h2)) {
-          accepted += 1;
-          acc_rate += rs;
-          confirm();
-          int32_t i = (E == e2) ? i2 : e_index(E);
-          if (!idx_ok(i)) return;
-          const int32_t p = base + i;
-          if (p != cur_p) {
-            if (p == p2) {
-              set_current(p, w2, h2);
-            } else {
-              HotBin hb = H[p];
-              set_current(p, hb.lnw, hb.hist);
-            }
+      go = !oob;
+    }
+    if (go || regrow) {
+      double lo = go ? e2 : E, hi = lo;
+      if (regrow) {
+        lo = E < lo ? E : lo;
+        hi = E > hi ? E : hi;
+      }
+      if (!prepare_for_range(lo, hi)) return;
+    }
+    if (go) {
+      i1 = cur_i;
+      int32_t i2 = e_index(e2);
+      if (!indices_valid() || !idx_ok(i2)) {
+        status = SADMC_WS_INDEX;
+        return;
+      }
+      const int32_t p2 = base + i2;
+      double w2;
+      uint64_t h2;
+      if (p2 == cur_p) {
+        w2 = c_lnw;
+        h2 = c_hist;
+      } else {
+        HotBin hb = H[p2];
+        w2 = hb.lnw;
+        h2 = hb.hist;
+      }
+      // independent of the record being loaded
+      if (METH == M_SAD) {
+        gpre = sad_gamma();
+        gdirty = false;
+      }
+      spec_draw();
+      if (!reject_move(e1, e2, w2, h2)) {
+        accepted += 1;
+        acc_rate += rs;
+        confirm();
+        int32_t i = (E == e2) ? i2 : e_index(E);
+        if (!idx_ok(i)) return;
+        const int32_t p = base + i;
+        if (p != cur_p) {
+          if (p == p2) {
+            set_current(p, w2, h2);
+          } else {
+            HotBin hb = H[p];
+            set_current(p, hb.lnw, hb.hist);
           }
-          cur_i = i;
         }
+        cur_i = i;
       }
     }
     if (status) return;

@@ -120,3 +120,25 @@ def test_fp32_pairs_drift_stays_bounded(gpu_ctx_factory):
             worst = max(worst, abs(st[w]["E"] - e_ref) / abs(e_ref))
     print("max relative drift", worst)
     assert worst < 1e-4
+
+
+def test_fp32_refresh_past_the_window_edge(gpu_ctx_factory):
+    """LJ31 walker 83142 at 4 lanes: the FP64 refresh at move 2048 puts the
+    energy one bin above the window it had grown to; the bins grow to it and
+    the walker goes on, bit-identical to the oracle (with its neighbours)."""
+    print("case fp32 refresh past the window edge", flush=True)
+    sp = S.make_params(system=S.LJ, n=31, lanes=4, method=S.SAD, method_param=0.05, move_param=0.1,
+                       energy_bin=0.01, has_max_e=1, max_e=0.0, precision=S.FP32_PAIRS, refresh=1024)
+    ctx = gpu_ctx_factory()
+    seeds = np.array([83141, 83142, 83143], dtype=np.uint64)
+    sid = ctx.add_set(sp, seeds, geom=3.0)
+    ctx.init()
+    ctx.run(2047)
+    before = int(ctx.state(sid, 1)["len"])
+    ctx.run(3000 - 2047)
+    assert int(ctx.state(sid, 1)["len"]) > before
+    for w, seed in enumerate(seeds):
+        orc = O.Walker(oracle_params(sp, seed, 3.0))
+        orc.run(3000)
+        compare_walker(ctx.state(sid, w), ctx.bins(sid, w), orc, sp.method, label=f"lj31 fp32 g4 walker {seed}")
+        np.testing.assert_array_equal(ctx.coords(sid, w), orc.coords())

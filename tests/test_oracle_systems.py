@@ -139,6 +139,27 @@ def test_lj_fp32_pairs_error_reported():
     assert abs(w.scalars()["E"] - O.lj_energy_ld(w.coords())) <= 1e-4 * abs(w.scalars()["E"])
 
 
+def test_lj_fp32_refresh_past_the_window_edge_grows_the_bins():
+    """LJ31 walker 83142 (4 lanes) drifts to an FP32-summed energy inside the
+    window's top bin at move 2047 whose FP64 recompute lies above that bin;
+    the refresh at move 2048 grows the bins to the refreshed energy, as a
+    proposed energy would, and the walker goes on."""
+    p = O.make_params(system=O.LJ, n=31, radius=3.0, method=O.SAD, method_param=0.05, move_param=0.1,
+                      energy_bin=0.01, has_max_e=1, max_e=0.0, precision=1, refresh=1024, order_g=4, seed=83142)
+    w = O.Walker(p)
+    assert w.run(2047) == 0
+    s = w.scalars()
+    top = s["bins_min"] + s["bins_width"] * len(w.bins()["lnw"])
+    e64 = O.lj_energy_ld(w.coords())
+    assert s["E"] < top <= e64 < 0.0
+    n0 = len(w.bins()["lnw"])
+    assert w.run(1) == 0
+    s = w.scalars()
+    assert s["status"] == 0 and len(w.bins()["lnw"]) == n0 + 1
+    assert s["bins_min"] + s["bins_width"] * (n0 + 1) > e64
+    assert w.run(3000) == 0 and w.scalars()["status"] == 0
+
+
 def test_lj_orders_agree():
     for seed in range(3):
         c = O.lj_init(38, 3.2, seed)
