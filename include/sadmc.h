@@ -81,7 +81,8 @@ typedef struct {
   double min_e;
   double max_e;
   double rc2;            /* WCA cutoff squared (2^(1/3) sigma^2) */
-  int32_t refresh;       /* 0: reference error-estimate refresh; >0: FP64 refresh every `refresh` moves */
+  int32_t refresh;       /* 0: reference error-estimate refresh; >0: FP64 refresh every `refresh` moves
+                            (a refreshed energy past the bins' edge grows them, energy.rs:400-434) */
   int32_t ising_init;    /* 0: reference spins (seed 10137, ising.rs:45); 1: per-walker seed */
   int32_t collect_pressure; /* WCA: virial pressure every N^2 moves (wca.rs:202-218) */
   int32_t lanes;         /* lanes per walker for LJ/WCA (power of two 2..64); 0 = auto */
