@@ -142,3 +142,38 @@ def test_fp32_refresh_past_the_window_edge(gpu_ctx_factory):
         orc.run(3000)
         compare_walker(ctx.state(sid, w), ctx.bins(sid, w), orc, sp.method, label=f"lj31 fp32 g4 walker {seed}")
         np.testing.assert_array_equal(ctx.coords(sid, w), orc.coords())
+
+
+def test_fp32_refresh_below_the_window_grows_the_front(gpu_ctx_factory):
+    """Walkers at move 1000 given the positions of their own states at move
+    200000, below every bin they have: the refresh at move 1024 grows the
+    bins at the front (hundreds of bins) on the device as in the oracle,
+    bit for bit through move 3000."""
+    print("case fp32 refresh below the window", flush=True)
+    sp = S.make_params(system=S.LJ, n=31, lanes=8, method=S.SAD, method_param=0.05, move_param=0.1,
+                       energy_bin=0.01, has_max_e=1, max_e=0.0, precision=S.FP32_PAIRS, refresh=1024)
+    seeds = np.array([4, 5], dtype=np.uint64)
+    deep = gpu_ctx_factory()
+    sd = deep.add_set(sp, seeds, geom=3.0)
+    deep.init()
+    deep.run(200000)
+    ctx = gpu_ctx_factory()
+    sid = ctx.add_set(sp, seeds, geom=3.0)
+    ctx.init()
+    ctx.run(1000)
+    st, bins = ctx.download_raw(sid)
+    coords = np.concatenate([deep.coords(sd, w) for w in range(len(seeds))])
+    ctx.upload_raw(sid, st, bins, coords=coords)
+    ctx.run(24)
+    for w in range(len(seeds)):
+        assert float(ctx.state(sid, w)["bins_min"]) < float(st[w]["bins_min"])
+    ctx.run(3000 - 1024)
+    for w, seed in enumerate(seeds):
+        d = O.Walker(oracle_params(sp, seed, 3.0))
+        d.run(200000)
+        orc = O.Walker(oracle_params(sp, seed, 3.0))
+        orc.run(1000)
+        orc.load_state(orc.scalars(), orc.bins(), d.coords())
+        orc.run(2000)
+        compare_walker(ctx.state(sid, w), ctx.bins(sid, w), orc, sp.method, label=f"lj31 fp32 transplant {seed}")
+        np.testing.assert_array_equal(ctx.coords(sid, w), orc.coords())

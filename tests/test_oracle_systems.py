@@ -160,6 +160,31 @@ def test_lj_fp32_refresh_past_the_window_edge_grows_the_bins():
     assert w.run(3000) == 0 and w.scalars()["status"] == 0
 
 
+@pytest.mark.parametrize("seed", [4, 5])
+def test_lj_fp32_refresh_below_the_window_grows_the_front(seed):
+    """A walker at move 1000 given the positions of its own state at move
+    200000, which lie below every bin it has: the refresh at move 1024
+    recomputes the energy from those positions and the bins grow at the
+    front to reach it; the walker goes on."""
+    p = O.make_params(system=O.LJ, n=31, radius=3.0, method=O.SAD, method_param=0.05, move_param=0.1,
+                      energy_bin=0.01, has_max_e=1, max_e=0.0, precision=1, refresh=1024, order_g=8, seed=seed)
+    deep = O.Walker(p)
+    deep.run(200000)
+    w = O.Walker(p)
+    w.run(1000)
+    s0 = w.scalars()
+    assert O.lj_energy_ld(deep.coords()) < s0["bins_min"]
+    n0 = len(w.bins()["lnw"])
+    w.load_state(s0, w.bins(), deep.coords())
+    assert w.run(24) == 0
+    s = w.scalars()
+    assert s["status"] == 0 and s["bins_min"] <= s["E"] < s0["bins_min"]
+    # refreshed at the start of move 1024; that move's FP32 difference follows
+    assert abs(s["E"] - O.lj_energy_ld(w.coords())) <= 1e-6 * abs(s["E"])
+    assert len(w.bins()["lnw"]) >= n0 + round((s0["bins_min"] - s["bins_min"]) / 0.01) > n0
+    assert w.run(2000) == 0 and w.scalars()["status"] == 0
+
+
 def test_lj_orders_agree():
     for seed in range(3):
         c = O.lj_init(38, 3.2, seed)
