@@ -704,20 +704,24 @@ int sadmc_run(sadmc_ctx *c, uint64_t moves) {
     HIPCHECK(c, hipEventRecord(c->ev_fork, c->stream));
     for (int k = 0; k < ns; k++) HIPCHECK(c, hipStreamWaitEvent(c->side[k], c->ev_fork, 0));
   }
-  for (int i = 0; i < nsets; i++) {
+  hipError_t le = hipSuccess;
+  for (int i = 0; i < nsets && le == hipSuccess; i++) {
     SetHost &s = c->sets[i];
     LaunchArgs la{};
     la.moves = moves;
     la.waves = s.waves;
     hipStream_t st = ns > 1 ? c->side[i % ns] : c->stream;
-    HIPCHECK(c, launch(s.p.system, s.p.precision, s.p.method, s.KMAX, KK_RUN, s.args, la, st));
+    le = launch(s.p.system, s.p.precision, s.p.method, s.KMAX, KK_RUN, s.args, la, st);
   }
   if (ns > 1) {
+    // joined even after a failed launch, so that the context's stream still
+    // orders everything already queued on the side streams
     for (int k = 0; k < ns; k++) {
       HIPCHECK(c, hipEventRecord(c->ev_join[k], c->side[k]));
       HIPCHECK(c, hipStreamWaitEvent(c->stream, c->ev_join[k], 0));
     }
   }
+  if (le != hipSuccess) return fail(c, SADMC_E_HIP, std::string("k_run launch: ") + hipGetErrorString(le));
   HIPCHECK(c, hipEventRecord(c->ev1, c->stream));
   c->timed = true;
   return SADMC_OK;
