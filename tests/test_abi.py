@@ -131,3 +131,18 @@ def test_system_info_matches_the_reference_trait_constants():
     assert w["max_size"] == np.sqrt(48.0)
     with pytest.raises(S.SadmcError):
         S.system_info(S.make_params(system=S.ISING, n=9))
+
+
+def test_rust_binding_declares_every_function_with_its_arity():
+    """The Rust extern block a maintainer would add names
+    every function of include/sadmc.h with the same number of arguments."""
+    doc = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    block = doc[doc.index('extern "C" {'):]
+    block = block[:block.index("\n}\n")]
+    rust = {m.group(1): m.group(2) for m in re.finditer(r"pub fn (sadmc_\w+)\((.*?)\)\s*(?:->|;)", block, re.S)}
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    c = {m.group(1): m.group(2) for m in re.finditer(r"\b(sadmc_\w+)\s*\(([^;{]*?)\)\s*;", text, re.S)}
+    assert set(rust) == set(c), (sorted(set(c) - set(rust)), sorted(set(rust) - set(c)))
+    arity = lambda args: 0 if args.strip() in ("", "void") else args.count(",") + 1
+    for name in c:
+        assert arity(rust[name]) == arity(c[name]), name
