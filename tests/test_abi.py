@@ -146,3 +146,39 @@ def test_rust_binding_declares_every_function_with_its_arity():
     arity = lambda args: 0 if args.strip() in ("", "void") else args.count(",") + 1
     for name in c:
         assert arity(rust[name]) == arity(c[name]), name
+
+
+def _c_struct_types(name):
+    """Field types of a header struct in order, arrays expanded, nested
+    structs by name."""
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    body = {m.group(2): m.group(1) for m in re.finditer(r"typedef struct(?: \w+)? \{(.*?)\}\s*(\w+);", text, re.S)}[name]
+    out = []
+    for decl in (d.strip() for d in body.split(";") if d.strip()):
+        ty, rest = decl.split(None, 1)
+        for v in (x.strip() for x in rest.split(",")):
+            m = re.match(r"\w+(?:\[(\d+)\])?$", v)
+            out += [ty] * int(m.group(1) or 1)
+    return out
+
+
+def _rust_struct_types(doc, name):
+    body = re.search(r"pub struct " + name + r" \{(.*?)\n\}", doc, re.S).group(1)
+    body = re.sub(r"//[^\n]*", "", body)
+    out = []
+    for ty in re.findall(r"pub \w+: ([^,]+?)\s*(?:,|$)", body):
+        m = re.match(r"\[(\w+); (\d+)\]$", ty)
+        out += [m.group(1)] * int(m.group(2)) if m else [ty]
+    return out
+
+
+def test_rust_structs_mirror_the_header():
+    """Each #[repr(C)] struct of the Rust binding has its C struct's field
+    types in the same order (so the same layout)."""
+    doc = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    to_rust = {"int32_t": "i32", "int64_t": "i64", "uint64_t": "u64", "double": "f64", "sadmc_bcounts": "BCounts"}
+    pairs = {"SetParams": "sadmc_set_params", "WalkerState": "sadmc_walker_state", "Bin": "sadmc_bin",
+             "SystemInfo": "sadmc_system_info_t", "BCounts": "sadmc_bcounts", "BWalkerState": "sadmc_bwalker_state",
+             "BBin": "sadmc_bbin", "HiRes": "sadmc_hires", "ZReplica": "sadmc_zreplica", "Replica": "sadmc_replica"}
+    for rust, c in pairs.items():
+        assert _rust_struct_types(doc, rust) == [to_rust[t] for t in _c_struct_types(c)], rust
