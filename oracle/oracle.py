@@ -69,7 +69,9 @@ def lib():
     global _LIB
     if _LIB is not None:
         return _LIB
-    path = os.path.join(_HERE, "liboracle.so")
+    # SADMC_ORACLE_LIB: another build of the same sources (the sanitizer
+    # build of the Makefile's `asan` target)
+    path = os.environ.get("SADMC_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
     if not os.path.exists(path):
         build()
     L = C.CDLL(path)

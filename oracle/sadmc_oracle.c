@@ -2870,17 +2870,19 @@ void or_zmc_run(or_zmc *z, uint64_t rounds) {
         double median_below = zmed_median(z);
         z->med[0] = median_below;
         z->med_len = 1;
+        /* the new replica is built from a copy taken before the arrays may
+           move (r points into z->r) */
+        or_zrep nr = *r;
         if (z->R == z->cap) {
           z->cap *= 2;
           z->sys = (or_mc **)realloc(z->sys, sizeof(or_mc *) * z->cap);
           z->r = (or_zrep *)realloc(z->r, sizeof(or_zrep) * z->cap);
         }
-        or_zrep nr = *r;
-        nr.max_energy = r->cutoff_energy;
+        nr.max_energy = nr.cutoff_energy;
         nr.cutoff_energy = median_below;
         zrep_decimate(&nr);
         nr.lowest_max_energy = -INFINITY;
-        nr.translation_scale = r->translation_scale * pow(0.5, 1.0 / (double)zmc_dim(z->sys[z->R - 1]));
+        nr.translation_scale = nr.translation_scale * pow(0.5, 1.0 / (double)zmc_dim(z->sys[z->R - 1]));
         or_rng_jump(nr.rng);
         z->sys[z->R] = mc_clone(z->sys[z->R - 1]);
         z->r[z->R] = nr;
