@@ -21,8 +21,20 @@ CFLAGS = [ARCH, "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Xclang", "-
           "-packed-fp32-ops", "-Wno-unused-command-line-argument"]
 
 
-def build(verbose=False, jobs=None):
+# host_asan: the host side of every unit under AddressSanitizer, device code
+# untouched, linked without the runtime; load it with clang's ASan runtime
+# preloaded and SADMC_LIB pointing at it:
+#   python sad-monte-carlo_amd/build_lib.py --host-asan
+#   SADMC_LIB=sad-monte-carlo_amd/libsadmc_amd_hostasan.so ASAN_OPTIONS=detect_leaks=0 \
+#   LD_PRELOAD=$(/opt/rocm/llvm/bin/clang -print-file-name=libclang_rt.asan-x86_64.so) \
+#   python -m pytest tests -m "not gpu" -p no:xdist
+HOST_ASAN = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fno-omit-frame-pointer", "-Xarch_host", "-g"]
+OUT_HOST_ASAN = os.path.join(HERE, "libsadmc_amd_hostasan.so")
+
+
+def build(verbose=False, jobs=None, host_asan=False):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    out_path = OUT_HOST_ASAN if host_asan else OUT
     tmpdir = tempfile.mkdtemp(prefix="sadmc_build_")
     objs, procs = [], []
     jobs = jobs or min(len(UNITS), max(1, (os.cpu_count() or 4)))
@@ -32,7 +44,7 @@ def build(verbose=False, jobs=None):
         while pending and len(procs) < jobs:
             u = pending.pop(0)
             obj = os.path.join(tmpdir, u.replace(".hip", ".o"))
-            cmd = [hipcc] + CFLAGS + ["-c", "-o", obj, os.path.join(CSRC, u)]
+            cmd = [hipcc] + CFLAGS + (HOST_ASAN if host_asan else []) + ["-c", "-o", obj, os.path.join(CSRC, u)]
             procs.append((u, obj, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
         u, obj, p = procs.pop(0)
         out, _ = p.communicate()
@@ -45,17 +57,17 @@ def build(verbose=False, jobs=None):
         for u, out in failed:
             sys.stderr.write(f"--- {u}\n{out}\n")
         raise RuntimeError("hipcc failed: " + ", ".join(u for u, _ in failed))
-    tmp = OUT + ".tmp"
+    tmp = out_path + ".tmp"
     r = subprocess.run([hipcc, ARCH, "-shared", "-fPIC", "-o", tmp] + objs, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("link failed")
-    os.replace(tmp, OUT)
+    os.replace(tmp, out_path)
     for o in objs:
         os.remove(o)
     os.rmdir(tmpdir)
-    return OUT
+    return out_path
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    print(build(verbose="-v" in sys.argv, host_asan="--host-asan" in sys.argv))

@@ -13,7 +13,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libsadmc_amd.so")
+# SADMC_LIB: another build of the same sources (build_lib.py --host-asan)
+LIB_PATH = os.environ.get("SADMC_LIB") or os.path.join(_HERE, "libsadmc_amd.so")
 _LIB = None
 
 ISING, LJ, WCA = 0, 1, 2
