@@ -139,6 +139,27 @@ def test_lj_fp32_pairs_error_reported():
     assert abs(w.scalars()["E"] - O.lj_energy_ld(w.coords())) <= 1e-4 * abs(w.scalars()["E"])
 
 
+def test_lj_fp32_pair_term_is_binary32():
+    """The precision variant's pair term is computed in binary32 (numpy's
+    float32 arithmetic, no extended precision or contraction), and lies
+    within its rounding bound of the binary64 term."""
+    rng = np.random.default_rng(7)
+    r2 = rng.uniform(0.6, 9.0, 20000).astype(np.float32)
+    got = np.array([O.lib().or_lj_pair_f32(float(x)) for x in r2], np.float32)
+    s = np.float32(1.0) / r2
+    s2 = s * s
+    s4 = s2 * s2
+    s6 = s2 * s4
+    s3 = s * s2
+    want = np.float32(4.0) * (s6 - s3)
+    np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32))
+    s64 = 1.0 / r2.astype(np.float64)
+    exact = 4.0 * (s64 ** 6 - s64 ** 3)
+    bound = 4.0 * (s64 ** 6 + s64 ** 3) * 2.0 ** -20
+    assert np.all(np.abs(got.astype(np.float64) - exact) <= bound)
+    assert np.count_nonzero(got.astype(np.float64) != exact) > len(r2) // 2
+
+
 def test_lj_fp32_refresh_past_the_window_edge_grows_the_bins():
     """LJ31 walker 83142 (4 lanes) drifts to an FP32-summed energy inside the
     window's top bin at move 2047 whose FP64 recompute lies above that bin;
