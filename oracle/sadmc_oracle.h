@@ -22,7 +22,10 @@
  * the published jump polynomial; Ising against brute force and the exact
  * 8x8 density of states; LJ against closed-form potentials and the known
  * cluster minima; SAD gamma against its closed form; the SAD/WL/SAMC
- * bookkeeping against hand-built scripted-energy trajectories.  The rand 0.7
+ * bookkeeping against hand-built scripted-energy trajectories; the
+ * precision variant's pair term against numpy's binary32 arithmetic and its
+ * rounding bound, its refreshes against long-double recomputes, and the bin
+ * growth a refresh past the bins' edge causes.  The rand 0.7
  * integer/float adapters (gen_range, Uniform) are restated from the rand
  * 0.7.3 sources as recalled -- parity unpinned against the Rust crate,
  * which cannot be built here.
